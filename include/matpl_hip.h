@@ -1,0 +1,143 @@
+/*
+ * matpl_hip.h — C ABI of the MI355X (gfx950) descriptor-and-force library
+ * (libmatpl_hip.so).
+ *
+ * Every entry point takes plain device pointers and sizes, enqueues its work
+ * on `stream` (a hipStream_t passed as void*, NULL = default stream) and
+ * returns a status code: 0 on success, MATPL_ERR_* otherwise, with a message
+ * in matpl_last_error(). Nothing here synchronizes the stream or allocates
+ * device memory; entry points that need scratch take a caller workspace
+ * sized by the matching *_workspace_* query.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the MatPL source tree).
+ */
+#ifndef MATPL_HIP_H
+#define MATPL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MATPL_OK 0
+#define MATPL_ERR_ARGUMENT -1  /* bad size, null or misaligned pointer */
+#define MATPL_ERR_HIP -2       /* a HIP runtime call or launch failed */
+#define MATPL_ERR_OVERFLOW -3  /* a neighbor bucket overflowed */
+
+const char *matpl_last_error(void);
+int matpl_abi_version(void);
+
+/* ------------------------------------------------------------------ DP ops */
+/*
+ * calculateForce: force[b][0:N] = F_center[b]; force[b][j] += dE.Ri_d for
+ * every slot with nlist >= 0.  nlist [B][N][S] int32 0-based (-1 empty),
+ * dE [B][N][S*4], Ri_d [B][N][S*4][3], F_center [B][N][3],
+ * force out [B][N+nghost][3]. dE and Ri_d 16-byte aligned.
+ * Replaces CalculateForceFuncs::forward (src/op/src/CalcOps.cpp:6-23) ->
+ * torch_launch_calculate_force (src/op/src/calculate_force.cpp:4-41) ->
+ * force_calc (src/op/kernel/calculateForce.cu:49-142).
+ */
+int matpl_dp_force_f32(const int32_t *nlist, const float *dE, const float *Ri_d, const float *F_center,
+                       int64_t B, int64_t N, int64_t S, int64_t nghost, float *force, void *stream);
+int matpl_dp_force_f64(const int32_t *nlist, const double *dE, const double *Ri_d, const double *F_center,
+                       int64_t B, int64_t N, int64_t S, int64_t nghost, double *force, void *stream);
+
+/*
+ * calculateVirial: atom_virial[b][j][3a+c] = sum over slots pointing at j of
+ * (dE.Ri_d)_a * Rij_c; virial[b][v] = sum_j atom_virial[b][j][v].
+ * Rij [B][N][S][3]; workspace of matpl_dp_virial_workspace_elems(B,N,nghost)
+ * elements of the same type.
+ * Replaces CalculateVirialFuncs::forward (src/op/src/CalcOps.cpp:170-186) ->
+ * torch_launch_calculate_virial_force (src/op/src/calculate_virial_force.cpp:4-48)
+ * -> virial_deriv_wrt_neighbors_a + atom_virial_reduction
+ * (src/op/kernel/calculateVirial.cu:18-120).
+ */
+int matpl_dp_virial_workspace_elems(int64_t B, int64_t N, int64_t nghost);
+int matpl_dp_virial_f32(const int32_t *nlist, const float *dE, const float *Rij, const float *Ri_d, int64_t B,
+                        int64_t N, int64_t S, int64_t nghost, float *virial, float *atom_virial, float *workspace,
+                        int64_t workspace_elems, void *stream);
+int matpl_dp_virial_f64(const int32_t *nlist, const double *dE, const double *Rij, const double *Ri_d, int64_t B,
+                        int64_t N, int64_t S, int64_t nghost, double *virial, double *atom_virial,
+                        double *workspace, int64_t workspace_elems, void *stream);
+
+/*
+ * Backward of calculateForce wrt dE: g[b][i][s][w] = -gF[b][i].Ri_d[s][w]
+ * + gF[b][j mod N].Ri_d[s][w] (j >= 0). gF [B][N+nghost][3], g [B][N][S*4].
+ * Replaces CalculateForceFuncs::backward (src/op/src/CalcOps.cpp:25-39) ->
+ * force_grad_wrt_center_atom / force_grad_wrt_neighbors_a
+ * (src/op/kernel/calculateForceGrad.cu:9-90).
+ */
+int matpl_dp_force_grad_f32(const int32_t *nlist, const float *Ri_d, const float *gF, int64_t B, int64_t N,
+                            int64_t S, int64_t nghost, float *g, void *stream);
+int matpl_dp_force_grad_f64(const int32_t *nlist, const double *Ri_d, const double *gF, int64_t B, int64_t N,
+                            int64_t S, int64_t nghost, double *g, void *stream);
+
+/*
+ * Backward of calculateVirial wrt dE: g[b][i][s][w] = sum_{a,c} gV[b][3a+c]
+ * Rij[s][c] Ri_d[s][w][a] (0 on empty slots). gV [B][9].
+ * Replaces CalculateVirialFuncs::backward (src/op/src/CalcOps.cpp:188-202) ->
+ * virial_grad_wrt_neighbors_a (src/op/kernel/calculateVirialGrad.cu:15-92).
+ */
+int matpl_dp_virial_grad_f32(const int32_t *nlist, const float *Rij, const float *Ri_d, const float *gV,
+                             int64_t B, int64_t N, int64_t S, float *g, void *stream);
+int matpl_dp_virial_grad_f64(const int32_t *nlist, const double *Rij, const double *Ri_d, const double *gV,
+                             int64_t B, int64_t N, int64_t S, double *g, void *stream);
+
+/* ---------------------------------------------------------- DP neighbors */
+/*
+ * Neighbor lists in the find_neighbore layout: list [Ntot][T][m] int32
+ * (1-based index within the frame, 0 empty), dR [Ntot][T][m][3] float64
+ * (r_j - r_i; may be NULL), counts [Ntot][T] (may be NULL), shift
+ * [Ntot][T][m] packed image (may be NULL), frac_wrapped [Ntot][3] out.
+ * status (device int32, zero before the call) receives 1 + the first atom
+ * whose bucket overflowed m. Replaces neighconst.find_neighbore
+ * (src/pre_data/gen_feature/NeighConst.f90:14-149) as called by
+ * find_neighbore (src/pre_data/dpuni_data_loader.py:595-650).
+ *
+ * _brute: any number of frames (lattice [F][9] device, frame_start [F+1]
+ * int64 device, atom_frame [Ntot] int32 device), any box size.
+ * _cell : one frame whose lattice vectors are all >= 2*rc_m (lattice host
+ * [9]); workspace of matpl_dp_neighbor_cell_workspace_bytes bytes.
+ */
+int matpl_dp_neighbor_is_small_box(const double *lattice_host, double rc_m);
+int64_t matpl_dp_neighbor_cell_workspace_bytes(const double *lattice_host, double rc_m, int64_t n);
+int matpl_dp_neighbor_brute(const double *lattice, const double *frac, const int64_t *frame_start,
+                            const int32_t *atom_frame, const int32_t *type_map, const double *rc_type, int ntypes,
+                            int m, double rc_m, int64_t nframes, int64_t ntot, double *frac_wrapped, int32_t *list,
+                            double *dR, int32_t *counts, int32_t *shift, int32_t *status, void *stream);
+int matpl_dp_neighbor_cell(const double *lattice_host, const double *frac, const int32_t *type_map,
+                           const double *rc_type, int ntypes, int m, double rc_m, int64_t n, double *frac_wrapped,
+                           int32_t *list, double *dR, int32_t *counts, int32_t *shift, int32_t *status,
+                           void *workspace, int64_t workspace_bytes, void *stream);
+
+/* ------------------------------------------------- DP environment matrix */
+/*
+ * Ri [B][N][S][4] and Ri_d [B][N][S][4][3] from ImageDR [B][N][S][4]
+ * = (r, dx, dy, dz); type_map [N]; davg/dstd [T][S*4].
+ * Replaces DP.calculate_Ri (src/model/dp_dp.py:217-302), which the
+ * reference computes with torch tensor code.
+ */
+int matpl_dp_env_mat_f32(const float *image_dr, const int32_t *type_map, const float *davg, const float *dstd,
+                         int64_t B, int64_t N, int64_t S, int ntypes, double rmin, double rmax, float *Ri,
+                         float *Ri_d, void *stream);
+int matpl_dp_env_mat_f64(const double *image_dr, const int32_t *type_map, const double *davg, const double *dstd,
+                         int64_t B, int64_t N, int64_t S, int ntypes, double rmin, double rmax, double *Ri,
+                         double *Ri_d, void *stream);
+
+/*
+ * davg/dstd [T][4] from list [N][S] (1-based) and dR [N][S][3].
+ * Replaces calc_stat + smooth + compute_std
+ * (src/pre_data/dpuni_data_loader.py:422-560).
+ */
+int64_t matpl_dp_calc_stat_workspace_elems(int ntypes);
+int matpl_dp_calc_stat(const int32_t *list, const double *dR, const int32_t *type_map, int64_t N, int64_t S,
+                       int ntypes, double rmin, double rmax, double *davg, double *dstd, double *workspace,
+                       int64_t workspace_elems, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MATPL_HIP_H */

@@ -1,0 +1,79 @@
+"""Loads libmatpl_hip.so (the C-ABI library) and checks every call.
+
+The library is built in-tree by ``python -m matpl_amd.build``. There is no
+CPU fallback: if the library is missing or a call fails, this raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmatpl_hip.so")
+_LIB = None
+
+# (name, restype, argtypes) for every exported symbol; must match include/matpl_hip.h
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+I32 = ctypes.c_int
+F64 = ctypes.c_double
+SIGNATURES = {
+    "matpl_last_error": (ctypes.c_char_p, []),
+    "matpl_abi_version": (I32, []),
+    # DP operator contract
+    "matpl_dp_force_f32": (I32, [P, P, P, P, I64, I64, I64, I64, P, P]),
+    "matpl_dp_force_f64": (I32, [P, P, P, P, I64, I64, I64, I64, P, P]),
+    "matpl_dp_virial_workspace_elems": (I32, [I64, I64, I64]),
+    "matpl_dp_virial_f32": (I32, [P, P, P, P, I64, I64, I64, I64, P, P, P, I64, P]),
+    "matpl_dp_virial_f64": (I32, [P, P, P, P, I64, I64, I64, I64, P, P, P, I64, P]),
+    "matpl_dp_force_grad_f32": (I32, [P, P, P, I64, I64, I64, I64, P, P]),
+    "matpl_dp_force_grad_f64": (I32, [P, P, P, I64, I64, I64, I64, P, P]),
+    "matpl_dp_virial_grad_f32": (I32, [P, P, P, P, I64, I64, I64, P, P]),
+    "matpl_dp_virial_grad_f64": (I32, [P, P, P, P, I64, I64, I64, P, P]),
+    # DP neighbor search
+    "matpl_dp_neighbor_is_small_box": (I32, [P, F64]),
+    "matpl_dp_neighbor_cell_workspace_bytes": (I64, [P, F64, I64]),
+    "matpl_dp_neighbor_brute": (I32, [P, P, P, P, P, P, I32, I32, F64, I64, I64, P, P, P, P, P, P, P]),
+    "matpl_dp_neighbor_cell": (I32, [P, P, P, P, I32, I32, F64, I64, P, P, P, P, P, P, P, I64, P]),
+    # DP environment matrix and statistics
+    "matpl_dp_env_mat_f32": (I32, [P, P, P, P, I64, I64, I64, I32, F64, F64, P, P, P]),
+    "matpl_dp_env_mat_f64": (I32, [P, P, P, P, I64, I64, I64, I32, F64, F64, P, P, P]),
+    "matpl_dp_calc_stat_workspace_elems": (I64, [I32]),
+    "matpl_dp_calc_stat": (I32, [P, P, P, I64, I64, I32, F64, F64, P, P, P, I64, P]),
+}
+
+
+class MatplError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise MatplError(f"{LIB_PATH} is missing; build it with `python -m matpl_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return list(SIGNATURES)
+
+
+def call(name: str, *args) -> int:
+    """Calls an entry point returning a status code; raises on failure."""
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        msg = lib().matpl_last_error().decode(errors="replace")
+        raise MatplError(f"{name} failed ({rc}): {msg}")
+    return rc
+
+
+def query(name: str, *args):
+    """Calls an entry point that returns a value (sizes, flags)."""
+    return getattr(lib(), name)(*args)

@@ -1,0 +1,453 @@
+// DP neighbor search on gfx950 (the NeighConst.find_neighbore contract,
+// src/pre_data/gen_feature/NeighConst.f90:14-149).
+//
+// Output layout is the reference's, per frame-packed atom i:
+//   list  [Ntot][T][m] int32, 1-based neighbor index within the frame, 0 = empty
+//   dR    [Ntot][T][m][3] float64 = r_j - r_i (Cartesian)
+// plus, for the fused path and MD,
+//   counts[Ntot][T] int32 stored neighbors per bucket
+//   shift [Ntot][T][m] int32, packed integer image (s1,s2,s3), each +128 in a byte,
+//         such that dR = (f_j - f_i + s) . lattice with f the wrapped fractions.
+//
+// Membership and dR follow the reference bit for bit: positions are wrapped
+// with fmod(x+10,1); a frame is "small" when any lattice vector is shorter than
+// 2*rc_m, in which case every image in [-nperiod, nperiod]^3 of every atom is
+// tested in (j, i1, i2, i3) order; otherwise each fractional difference takes
+// the nearest of d, d+1, d-1. dR and |dR|^2 are evaluated in FP64 in the
+// reference's operand order with FMA contraction disabled in this file.
+// Buckets list neighbors in the reference's order (ascending j, then image).
+//
+// Two kernels:
+//   brute  : one wavefront per atom, j in 64-wide chunks, integer wave scans to
+//            place hits; any number of frames (batched training data).
+//   cell   : one large frame; atoms binned into a fractional cell grid with a
+//            counting sort, each wavefront scans the <=27 neighbor cells,
+//            appends hits to per-type LDS buckets, then ranks them by j.
+#include "common.h"
+
+#pragma clang fp contract(off)
+
+namespace matpl {
+namespace {
+
+constexpr int WAVE = 64;
+
+struct Lattice {
+    double a[9];   // row i = lattice vector i
+};
+
+__device__ __forceinline__ int pack_shift(int s1, int s2, int s3) {
+    return (s1 + 128) | ((s2 + 128) << 8) | ((s3 + 128) << 16);
+}
+
+__device__ __forceinline__ void frac_to_cart(const double *L, double dx1, double dx2, double dx3,
+                                             double &dx, double &dy, double &dz) {
+    dx = L[0] * dx1 + L[3] * dx2 + L[6] * dx3;
+    dy = L[1] * dx1 + L[4] * dx2 + L[7] * dx3;
+    dz = L[2] * dx1 + L[5] * dx2 + L[8] * dx3;
+}
+
+// Nearest-of {d, d+1, d-1} as NeighConst.f90:98-108; returns the shift taken.
+__device__ __forceinline__ int near_image(double &d) {
+    int s = 0;
+    if (fabs(d + 1.0) < fabs(d)) { d = d + 1.0; s += 1; }
+    if (fabs(d - 1.0) < fabs(d)) { d = d - 1.0; s -= 1; }
+    return s;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        int u = __shfl_up(v, off, WAVE);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+__global__ void wrap_kernel(const double *__restrict__ frac, double *__restrict__ frac_w, int64_t n3) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3;
+         i += (int64_t)gridDim.x * blockDim.x)
+        frac_w[i] = fmod(frac[i] + 10.0, 1.0);
+}
+
+// ----------------------------------------------------------------- brute force
+// One wavefront per atom. Frame geometry is read from lattice[frame*9].
+__global__ void __launch_bounds__(256)
+brute_kernel(const double *__restrict__ lattice, const double *__restrict__ fw,
+             const int64_t *__restrict__ frame_start, const int32_t *__restrict__ atom_frame,
+             const int32_t *__restrict__ type_map, const double *__restrict__ rc_type, int T, int m,
+             double rc_m, int64_t ntot, int32_t *__restrict__ list, double *__restrict__ dR,
+             int32_t *__restrict__ counts, int32_t *__restrict__ shift, int32_t *__restrict__ status) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t i = blockIdx.x * (int64_t)(blockDim.x / WAVE) + (threadIdx.x / WAVE);
+    if (i >= ntot) return;
+    const int f = atom_frame[i];
+    const int64_t j0 = frame_start[f], j1 = frame_start[f + 1];
+    const double *L = lattice + 9 * (int64_t)f;
+    int np[3];
+    bool small_box = false;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double len = sqrt(L[3 * d] * L[3 * d] + L[3 * d + 1] * L[3 * d + 1] + L[3 * d + 2] * L[3 * d + 2]);
+        np[d] = (int)(rc_m / len) + 1;
+        if (len < 2.0 * rc_m) small_box = true;
+    }
+    if (!small_box) np[0] = np[1] = np[2] = 0;
+    const double rc2 = rc_m * rc_m;
+    const double rci = rc_type[type_map[i]];
+    const double rci2 = rci * rci;
+    const double fi0 = fw[3 * i], fi1 = fw[3 * i + 1], fi2 = fw[3 * i + 2];
+    const int64_t S = (int64_t)T * m;
+    int32_t *li = list + i * S;
+    double *di = dR ? dR + i * S * 3 : nullptr;
+    int32_t *si = shift ? shift + i * S : nullptr;
+
+    // running stored / counted (rc_m) totals per type; T <= 16
+    int base[16], baseM[16];
+    for (int t = 0; t < T; t++) base[t] = baseM[t] = 0;
+    bool overflow = false;
+
+    for (int64_t jc = j0; jc < j1; jc += WAVE) {
+        const int64_t j = jc + lane;
+        const bool valid = j < j1;
+        const int tj = valid ? type_map[j] : 0;
+        double fj0 = 0, fj1 = 0, fj2 = 0;
+        if (valid) { fj0 = fw[3 * j]; fj1 = fw[3 * j + 1]; fj2 = fw[3 * j + 2]; }
+        // pass 1: counts for this lane's j
+        int c = 0, cM = 0;
+        if (valid) {
+            for (int i1 = -np[0]; i1 <= np[0]; i1++)
+            for (int i2 = -np[1]; i2 <= np[1]; i2++)
+            for (int i3 = -np[2]; i3 <= np[2]; i3++) {
+                double dx1 = fj0 - fi0, dx2 = fj1 - fi1, dx3 = fj2 - fi2;
+                if (small_box) { dx1 = dx1 + (double)i1; dx2 = dx2 + (double)i2; dx3 = dx3 + (double)i3; }
+                else { near_image(dx1); near_image(dx2); near_image(dx3); }
+                double dx, dy, dz;
+                frac_to_cart(L, dx1, dx2, dx3, dx, dy, dz);
+                const double dd = dx * dx + dy * dy + dz * dz;
+                if (dd < rc2 && dd > 1e-8) {
+                    cM++;
+                    if (dd < rci2) c++;
+                }
+            }
+        }
+        // place this chunk's hits after the running totals, type by type
+        int off = 0;
+        for (int t = 0; t < T; t++) {
+            const int v = (tj == t && valid) ? c : 0;
+            const int vM = (tj == t && valid) ? cM : 0;
+            const int incl = wave_incl_scan(v);
+            const int inclM = wave_incl_scan(vM);
+            const int tot = __shfl(incl, WAVE - 1, WAVE);
+            const int totM = __shfl(inclM, WAVE - 1, WAVE);
+            if (tj == t) off = base[t] + incl - v;
+            base[t] += tot;
+            baseM[t] += totM;
+            if (baseM[t] > m) overflow = true;
+        }
+        if (overflow) break;
+        // pass 2: write this lane's hits in image order
+        if (valid && c > 0) {
+            int k = off;
+            for (int i1 = -np[0]; i1 <= np[0]; i1++)
+            for (int i2 = -np[1]; i2 <= np[1]; i2++)
+            for (int i3 = -np[2]; i3 <= np[2]; i3++) {
+                double dx1 = fj0 - fi0, dx2 = fj1 - fi1, dx3 = fj2 - fi2;
+                int s1 = i1, s2 = i2, s3 = i3;
+                if (small_box) { dx1 = dx1 + (double)i1; dx2 = dx2 + (double)i2; dx3 = dx3 + (double)i3; }
+                else { s1 = near_image(dx1); s2 = near_image(dx2); s3 = near_image(dx3); }
+                double dx, dy, dz;
+                frac_to_cart(L, dx1, dx2, dx3, dx, dy, dz);
+                const double dd = dx * dx + dy * dy + dz * dz;
+                if (dd < rc2 && dd > 1e-8 && dd < rci2) {
+                    const int64_t slot = (int64_t)tj * m + k;
+                    li[slot] = (int32_t)(j - j0) + 1;
+                    if (di) { di[3 * slot] = dx; di[3 * slot + 1] = dy; di[3 * slot + 2] = dz; }
+                    if (si) si[slot] = pack_shift(s1, s2, s3);
+                    k++;
+                }
+            }
+        }
+    }
+    if (overflow) {
+        if (lane == 0) atomicCAS(status, 0, (int)(i + 1));
+        for (int t = 0; t < T; t++) base[t] = 0;
+    }
+    // zero the unused tail of every bucket
+    for (int t = 0; t < T; t++) {
+        for (int k = base[t] + lane; k < m; k += WAVE) {
+            const int64_t slot = (int64_t)t * m + k;
+            li[slot] = 0;
+            if (di) { di[3 * slot] = 0.0; di[3 * slot + 1] = 0.0; di[3 * slot + 2] = 0.0; }
+            if (si) si[slot] = pack_shift(0, 0, 0);
+        }
+        if (counts && lane == 0) counts[i * T + t] = base[t];
+    }
+}
+
+// ------------------------------------------------------------------ cell list
+struct Grid {
+    int nc[3];
+    int noff[3];       // number of distinct offsets per dim (1..3)
+    int off[3][3];     // distinct offsets per dim
+};
+
+__global__ void bin_kernel(const double *__restrict__ fw, int64_t n, Grid g, int32_t *__restrict__ atom_cell,
+                           int32_t *__restrict__ cell_count) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int c[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            int v = (int)(fw[3 * i + d] * g.nc[d]);
+            c[d] = v < 0 ? 0 : (v >= g.nc[d] ? g.nc[d] - 1 : v);
+        }
+        const int cell = (c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2];
+        atom_cell[i] = cell;
+        atomicAdd(cell_count + cell, 1);
+    }
+}
+
+// Exclusive scan of cell_count into cell_start (ncells+1 entries), one block.
+__global__ void __launch_bounds__(1024) scan_kernel(const int32_t *__restrict__ cnt, int32_t *__restrict__ start,
+                                                    int32_t *__restrict__ cursor, int ncells) {
+    __shared__ int32_t part[1024 / WAVE];
+    __shared__ int32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    for (int base = 0; base < ncells; base += 1024) {
+        const int idx = base + threadIdx.x;
+        const int v = idx < ncells ? cnt[idx] : 0;
+        const int incl = wave_incl_scan(v);
+        if (lane == WAVE - 1) part[wid] = incl;
+        __syncthreads();
+        int wofs = 0;
+        for (int w = 0; w < wid; w++) wofs += part[w];
+        const int excl = carry + wofs + incl - v;
+        if (idx < ncells) { start[idx] = excl; cursor[idx] = excl; }
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = excl + v;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[ncells] = carry;
+}
+
+__global__ void fill_cells_kernel(const int32_t *__restrict__ atom_cell, int64_t n, int32_t *__restrict__ cursor,
+                                  int32_t *__restrict__ cell_atoms) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        cell_atoms[atomicAdd(cursor + atom_cell[i], 1)] = (int32_t)i;
+}
+
+constexpr int CELL_WAVES = 4;   // wavefronts (atoms) per block
+
+__global__ void __launch_bounds__(CELL_WAVES * WAVE)
+cell_kernel(Lattice lat, Grid g, const double *__restrict__ fw, const int32_t *__restrict__ atom_cell,
+            const int32_t *__restrict__ cell_start, const int32_t *__restrict__ cell_atoms,
+            const int32_t *__restrict__ type_map, const double *__restrict__ rc_type, int T, int m, double rc_m,
+            int64_t n, int32_t *__restrict__ list, double *__restrict__ dR, int32_t *__restrict__ counts,
+            int32_t *__restrict__ shift, int32_t *__restrict__ status) {
+    extern __shared__ int32_t smem[];
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    int32_t *bucket = smem + wid * (T * m + 2 * T);   // [T][m] hit indices
+    int32_t *cnt = bucket + T * m;                    // [T] stored
+    int32_t *cntM = cnt + T;                          // [T] counted at rc_m
+    const int64_t i = blockIdx.x * (int64_t)CELL_WAVES + wid;
+    const bool active = i < n;
+    for (int t = lane; t < 2 * T; t += WAVE) cnt[t] = 0;
+    __syncthreads();
+    const double *L = lat.a;
+    const double rc2 = rc_m * rc_m;
+    double fi0 = 0, fi1 = 0, fi2 = 0, rci2 = 0;
+    int ci[3] = {0, 0, 0};
+    if (active) {
+        fi0 = fw[3 * i]; fi1 = fw[3 * i + 1]; fi2 = fw[3 * i + 2];
+        const double rci = rc_type[type_map[i]];
+        rci2 = rci * rci;
+        int c = atom_cell[i];
+        ci[2] = c % g.nc[2]; c /= g.nc[2];
+        ci[1] = c % g.nc[1]; ci[0] = c / g.nc[1];
+    }
+    if (active) {
+        for (int a = 0; a < g.noff[0]; a++)
+        for (int b = 0; b < g.noff[1]; b++)
+        for (int c = 0; c < g.noff[2]; c++) {
+            const int cx = (ci[0] + g.off[0][a] + g.nc[0]) % g.nc[0];
+            const int cy = (ci[1] + g.off[1][b] + g.nc[1]) % g.nc[1];
+            const int cz = (ci[2] + g.off[2][c] + g.nc[2]) % g.nc[2];
+            const int cell = (cx * g.nc[1] + cy) * g.nc[2] + cz;
+            const int s0 = cell_start[cell], s1 = cell_start[cell + 1];
+            for (int p = s0 + lane; p < s1; p += WAVE) {
+                const int j = cell_atoms[p];
+                double dx1 = fw[3 * j] - fi0, dx2 = fw[3 * j + 1] - fi1, dx3 = fw[3 * j + 2] - fi2;
+                near_image(dx1); near_image(dx2); near_image(dx3);
+                double dx, dy, dz;
+                frac_to_cart(L, dx1, dx2, dx3, dx, dy, dz);
+                const double dd = dx * dx + dy * dy + dz * dz;
+                if (dd < rc2 && dd > 1e-8) {
+                    const int tj = type_map[j];
+                    atomicAdd(cntM + tj, 1);
+                    if (dd < rci2) {
+                        const int k = atomicAdd(cnt + tj, 1);
+                        if (k < m) bucket[tj * m + k] = j;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    bool overflow = false;
+    for (int t = 0; t < T; t++) overflow |= cntM[t] > m;
+    if (overflow) {
+        if (lane == 0) atomicCAS(status, 0, (int)(i + 1));
+        for (int t = 0; t < T; t++) cnt[t] = 0;
+    }
+    const int64_t S = (int64_t)T * m;
+    int32_t *li = list + i * S;
+    double *di = dR ? dR + i * S * 3 : nullptr;
+    int32_t *si = shift ? shift + i * S : nullptr;
+    for (int t = 0; t < T; t++) {
+        const int nt = cnt[t];
+        const int32_t *bk = bucket + t * m;
+        for (int e = lane; e < nt; e += WAVE) {
+            const int j = bk[e];
+            int rank = 0;
+            for (int q = 0; q < nt; q++) rank += bk[q] < j;
+            double dx1 = fw[3 * j] - fi0, dx2 = fw[3 * j + 1] - fi1, dx3 = fw[3 * j + 2] - fi2;
+            const int s1 = near_image(dx1), s2 = near_image(dx2), s3 = near_image(dx3);
+            const int64_t slot = (int64_t)t * m + rank;
+            li[slot] = j + 1;
+            if (di) {
+                double dx, dy, dz;
+                frac_to_cart(L, dx1, dx2, dx3, dx, dy, dz);
+                di[3 * slot] = dx; di[3 * slot + 1] = dy; di[3 * slot + 2] = dz;
+            }
+            if (si) si[slot] = pack_shift(s1, s2, s3);
+        }
+        for (int k = nt + lane; k < m; k += WAVE) {
+            const int64_t slot = (int64_t)t * m + k;
+            li[slot] = 0;
+            if (di) { di[3 * slot] = 0.0; di[3 * slot + 1] = 0.0; di[3 * slot + 2] = 0.0; }
+            if (si) si[slot] = pack_shift(0, 0, 0);
+        }
+        if (counts && lane == 0) counts[i * T + t] = nt;
+    }
+}
+
+bool small_box_host(const double *L, double rc) {
+    for (int d = 0; d < 3; d++) {
+        const double len = sqrt(L[3 * d] * L[3 * d] + L[3 * d + 1] * L[3 * d + 1] + L[3 * d + 2] * L[3 * d + 2]);
+        if (len < 2.0 * rc) return true;
+    }
+    return false;
+}
+
+Grid make_grid(const double *L, double rc) {
+    // plane spacing h_d = V / |a_e x a_f|
+    const double *a = L, *b = L + 3, *c = L + 6;
+    auto cross = [](const double *u, const double *v, double *w) {
+        w[0] = u[1] * v[2] - u[2] * v[1];
+        w[1] = u[2] * v[0] - u[0] * v[2];
+        w[2] = u[0] * v[1] - u[1] * v[0];
+    };
+    double bc[3], ca[3], ab[3];
+    cross(b, c, bc); cross(c, a, ca); cross(a, b, ab);
+    const double V = fabs(a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2]);
+    const double *ar[3] = {bc, ca, ab};
+    Grid g;
+    for (int d = 0; d < 3; d++) {
+        const double area = sqrt(ar[d][0] * ar[d][0] + ar[d][1] * ar[d][1] + ar[d][2] * ar[d][2]);
+        int nc = (int)floor(V / area / rc);
+        if (nc < 1) nc = 1;
+        if (nc > 1024) nc = 1024;
+        g.nc[d] = nc;
+        if (nc >= 3) { g.noff[d] = 3; g.off[d][0] = -1; g.off[d][1] = 0; g.off[d][2] = 1; }
+        else if (nc == 2) { g.noff[d] = 2; g.off[d][0] = 0; g.off[d][1] = 1; g.off[d][2] = 0; }
+        else { g.noff[d] = 1; g.off[d][0] = 0; g.off[d][1] = 0; g.off[d][2] = 0; }
+    }
+    return g;
+}
+
+}  // namespace
+}  // namespace matpl
+
+using namespace matpl;
+
+MATPL_EXPORT int64_t matpl_dp_neighbor_cell_workspace_bytes(const double *lattice_host, double rc_m, int64_t n) {
+    Grid g = make_grid(lattice_host, rc_m);
+    const int64_t ncells = (int64_t)g.nc[0] * g.nc[1] * g.nc[2];
+    // atom_cell[n], cell_atoms[n], cell_count[ncells], cell_start[ncells+1], cursor[ncells]
+    return (int64_t)sizeof(int32_t) * (2 * n + 3 * ncells + 1 + 64);
+}
+
+MATPL_EXPORT int matpl_dp_neighbor_is_small_box(const double *lattice_host, double rc_m) {
+    return small_box_host(lattice_host, rc_m) ? 1 : 0;
+}
+
+MATPL_EXPORT int matpl_dp_neighbor_brute(const double *lattice, const double *frac, const int64_t *frame_start,
+                                         const int32_t *atom_frame, const int32_t *type_map, const double *rc_type,
+                                         int ntypes, int m, double rc_m, int64_t nframes, int64_t ntot,
+                                         double *frac_wrapped, int32_t *list, double *dR, int32_t *counts,
+                                         int32_t *shift, int32_t *status, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    MATPL_CHECK_ARG(ntypes >= 1 && ntypes <= 16, "dp_neighbor: ntypes must be in [1,16]");
+    MATPL_CHECK_ARG(m >= 1 && nframes >= 0 && ntot >= 0 && rc_m > 0, "dp_neighbor: bad sizes");
+    if (ntot == 0) return OK;
+    MATPL_CHECK_ARG(lattice && frac && frame_start && atom_frame && type_map && rc_type && frac_wrapped &&
+                        list && status, "dp_neighbor: null pointer");
+    hipLaunchKernelGGL(wrap_kernel, dim3(grid_cap(ntot * 3)), dim3(256), 0, st, frac, frac_wrapped, ntot * 3);
+    int rc = check_launch("wrap_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(brute_kernel, dim3(ceil_div(ntot, 4)), dim3(256), 0, st, lattice, frac_wrapped, frame_start,
+                       atom_frame, type_map, rc_type, ntypes, m, rc_m, ntot, list, dR, counts, shift, status);
+    return check_launch("brute_kernel");
+}
+
+MATPL_EXPORT int matpl_dp_neighbor_cell(const double *lattice_host, const double *frac, const int32_t *type_map,
+                                        const double *rc_type, int ntypes, int m, double rc_m, int64_t n,
+                                        double *frac_wrapped, int32_t *list, double *dR, int32_t *counts,
+                                        int32_t *shift, int32_t *status, void *workspace, int64_t workspace_bytes,
+                                        void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    MATPL_CHECK_ARG(ntypes >= 1 && ntypes <= 16, "dp_neighbor_cell: ntypes must be in [1,16]");
+    MATPL_CHECK_ARG(m >= 1 && n >= 0 && rc_m > 0, "dp_neighbor_cell: bad sizes");
+    MATPL_CHECK_ARG(lattice_host, "dp_neighbor_cell: null lattice");
+    MATPL_CHECK_ARG(!small_box_host(lattice_host, rc_m),
+                    "dp_neighbor_cell: a lattice vector is shorter than 2*rc; use the brute-force search");
+    if (n == 0) return OK;
+    MATPL_CHECK_ARG(frac && type_map && rc_type && frac_wrapped && list && status && workspace,
+                    "dp_neighbor_cell: null pointer");
+    MATPL_CHECK_ARG(n < (int64_t)0x7fffffff, "dp_neighbor_cell: too many atoms");
+    const int64_t need = matpl_dp_neighbor_cell_workspace_bytes(lattice_host, rc_m, n);
+    MATPL_CHECK_ARG(workspace_bytes >= need, "dp_neighbor_cell: workspace %lld < %lld bytes",
+                    (long long)workspace_bytes, (long long)need);
+    const size_t lds = sizeof(int32_t) * CELL_WAVES * ((size_t)ntypes * m + 2 * ntypes);
+    MATPL_CHECK_ARG(lds <= 64 * 1024, "dp_neighbor_cell: ntypes*m too large for LDS buckets");
+    Grid g = make_grid(lattice_host, rc_m);
+    const int ncells = g.nc[0] * g.nc[1] * g.nc[2];
+    int32_t *ws = (int32_t *)workspace;
+    int32_t *atom_cell = ws;
+    int32_t *cell_atoms = atom_cell + n;
+    int32_t *cell_count = cell_atoms + n;
+    int32_t *cell_start = cell_count + ncells;
+    int32_t *cursor = cell_start + ncells + 1;
+    Lattice lat;
+    for (int k = 0; k < 9; k++) lat.a[k] = lattice_host[k];
+    int rc;
+    hipLaunchKernelGGL(wrap_kernel, dim3(grid_cap(n * 3)), dim3(256), 0, st, frac, frac_wrapped, n * 3);
+    if ((rc = check_launch("wrap_kernel"))) return rc;
+    if ((rc = fill_async<int32_t>(cell_count, 0, ncells, st))) return rc;
+    hipLaunchKernelGGL(bin_kernel, dim3(grid_cap(n)), dim3(256), 0, st, frac_wrapped, n, g, atom_cell, cell_count);
+    if ((rc = check_launch("bin_kernel"))) return rc;
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, cell_count, cell_start, cursor, ncells);
+    if ((rc = check_launch("scan_kernel"))) return rc;
+    hipLaunchKernelGGL(fill_cells_kernel, dim3(grid_cap(n)), dim3(256), 0, st, atom_cell, n, cursor, cell_atoms);
+    if ((rc = check_launch("fill_cells_kernel"))) return rc;
+    hipLaunchKernelGGL(cell_kernel, dim3(ceil_div(n, CELL_WAVES)), dim3(CELL_WAVES * WAVE), lds, st, lat, g,
+                       frac_wrapped, atom_cell, cell_start, cell_atoms, type_map, rc_type, ntypes, m, rc_m, n, list,
+                       dR, counts, shift, status);
+    return check_launch("cell_kernel");
+}

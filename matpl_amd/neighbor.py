@@ -1,0 +1,143 @@
+"""DP neighbor search on the GPU (the find_neighbore contract).
+
+``find_neighbore`` mirrors src/pre_data/dpuni_data_loader.py:595-650 for one
+frame: it returns list_neigh [N, T*m] (1-based, 0 = empty) and dR [N, T*m, 3]
+(float64, r_j - r_i), identical to NeighConst.f90:14-149. Large boxes use the
+cell-list kernel, boxes with a lattice vector shorter than 2*rc use the
+image-enumerating brute-force kernel (the reference's own small-box rule).
+``find_neighbore_batch`` runs many frames in one launch.
+
+An overflowing bucket (more than m neighbors of one type) raises
+OverflowError, as the reference stops with "maxNeighborNum too small".
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .calcops import _ptr, _stream
+
+
+@dataclass
+class NeighborList:
+    list: torch.Tensor          # [Ntot, T*m] int32, 1-based within frame, 0 = empty
+    dR: Optional[torch.Tensor]  # [Ntot, T*m, 3] float64
+    counts: torch.Tensor        # [Ntot, T] int32
+    shift: Optional[torch.Tensor]  # [Ntot, T*m] int32 packed image shift
+    frac_wrapped: torch.Tensor  # [Ntot, 3] float64
+    status: torch.Tensor        # [1] int32, 0 or 1 + first overflowing atom
+
+    def check(self):
+        s = int(self.status.item())
+        if s != 0:
+            raise OverflowError(f"maxNeighborNum too small (atom {s - 1})")
+        return self
+
+
+def _rc_type_tensor(rc_type, ntypes, rc_m, device):
+    if rc_type is None:
+        rc_type = np.full(ntypes, rc_m)
+    return torch.as_tensor(np.asarray(rc_type, np.float64), device=device)
+
+
+def find_neighbore_gpu(lattice, frac, type_map, ntypes: int, m: int, rc_m: float, rc_type=None,
+                       want_dR=True, want_shift=False, device="cuda", force_brute=False,
+                       check=True) -> NeighborList:
+    """One frame. lattice [3,3] (host array; rows are cell vectors); frac [N,3]."""
+    lat = np.ascontiguousarray(np.asarray(lattice, np.float64).reshape(3, 3))
+    frac_t = torch.as_tensor(frac, dtype=torch.float64, device=device).contiguous()
+    tm = torch.as_tensor(type_map, dtype=torch.int32, device=device).contiguous()
+    n = frac_t.shape[0]
+    S = ntypes * m
+    rct = _rc_type_tensor(rc_type, ntypes, rc_m, device)
+    out = NeighborList(
+        list=torch.empty((n, S), dtype=torch.int32, device=device),
+        dR=torch.empty((n, S, 3), dtype=torch.float64, device=device) if want_dR else None,
+        counts=torch.empty((n, ntypes), dtype=torch.int32, device=device),
+        shift=torch.empty((n, S), dtype=torch.int32, device=device) if want_shift else None,
+        frac_wrapped=torch.empty((n, 3), dtype=torch.float64, device=device),
+        status=torch.zeros(1, dtype=torch.int32, device=device),
+    )
+    small = _lib.query("matpl_dp_neighbor_is_small_box", lat.ctypes.data, float(rc_m))
+    if small or force_brute:
+        lat_d = torch.as_tensor(lat.reshape(1, 9), device=device)
+        fs = torch.tensor([0, n], dtype=torch.int64, device=device)
+        af = torch.zeros(n, dtype=torch.int32, device=device)
+        _lib.call("matpl_dp_neighbor_brute", _ptr(lat_d), _ptr(frac_t), _ptr(fs), _ptr(af), _ptr(tm), _ptr(rct),
+                  ntypes, m, float(rc_m), 1, n, _ptr(out.frac_wrapped), _ptr(out.list), _ptr(out.dR),
+                  _ptr(out.counts), _ptr(out.shift), _ptr(out.status), _stream())
+    else:
+        nbytes = _lib.query("matpl_dp_neighbor_cell_workspace_bytes", lat.ctypes.data, float(rc_m), n)
+        ws = torch.empty(max(int(nbytes), 16), dtype=torch.uint8, device=device)
+        _lib.call("matpl_dp_neighbor_cell", lat.ctypes.data, _ptr(frac_t), _ptr(tm), _ptr(rct), ntypes, m,
+                  float(rc_m), n, _ptr(out.frac_wrapped), _ptr(out.list), _ptr(out.dR), _ptr(out.counts),
+                  _ptr(out.shift), _ptr(out.status), _ptr(ws), ws.numel(), _stream())
+    return out.check() if check else out
+
+
+def find_neighbore_batch(lattices, fracs: List, type_maps: List, ntypes: int, m: int, rc_m: float,
+                         rc_type=None, want_dR=True, want_shift=False, device="cuda", check=True) -> NeighborList:
+    """Many frames packed along the atom axis (brute-force kernel, any box size)."""
+    lat = torch.as_tensor(np.stack([np.asarray(L, np.float64).reshape(9) for L in lattices]), device=device)
+    sizes = [len(f) for f in fracs]
+    start = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    ntot = int(start[-1])
+    frac_t = torch.as_tensor(np.concatenate([np.asarray(f, np.float64) for f in fracs]), device=device)
+    tm = torch.as_tensor(np.concatenate([np.asarray(t, np.int32) for t in type_maps]), device=device)
+    af = torch.as_tensor(np.repeat(np.arange(len(sizes), dtype=np.int32), sizes), device=device)
+    fs = torch.as_tensor(start, device=device)
+    S = ntypes * m
+    rct = _rc_type_tensor(rc_type, ntypes, rc_m, device)
+    out = NeighborList(
+        list=torch.empty((ntot, S), dtype=torch.int32, device=device),
+        dR=torch.empty((ntot, S, 3), dtype=torch.float64, device=device) if want_dR else None,
+        counts=torch.empty((ntot, ntypes), dtype=torch.int32, device=device),
+        shift=torch.empty((ntot, S), dtype=torch.int32, device=device) if want_shift else None,
+        frac_wrapped=torch.empty((ntot, 3), dtype=torch.float64, device=device),
+        status=torch.zeros(1, dtype=torch.int32, device=device),
+    )
+    _lib.call("matpl_dp_neighbor_brute", _ptr(lat), _ptr(frac_t), _ptr(fs), _ptr(af), _ptr(tm), _ptr(rct), ntypes, m,
+              float(rc_m), len(sizes), ntot, _ptr(out.frac_wrapped), _ptr(out.list), _ptr(out.dR), _ptr(out.counts),
+              _ptr(out.shift), _ptr(out.status), _stream())
+    return out.check() if check else out
+
+
+def image_dr(lst: torch.Tensor, dR: torch.Tensor) -> torch.Tensor:
+    """ImageDR = [|dR|, dx, dy, dz], |dR| only where list > 0 (dpuni_data_loader.py:669-673)."""
+    r = torch.sqrt((dR * dR).sum(-1)) * (lst > 0)
+    return torch.cat([r.unsqueeze(-1), dR], -1)
+
+
+def calc_stat(lst: torch.Tensor, dR: torch.Tensor, type_map: torch.Tensor, ntypes: int, rmin: float, rmax: float):
+    """davg/dstd [T, 4] on the GPU (dpuni_data_loader.py:422-560)."""
+    n, S = lst.shape
+    davg = torch.empty((ntypes, 4), dtype=torch.float64, device=lst.device)
+    dstd = torch.empty((ntypes, 4), dtype=torch.float64, device=lst.device)
+    ws = torch.empty(int(_lib.query("matpl_dp_calc_stat_workspace_elems", ntypes)), dtype=torch.float64,
+                     device=lst.device)
+    _lib.call("matpl_dp_calc_stat", _ptr(lst.contiguous()), _ptr(dR.contiguous()),
+              _ptr(type_map.to(torch.int32).contiguous()), n, S, ntypes, float(rmin), float(rmax), _ptr(davg),
+              _ptr(dstd), _ptr(ws), ws.numel(), _stream())
+    return davg, dstd
+
+
+def env_mat(image_dr_t: torch.Tensor, type_map: torch.Tensor, davg_full: torch.Tensor, dstd_full: torch.Tensor,
+            rmin: float, rmax: float):
+    """Ri [B,N,S,4], Ri_d [B,N,S,4,3] from ImageDR [B,N,S,4] (DP.calculate_Ri, dp_dp.py:217-302)."""
+    if image_dr_t.dim() == 3:
+        image_dr_t = image_dr_t.unsqueeze(0)
+    B, N, S, _ = image_dr_t.shape
+    dt = image_dr_t.dtype
+    sfx = {torch.float32: "f32", torch.float64: "f64"}[dt]
+    ntypes = davg_full.shape[0]
+    idr = image_dr_t.contiguous()
+    Ri = torch.empty((B, N, S, 4), dtype=dt, device=idr.device)
+    Ri_d = torch.empty((B, N, S, 4, 3), dtype=dt, device=idr.device)
+    _lib.call(f"matpl_dp_env_mat_{sfx}", _ptr(idr), _ptr(type_map.to(torch.int32).contiguous()),
+              _ptr(davg_full.to(dt).contiguous()), _ptr(dstd_full.to(dt).contiguous()), B, N, S, ntypes,
+              float(rmin), float(rmax), _ptr(Ri), _ptr(Ri_d), _stream())
+    return Ri, Ri_d
