@@ -136,6 +136,37 @@ int matpl_dp_calc_stat(const int32_t *list, const double *dR, const int32_t *typ
                        int ntypes, double rmin, double rmax, double *davg, double *dstd, double *workspace,
                        int64_t workspace_elems, void *stream);
 
+/* ------------------------------------------- fused DP descriptor + nets */
+/*
+ * Forward: for every atom, the in-cutoff neighbors of its neighbor list
+ * (list/shift [N][T][mnl], counts [N][T], positions pos [N][3] float64,
+ * lattice [F][9] device, atom_frame [N] / frame_start [F+1] or NULL for one
+ * frame) give the normalized Ri (FP32), the embedding net output G on the
+ * matrix cores (bf16 != 0: BF16 in / FP32 accumulate; 0: FP32), and
+ * A[i] = scale * (sum Ri^T G + empty-slot term) [N][4][E], E in {16,25,32}.
+ * vec/mat/tinfo are the packed net parameters (matpl_amd/dp_fused.py).
+ * Backward: given dA = dEtot/dA, force [N][3] (float, accumulated: zero it
+ * first) receives -dE/dx and virial [F][9] (float64, accumulated) the
+ * calculateVirial-convention virial.
+ * Together they replace DP.calculate_Ri + calculate_xyz_scater + the
+ * autograd dE/dRi + calculateForce/calculateVirial of the reference
+ * (src/model/dp_dp.py:217-469, src/model/dp_embedding.py:40-94); the fitting
+ * net between them is evaluated by the caller.
+ * status (device int32, zero before the call) receives 1 + the first atom
+ * with more than m in-cutoff neighbors of one type.
+ */
+int matpl_dp_fused_forward(const double *pos, const double *lattice, const int32_t *atom_frame,
+                           const int64_t *frame_start, const int32_t *list, const int32_t *shift,
+                           const int32_t *counts, const int32_t *type_map, const float *vec, const void *mat,
+                           const float *tinfo, int64_t natoms, int ntypes, int mnl, int m, int E, int bf16,
+                           double rcut, double rcut_smooth, double scale, float *A, int32_t *status, void *stream);
+int matpl_dp_fused_backward(const double *pos, const double *lattice, const int32_t *atom_frame,
+                            const int64_t *frame_start, const int32_t *list, const int32_t *shift,
+                            const int32_t *counts, const int32_t *type_map, const float *vec, const void *mat,
+                            const float *tinfo, int64_t natoms, int ntypes, int mnl, int m, int E, int bf16,
+                            double rcut, double rcut_smooth, double scale, const float *dA, float *force,
+                            double *virial, int32_t *status, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,9 @@ SIGNATURES = {
     "matpl_dp_env_mat_f64": (I32, [P, P, P, P, I64, I64, I64, I32, F64, F64, P, P, P]),
     "matpl_dp_calc_stat_workspace_elems": (I64, [I32]),
     "matpl_dp_calc_stat": (I32, [P, P, P, I64, I64, I32, F64, F64, P, P, P, I64, P]),
+    # fused DP descriptor + embedding net
+    "matpl_dp_fused_forward": (I32, [P] * 11 + [I64, I32, I32, I32, I32, I32, F64, F64, F64, P, P, P]),
+    "matpl_dp_fused_backward": (I32, [P] * 11 + [I64, I32, I32, I32, I32, I32, F64, F64, F64, P, P, P, P, P]),
 }
 
 

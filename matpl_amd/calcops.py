@@ -98,9 +98,13 @@ def dp_force_grad(nlist, Ri_d, gF, nghost: int, dE_shape):
     S = nlist.numel() // max(B * N, 1)
     sfx = _sfx(Ri_d)
     g = torch.empty(dE_shape, dtype=Ri_d.dtype, device=Ri_d.device)
+    # converted tensors are held in locals until the launch is enqueued: a
+    # temporary freed earlier could be reused by the next conversion
     gF = gF.to(Ri_d.dtype).contiguous()
-    _lib.call(f"matpl_dp_force_grad_{sfx}", _ptr(nlist.to(torch.int32).contiguous()), _ptr(_aligned(Ri_d)),
-              _ptr(gF), B, N, S, nghost, _ptr(g), _stream())
+    nlist = nlist.to(torch.int32).contiguous()
+    Ri_d = _aligned(Ri_d)
+    _lib.call(f"matpl_dp_force_grad_{sfx}", _ptr(nlist), _ptr(Ri_d), _ptr(gF), B, N, S, nghost, _ptr(g),
+              _stream())
     return g
 
 
@@ -109,9 +113,12 @@ def dp_virial_grad(nlist, Rij, Ri_d, gV, dE_shape):
     S = nlist.numel() // max(B * N, 1)
     sfx = _sfx(Ri_d)
     g = torch.empty(dE_shape, dtype=Ri_d.dtype, device=Ri_d.device)
-    _lib.call(f"matpl_dp_virial_grad_{sfx}", _ptr(nlist.to(torch.int32).contiguous()),
-              _ptr(Rij.to(Ri_d.dtype).contiguous()), _ptr(_aligned(Ri_d)), _ptr(gV.to(Ri_d.dtype).contiguous()),
-              B, N, S, _ptr(g), _stream())
+    nlist = nlist.to(torch.int32).contiguous()
+    Rij = Rij.to(Ri_d.dtype).contiguous()
+    gV = gV.to(Ri_d.dtype).contiguous()
+    Ri_d = _aligned(Ri_d)
+    _lib.call(f"matpl_dp_virial_grad_{sfx}", _ptr(nlist), _ptr(Rij), _ptr(Ri_d), _ptr(gV), B, N, S, _ptr(g),
+              _stream())
     return g
 
 

@@ -1,0 +1,186 @@
+"""Fused DP evaluation: energy, forces and virial from positions + neighbor list.
+
+Three stages per evaluation, all on the current stream:
+  1. ``matpl_dp_fused_forward``: geometry (FP64 minimum image, FP32
+     descriptor), embedding net on the matrix cores, contraction
+     -> A [N, 4, E] (the reference's xyz_scater_a, dp_dp.py:344-415);
+  2. fitting: DR = A^T A[:, :M2] -> fitting net -> Ei, and dEtot/dA;
+  3. ``matpl_dp_fused_backward``: recomputed embedding forward + backward,
+     dE/dRi -> force scatter and virial (dp_dp.py:417-469 semantics).
+
+Precision modes: "bf16" runs the embedding GEMMs on BF16 matrix cores with FP32
+accumulation (and a fast exp2/rcp tanh); "fp32" runs them on FP32 matrix
+cores with tanhf. Geometry, descriptors, contraction and scatter are FP32 in
+both; the cutoff test, Etot and the virial are FP64.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .calcops import _ptr, _stream
+from .dpnet import DPWeights
+
+NVEC, NMAT, NTI = 7, 4, 16
+V_W1, V_B1, V_B2, V_B3, V_R2, V_R3, V_GPAD = range(NVEC)
+M_W2F, M_W3F, M_W2B, M_W3B = range(NMAT)
+
+
+def feat(r: int, h: int) -> int:
+    """Feature held by accumulator register r in lane half h (32x32 MFMA layout)."""
+    return 8 * (r // 4) + 4 * h + (r % 4)
+
+
+def _emb_forward_np(layers, s: float) -> np.ndarray:
+    """FP64 embedding net at one input (dp_embedding.py:73-94)."""
+    x = np.array([s], np.float64)
+    for W, b, r in layers:
+        hdn = np.tanh(x @ W + (b if b is not None else 0.0))
+        hs = hdn * r if r is not None else hdn
+        if W.shape[1] == W.shape[0]:
+            x = hs + x
+        elif W.shape[1] == 2 * W.shape[0]:
+            x = np.concatenate([x, x]) + hs
+        else:
+            x = hdn
+    return x
+
+
+@dataclass
+class FusedWeights:
+    vec: torch.Tensor      # [T*T, NVEC, 2, 16] float32
+    mat: torch.Tensor      # [T*T, NMAT, 64, 16] bfloat16 or float32
+    tinfo: torch.Tensor    # [T, NTI] float32
+    E: int
+    bf16: bool
+
+
+def pack_fused(w: DPWeights, precision: str = "bf16", device="cuda") -> FusedWeights:
+    cfg = w.cfg
+    T = cfg.ntypes
+    es = cfg.emb_sizes
+    if len(es) != 3 or es[1] != es[0] or es[2] != es[0] or es[0] not in (16, 25, 32):
+        raise ValueError("fused DP path needs embedding sizes [E, E, E] with E in {16, 25, 32}")
+    E = es[0]
+    bf16 = precision == "bf16"
+    if precision not in ("bf16", "fp32"):
+        raise ValueError("precision must be 'bf16' or 'fp32'")
+    vec = np.zeros((T * T, NVEC, 2, 16), np.float32)
+    mat = np.zeros((T * T, NMAT, 64, 16), np.float32)
+    tinfo = np.zeros((T, NTI), np.float32)
+    ri_pad = -w.davg / w.dstd                                         # Ri of an empty slot
+    for ti in range(T):
+        tinfo[ti, 0:4] = w.davg[ti]
+        tinfo[ti, 4:8] = 1.0 / w.dstd[ti]
+        tinfo[ti, 8:12] = w.dstd[ti]
+        tinfo[ti, 12:16] = ri_pad[ti]
+    for p in range(T * T):
+        ti = p // T
+        (W1, b1, _), (W2, b2, r2), (W3, b3, r3) = w.emb[p]
+        gpad = _emb_forward_np(w.emb[p], float(ri_pad[ti, 0]))
+        one = np.ones(E)
+        vecs = {V_W1: W1[0], V_B1: b1 if b1 is not None else np.zeros(E), V_B2: b2 if b2 is not None else np.zeros(E),
+                V_B3: b3 if b3 is not None else np.zeros(E), V_R2: r2 if r2 is not None else one,
+                V_R3: r3 if r3 is not None else one, V_GPAD: gpad}
+        for h in range(2):
+            for r in range(16):
+                f = feat(r, h)
+                if f < E:
+                    for k, v in vecs.items():
+                        vec[p, k, h, r] = v[f]
+        for lane in range(64):
+            h, i = lane // 32, lane % 32
+            for r in range(16):
+                f = feat(r, h)
+                if f < E and i < E:
+                    mat[p, M_W2F, lane, r] = W2[f, i]
+                    mat[p, M_W3F, lane, r] = W3[f, i]
+                    mat[p, M_W2B, lane, r] = W2[i, f]
+                    mat[p, M_W3B, lane, r] = W3[i, f]
+    mt = torch.as_tensor(mat, device=device)
+    return FusedWeights(torch.as_tensor(vec, device=device), mt.to(torch.bfloat16) if bf16 else mt,
+                        torch.as_tensor(tinfo, device=device), E, bf16)
+
+
+class _Fitting:
+    """Per-type fitting nets in FP32 torch with autograd for dE/dA."""
+
+    def __init__(self, w: DPWeights, device, dtype=torch.float32):
+        self.cfg = w.cfg
+        self.nets = []
+        for t in range(w.cfg.ntypes):
+            self.nets.append([(torch.as_tensor(W, dtype=dtype, device=device),
+                               torch.as_tensor(b, dtype=dtype, device=device) if b is not None else None,
+                               torch.as_tensor(r, dtype=dtype, device=device) if r is not None else None)
+                              for (W, b, r) in w.fit[t]])
+
+    def __call__(self, A: torch.Tensor, type_map: torch.Tensor, type_ranges):
+        """A [N, 4, E] -> (Ei [N], dEtot/dA [N, 4, E])."""
+        M2 = self.cfg.M2
+        A = A.detach().requires_grad_()
+        Ei = torch.empty(A.shape[0], dtype=A.dtype, device=A.device)
+        parts = []
+        for t, (a, b) in enumerate(type_ranges):
+            if b <= a:
+                continue
+            At = A[a:b]
+            DR = (At.transpose(1, 2) @ At[:, :, :M2]).reshape(b - a, -1)
+            x = DR
+            layers = self.nets[t]
+            for l, (W, bias, r) in enumerate(layers):
+                z = x @ W + (bias if bias is not None else 0.0)
+                if l == len(layers) - 1:
+                    x = z
+                    break
+                hdn = torch.tanh(z)
+                x = hdn if l == 0 else (hdn * r if r is not None else hdn) + x
+            parts.append((a, b, x.reshape(-1)))
+        Etot = sum(x.sum() for _, _, x in parts)
+        (dA,) = torch.autograd.grad(Etot, A)
+        for a, b, x in parts:
+            Ei[a:b] = x.detach()
+        return Ei, dA
+
+
+class FusedDP:
+    """DP E/F/V for one (or a batch of) frame(s) given a neighbor list.
+
+    Atoms must be grouped by type (type ranges are contiguous).
+    """
+
+    def __init__(self, w: DPWeights, precision: str = "bf16", device="cuda"):
+        self.w = w
+        self.cfg = w.cfg
+        self.device = torch.device(device)
+        self.fw = pack_fused(w, precision, self.device)
+        self.fit = _Fitting(w, self.device)
+        self.precision = precision
+
+    def __call__(self, pos: torch.Tensor, lattice: torch.Tensor, type_map: torch.Tensor, nl_list: torch.Tensor,
+                 nl_shift: torch.Tensor, nl_counts: torch.Tensor, mnl: int, atom_frame: Optional[torch.Tensor] = None,
+                 frame_start: Optional[torch.Tensor] = None, nframes: int = 1, type_ranges=None):
+        cfg = self.cfg
+        N = pos.shape[0]
+        T = cfg.ntypes
+        dev = pos.device
+        if type_ranges is None:
+            tm = type_map.cpu().numpy()
+            type_ranges = [(int(np.searchsorted(tm, t, "left")), int(np.searchsorted(tm, t, "right")))
+                           for t in range(T)]
+        scale = 1.0 / (cfg.m * T)
+        A = torch.empty((N, 4, self.fw.E), dtype=torch.float32, device=dev)
+        status = torch.zeros(1, dtype=torch.int32, device=dev)
+        args = (_ptr(pos), _ptr(lattice), _ptr(atom_frame), _ptr(frame_start), _ptr(nl_list), _ptr(nl_shift),
+                _ptr(nl_counts), _ptr(type_map), _ptr(self.fw.vec), _ptr(self.fw.mat), _ptr(self.fw.tinfo), N, T, mnl,
+                cfg.m, self.fw.E, int(self.fw.bf16), float(cfg.rcut), float(cfg.rcut_smooth), scale)
+        _lib.call("matpl_dp_fused_forward", *args, _ptr(A), _ptr(status), _stream())
+        Ei, dA = self.fit(A, type_map, type_ranges)
+        dA = dA.contiguous()
+        F = torch.zeros((N, 3), dtype=torch.float32, device=dev)
+        V = torch.zeros((nframes, 9), dtype=torch.float64, device=dev)
+        _lib.call("matpl_dp_fused_backward", *args, _ptr(dA), _ptr(F), _ptr(V), _ptr(status), _stream())
+        return Ei, F, V, status

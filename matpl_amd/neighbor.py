@@ -119,9 +119,12 @@ def calc_stat(lst: torch.Tensor, dR: torch.Tensor, type_map: torch.Tensor, ntype
     dstd = torch.empty((ntypes, 4), dtype=torch.float64, device=lst.device)
     ws = torch.empty(int(_lib.query("matpl_dp_calc_stat_workspace_elems", ntypes)), dtype=torch.float64,
                      device=lst.device)
-    _lib.call("matpl_dp_calc_stat", _ptr(lst.contiguous()), _ptr(dR.contiguous()),
-              _ptr(type_map.to(torch.int32).contiguous()), n, S, ntypes, float(rmin), float(rmax), _ptr(davg),
-              _ptr(dstd), _ptr(ws), ws.numel(), _stream())
+    # converted inputs stay referenced until the launch is enqueued
+    lst = lst.contiguous()
+    dR = dR.contiguous()
+    tm = type_map.to(torch.int32).contiguous()
+    _lib.call("matpl_dp_calc_stat", _ptr(lst), _ptr(dR), _ptr(tm), n, S, ntypes, float(rmin), float(rmax),
+              _ptr(davg), _ptr(dstd), _ptr(ws), ws.numel(), _stream())
     return davg, dstd
 
 
@@ -137,7 +140,10 @@ def env_mat(image_dr_t: torch.Tensor, type_map: torch.Tensor, davg_full: torch.T
     idr = image_dr_t.contiguous()
     Ri = torch.empty((B, N, S, 4), dtype=dt, device=idr.device)
     Ri_d = torch.empty((B, N, S, 4, 3), dtype=dt, device=idr.device)
-    _lib.call(f"matpl_dp_env_mat_{sfx}", _ptr(idr), _ptr(type_map.to(torch.int32).contiguous()),
-              _ptr(davg_full.to(dt).contiguous()), _ptr(dstd_full.to(dt).contiguous()), B, N, S, ntypes,
+    # converted inputs stay referenced until the launch is enqueued
+    tm = type_map.to(torch.int32).contiguous()
+    davg = davg_full.to(dt).contiguous()
+    dstd = dstd_full.to(dt).contiguous()
+    _lib.call(f"matpl_dp_env_mat_{sfx}", _ptr(idr), _ptr(tm), _ptr(davg), _ptr(dstd), B, N, S, ntypes,
               float(rmin), float(rmax), _ptr(Ri), _ptr(Ri_d), _stream())
     return Ri, Ri_d

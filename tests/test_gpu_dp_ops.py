@@ -34,18 +34,18 @@ def test_force_virial_ops(dev, dtype, B, N, S, ng):
         dE, Rid, Rij, Fc = [x.astype(np.float32).astype(np.float64) for x in (dE, Rid, Rij, Fc)]
     t = lambda x: torch.as_tensor(x, device=dev, dtype=dtype)
     nlt = torch.as_tensor(nl, device=dev).unsqueeze(2)
-    F = torch.ops.CalcOps_cuda.calculateForce(nlt, t(dE).view(B, N, 1, -1), t(Rid).view(B, N, -1, 3), t(Fc),
+    F = torch.ops.CalcOps_cuda.calculateForce(nlt, t(dE).view(B, N, 1, S * 4), t(Rid).view(B, N, S * 4, 3), t(Fc),
                                               torch.tensor(ng))[0]
     want = oracle.dp_force_op(nl, dE, Rid, Fc, ng)
-    scale = max(1.0, np.abs(want).max())
+    scale = max(1.0, np.abs(want).max(initial=0.0))
     assert F.shape == (B, N + ng, 3)
     np.testing.assert_allclose(F.cpu().double().numpy(), want, atol=TOL[dtype] * scale * 10)
-    V, AV = torch.ops.CalcOps_cuda.calculateVirial(nlt, t(dE).view(B, N, 1, -1), t(Rij), t(Rid).view(B, N, -1, 3),
+    V, AV = torch.ops.CalcOps_cuda.calculateVirial(nlt, t(dE).view(B, N, 1, S * 4), t(Rij), t(Rid).view(B, N, S * 4, 3),
                                                    torch.tensor(ng))
     wV, wAV = oracle.dp_virial_op(nl, dE, Rij, Rid, ng)
-    scale = max(1.0, np.abs(wAV).max())
+    scale = max(1.0, np.abs(wAV).max(initial=0.0))
     np.testing.assert_allclose(AV.cpu().double().numpy(), wAV, atol=TOL[dtype] * scale * 10)
-    np.testing.assert_allclose(V.cpu().double().numpy(), wV, atol=TOL[dtype] * max(1.0, np.abs(wV).max()) * 100)
+    np.testing.assert_allclose(V.cpu().double().numpy(), wV, atol=TOL[dtype] * max(1.0, np.abs(wV).max(initial=0.0)) * 100)
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
@@ -55,14 +55,14 @@ def test_force_virial_backward(dev, dtype):
     nl[nl >= N] = -1
     t = lambda x: torch.as_tensor(x, device=dev, dtype=dtype)
     nlt = torch.as_tensor(nl, device=dev).unsqueeze(2)
-    dEt = t(dE).view(B, N, 1, -1).requires_grad_()
-    F = torch.ops.CalcOps_cuda.calculateForce(nlt, dEt, t(Rid).view(B, N, -1, 3), t(Fc), torch.tensor(0))[0]
+    dEt = t(dE).view(B, N, 1, S * 4).requires_grad_()
+    F = torch.ops.CalcOps_cuda.calculateForce(nlt, dEt, t(Rid).view(B, N, S * 4, 3), t(Fc), torch.tensor(0))[0]
     gF = np.random.default_rng(1).normal(size=(B, N, 3))
     (g,) = torch.autograd.grad(F, dEt, t(gF))
     want = oracle.dp_force_grad_op(nl, Rid, gF, 0)
     np.testing.assert_allclose(g.detach().cpu().double().numpy().reshape(want.shape), want,
                                atol=TOL[dtype] * 100)
-    V = torch.ops.CalcOps_cuda.calculateVirial(nlt, dEt, t(Rij), t(Rid).view(B, N, -1, 3), torch.tensor(0))[0]
+    V = torch.ops.CalcOps_cuda.calculateVirial(nlt, dEt, t(Rij), t(Rid).view(B, N, S * 4, 3), torch.tensor(0))[0]
     gV = np.random.default_rng(2).normal(size=(B, 9))
     (g,) = torch.autograd.grad(V, dEt, t(gV))
     want = oracle.dp_virial_grad_op(nl, Rij, Rid, gV)
