@@ -112,6 +112,17 @@ int matpl_dp_neighbor_cell(const double *lattice_host, const double *frac, const
                            int32_t *list, double *dR, int32_t *counts, int32_t *shift, int32_t *status,
                            void *workspace, int64_t workspace_bytes, void *stream);
 
+/*
+ * Reverse slot map of a symmetric neighbor list: rev [N][T][m] int32, for a
+ * used entry s of atom i naming j with image sh, the index of the entry of
+ * j (bucket of i's type) naming i with image -sh; -1 on unused entries.
+ * status receives 1 + an atom whose partner entry is missing. No reference
+ * counterpart (the reference scatters with atomics, calculateForce.cu:92).
+ */
+int matpl_dp_neighbor_reverse(const int32_t *list, const int32_t *shift, const int32_t *counts,
+                              const int32_t *type_map, const int64_t *frame_start, const int32_t *atom_frame,
+                              int ntypes, int m, int64_t n, int32_t *rev, int32_t *status, void *stream);
+
 /* ------------------------------------------------- DP environment matrix */
 /*
  * Ri [B][N][S][4] and Ri_d [B][N][S][4][3] from ImageDR [B][N][S][4]
@@ -145,9 +156,13 @@ int matpl_dp_calc_stat(const int32_t *list, const double *dR, const int32_t *typ
  * matrix cores (bf16 != 0: BF16 in / FP32 accumulate; 0: FP32), and
  * A[i] = scale * (sum Ri^T G + empty-slot term) [N][4][E], E in {16,25,32}.
  * vec/mat/tinfo are the packed net parameters (matpl_amd/dp_fused.py).
- * Backward: given dA = dEtot/dA, force [N][3] (float, accumulated: zero it
- * first) receives -dE/dx and virial [F][9] (float64, accumulated) the
- * calculateVirial-convention virial.
+ * Backward: given dA = dEtot/dA, force [N][3] (float) receives -dE/dx and
+ * virial [F][9] (float64, accumulated: zero it first) the
+ * calculateVirial-convention virial. With rev (from
+ * matpl_dp_neighbor_reverse) the neighbor forces are gathered without
+ * atomics through workspaces fslot [N][T][mnl][4] float (16-byte aligned)
+ * and fcenter [N][3] float, and force is overwritten; with rev == NULL they
+ * are added with atomics and force must be zeroed first.
  * Together they replace DP.calculate_Ri + calculate_xyz_scater + the
  * autograd dE/dRi + calculateForce/calculateVirial of the reference
  * (src/model/dp_dp.py:217-469, src/model/dp_embedding.py:40-94); the fitting
@@ -164,8 +179,22 @@ int matpl_dp_fused_backward(const double *pos, const double *lattice, const int3
                             const int64_t *frame_start, const int32_t *list, const int32_t *shift,
                             const int32_t *counts, const int32_t *type_map, const float *vec, const void *mat,
                             const float *tinfo, int64_t natoms, int ntypes, int mnl, int m, int E, int bf16,
-                            double rcut, double rcut_smooth, double scale, const float *dA, float *force,
-                            double *virial, int32_t *status, void *stream);
+                            double rcut, double rcut_smooth, double scale, const float *dA, const int32_t *rev,
+                            float *fslot, float *fcenter, float *force, double *virial, int32_t *status,
+                            void *stream);
+
+/* ------------------------------------------------------------ MD helpers */
+/*
+ * Velocity Verlet for MD steps driven by the fused path (units A, fs, eV,
+ * amu): v += dt/2 F/m, and with drift != 0 also x += dt v. v, x [n][3]
+ * float64, F [n][3] float, inv_mass [n] float64 (1/amu). No reference
+ * counterpart: in the reference the MD integrator is LAMMPS's, which calls
+ * the model through PairMATPL::compute (src/lmps/lammps-libtorch/MATPL/pair_matpl.cpp:864).
+ */
+int matpl_md_kick(double *v, double *x, const float *F, const double *inv_mass, double dt, int64_t n, int drift,
+                  void *stream);
+/* out[i] = in[i] . M for row vectors (M host [9]); Cartesian <-> fractional. */
+int matpl_md_affine(const double *in, double *out, const double *M_host, int64_t n, void *stream);
 
 #ifdef __cplusplus
 }

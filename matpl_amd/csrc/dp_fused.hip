@@ -18,9 +18,14 @@
 //   bwd : given dA = dE/dA, recomputes the embedding forward for each tile,
 //         back-propagates through the net (same matrix-core layout), forms
 //         dE/dRi per neighbor, contracts it with the analytic dRi/dx
-//         (dp_dp.py:266-299) and scatters the force (FP32 atomics at j,
-//         wave-reduced centre term at i) and the virial
-//         V[a][c] += f_a dR_c (FP64 per frame; calculateVirial.cu:80).
+//         (dp_dp.py:266-299) and produces the virial V[a][c] += f_a dR_c
+//         (FP64 per frame; calculateVirial.cu:80) and the forces. With a
+//         reverse-slot map (neighbor list symmetric) each slot's neighbor
+//         force is stored in its list slot and gather_kernel sums, for every
+//         atom, its centre term and the slots that point back at it, in a
+//         fixed order and without atomics (device-scope atomics leave the
+//         per-XCD L2 on this part and throttle the scatter); without one,
+//         FP32 atomics at j are used.
 //
 // Matrix-core layout. Activations live as [feature x neighbor] tiles in the
 // 32x32 accumulator layout of v_mfma_f32_32x32x16_bf16: lane l holds
@@ -84,6 +89,8 @@ struct FusedParams {
     float *A;                   // [N][4][E] (fwd)
     float *force;               // [N][3] (bwd, accumulated)
     double *virial;             // [F][9] (bwd, accumulated)
+    float4 *fslot;              // [N][T][mnl] per-slot neighbor force (bwd, gather mode)
+    float *fcenter;             // [N][3] centre term (bwd, gather mode)
     int32_t *status;            // 1 + first atom whose in-cutoff count exceeds m
     int64_t natoms;
     int T, mnl, m;
@@ -210,10 +217,12 @@ __device__ __forceinline__ int64_t neighbor_disp(const FusedParams &p, const Ato
     return j;
 }
 
-// Compacts the in-cutoff neighbors of atom i into LDS: nb[t*mnl + k] =
-// (dx, dy, dz, j as float bits); nv[t] = count. Returns false on overflow.
+// Compacts the in-cutoff neighbors of atom i into LDS: nb[t*mnl + c] =
+// (dx, dy, dz, bucket slot k as float bits); nv[t] = count. Used bucket
+// entries beyond the cutoff get a zero per-slot force when fslot is given.
+// Returns false on overflow.
 __device__ __forceinline__ bool compact(const FusedParams &p, const AtomGeo &g, int64_t i, int lane, float4 *nb,
-                                        int *nv) {
+                                        int *nv, float4 *fslot) {
     bool ok_all = true;
     for (int t = 0; t < p.T; t++) {
         const int cnt = p.counts[i * p.T + t];
@@ -224,10 +233,12 @@ __device__ __forceinline__ bool compact(const FusedParams &p, const AtomGeo &g, 
             float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
             if (k < cnt) {
                 double dx, dy, dz;
-                const int64_t j = neighbor_disp(p, g, (i * p.T + t) * (int64_t)p.mnl + k, dx, dy, dz);
+                const int64_t slot = (i * p.T + t) * (int64_t)p.mnl + k;
+                neighbor_disp(p, g, slot, dx, dy, dz);
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 ok = r2 < p.rc2 && r2 > 1e-10;
-                e = make_float4((float)dx, (float)dy, (float)dz, __int_as_float((int)j));
+                e = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(k));
+                if (!ok && fslot) fslot[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
             const uint64_t bal = __ballot(ok);
             const int idx = n + __popcll(bal & ((1ull << lane) - 1ull));
@@ -273,7 +284,7 @@ dp_fused_fwd_kernel(FusedParams p) {
         const int ti = p.type_map[i];
         const float *tinfo = p.tinfo + ti * NTI;
         const AtomGeo g = atom_geo(p, i);
-        if (!compact(p, g, i, lane, nb, nvs) && lane == 0) atomicCAS(p.status, 0, (int)(i + 1));
+        if (!compact(p, g, i, lane, nb, nvs, nullptr) && lane == 0) atomicCAS(p.status, 0, (int)(i + 1));
         __builtin_amdgcn_wave_barrier();
         float Aacc[4][16];
 #pragma unroll
@@ -360,7 +371,7 @@ dp_fused_bwd_kernel(FusedParams p) {
             }
             vframe = frame;
         }
-        compact(p, g, i, lane, nb, nvs);
+        compact(p, g, i, lane, nb, nvs, p.fslot);
         __builtin_amdgcn_wave_barrier();
         // dA for this lane's features, with the 1/(m*type_nums) scale folded in
         float dAl[4][16];
@@ -427,10 +438,16 @@ dp_fused_bwd_kernel(FusedParams p) {
                     float f[3];
 #pragma unroll
                     for (int a = 0; a < 3; a++) f[a] = -(x[a] * cf + gw[a + 1] * s * ir);
-                    const int j = __float_as_int(e.w);
-                    atomic_add(p.force + 3 * (int64_t)j, f[0]);
-                    atomic_add(p.force + 3 * (int64_t)j + 1, f[1]);
-                    atomic_add(p.force + 3 * (int64_t)j + 2, f[2]);
+                    const int64_t slot = (i * p.T + t) * (int64_t)p.mnl + __float_as_int(e.w);
+                    if (p.fslot) {
+                        // gather mode: the neighbor term is summed later through the reverse slot map
+                        p.fslot[slot] = make_float4(f[0], f[1], f[2], 0.f);
+                    } else {
+                        const int64_t j = g.fbase + p.list[slot] - 1;
+                        atomic_add(p.force + 3 * j, f[0]);
+                        atomic_add(p.force + 3 * j + 1, f[1]);
+                        atomic_add(p.force + 3 * j + 2, f[2]);
+                    }
 #pragma unroll
                     for (int a = 0; a < 3; a++) {
                         fc[a] -= f[a];
@@ -443,15 +460,64 @@ dp_fused_bwd_kernel(FusedParams p) {
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             const float x = wave_sum(fc[a]);
-            if (lane == 0) atomic_add(p.force + 3 * i + a, x);
+            if (lane == 0) {
+                if (p.fcenter) p.fcenter[3 * i + a] = x;
+                else atomic_add(p.force + 3 * i + a, x);
+            }
         }
         __builtin_amdgcn_wave_barrier();
     }
-    if (vframe >= 0) {
+    // virial: wave sums -> one set of atomics per block and frame
+    __shared__ double vred[FUSED_WAVES][9];
+    __shared__ int vfr[FUSED_WAVES];
 #pragma unroll
-        for (int v = 0; v < 9; v++) {
-            double x = wave_sum(vir[v]);
-            if (lane == 0 && x != 0.0) atomic_add(p.virial + 9 * vframe + v, x);
+    for (int v = 0; v < 9; v++) {
+        const double x = wave_sum(vir[v]);
+        if (lane == 0) vred[wid][v] = x;
+    }
+    if (lane == 0) vfr[wid] = vframe;
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        const int v = threadIdx.x;
+        double acc = 0.0;
+        int fr = -1;
+        for (int w = 0; w < FUSED_WAVES; w++) {
+            if (vfr[w] < 0) continue;
+            if (vfr[w] != fr) {
+                if (fr >= 0 && acc != 0.0) atomic_add(p.virial + 9 * fr + v, acc);
+                fr = vfr[w];
+                acc = 0.0;
+            }
+            acc += vred[w][v];
+        }
+        if (fr >= 0 && acc != 0.0) atomic_add(p.virial + 9 * fr + v, acc);
+    }
+}
+
+// force[i] = fcenter[i] + sum over used slots s of i of fslot[rev[s]]; one
+// wavefront per atom, fixed reduction order (deterministic).
+__global__ void __launch_bounds__(256)
+gather_kernel(const int32_t *__restrict__ rev, const int32_t *__restrict__ counts, const float4 *__restrict__ fslot,
+              const float *__restrict__ fcenter, float *__restrict__ force, int T, int mnl, int64_t n) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int64_t i = (int64_t)blockIdx.x * 4 + threadIdx.x / WAVE; i < n; i += (int64_t)gridDim.x * 4) {
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        for (int t = 0; t < T; t++) {
+            const int cnt = counts[i * T + t];
+            const int32_t *rv = rev + (i * T + t) * (int64_t)mnl;
+            for (int k = lane; k < cnt; k += WAVE) {
+                const int r = rv[k];
+                if (r >= 0) {
+                    const float4 f = fslot[r];
+                    a0 += f.x; a1 += f.y; a2 += f.z;
+                }
+            }
+        }
+        a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
+        if (lane == 0) {
+            force[3 * i] = fcenter[3 * i] + a0;
+            force[3 * i + 1] = fcenter[3 * i + 1] + a1;
+            force[3 * i + 2] = fcenter[3 * i + 2] + a2;
         }
     }
 }
@@ -460,8 +526,9 @@ template <int E, bool BF16>
 int launch(FusedParams &p, bool fwd, hipStream_t st) {
     const size_t lds = sizeof(float4) * FUSED_WAVES * (size_t)p.T * p.mnl + sizeof(int) * FUSED_WAVES * MAXT;
     MATPL_CHECK_ARG(lds <= 64 * 1024, "dp_fused: T*mnl too large for LDS (%zu bytes)", lds);
+    // persistent-style grid: a few blocks per CU, each strides over atoms
     int64_t blocks = (p.natoms + FUSED_WAVES - 1) / FUSED_WAVES;
-    if (blocks > 65535) blocks = 65535;
+    if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
     if (fwd) {
         hipLaunchKernelGGL((dp_fused_fwd_kernel<E, BF16>), dim3((unsigned)blocks), dim3(FUSED_WAVES * WAVE), lds, st, p);
@@ -510,8 +577,8 @@ MATPL_EXPORT int matpl_dp_fused_backward(const double *pos, const double *lattic
                                          const int32_t *counts, const int32_t *type_map, const float *vec,
                                          const void *mat, const float *tinfo, int64_t natoms, int ntypes, int mnl,
                                          int m, int E, int bf16, double rcut, double rcut_smooth, double scale,
-                                         const float *dA, float *force, double *virial, int32_t *status,
-                                         void *stream) {
+                                         const float *dA, const int32_t *rev, float *fslot, float *fcenter,
+                                         float *force, double *virial, int32_t *status, void *stream) {
     FusedParams p{};
     p.pos = pos; p.lattice = lattice; p.atom_frame = atom_frame; p.frame_start = frame_start;
     p.list = list; p.shift = shift; p.counts = counts; p.type_map = type_map;
@@ -519,5 +586,14 @@ MATPL_EXPORT int matpl_dp_fused_backward(const double *pos, const double *lattic
     p.natoms = natoms; p.T = ntypes; p.mnl = mnl; p.m = m;
     p.rc2 = rcut * rcut; p.rmin = (float)rcut_smooth; p.rmax = (float)rcut; p.scale = (float)scale;
     MATPL_CHECK_ARG(natoms == 0 || (dA && force && virial), "dp_fused_backward: null output");
-    return dispatch(p, E, bf16, false, (hipStream_t)stream);
+    if (rev) {
+        MATPL_CHECK_ARG(fslot && fcenter && aligned16(fslot), "dp_fused_backward: gather mode needs fslot/fcenter");
+        p.fslot = reinterpret_cast<float4 *>(fslot);
+        p.fcenter = fcenter;
+    }
+    int rc = dispatch(p, E, bf16, false, (hipStream_t)stream);
+    if (rc || !rev || natoms == 0) return rc;
+    hipLaunchKernelGGL(gather_kernel, dim3(grid_cap(natoms, 4)), dim3(256), 0, (hipStream_t)stream, rev, counts,
+                       p.fslot, fcenter, force, ntypes, mnl, natoms);
+    return check_launch("gather_kernel");
 }

@@ -337,6 +337,47 @@ cell_kernel(Lattice lat, Grid g, const double *__restrict__ fw, const int32_t *_
     }
 }
 
+// Reverse slot of every used bucket entry: for s = (i, t, k) naming
+// neighbor j with image shift sh, rev[s] is the entry of j's bucket of type
+// t_i that names i with image -sh. The list is symmetric because dR_ji is the
+// exact negation of dR_ij in the cutoff test; a missing partner (per-type
+// cutoffs that differ, or a tie in the nearest-image rule) sets status.
+__global__ void reverse_kernel(const int32_t *__restrict__ list, const int32_t *__restrict__ shift,
+                               const int32_t *__restrict__ counts, const int32_t *__restrict__ type_map,
+                               const int64_t *__restrict__ frame_start, const int32_t *__restrict__ atom_frame, int T,
+                               int m, int64_t n, int32_t *__restrict__ rev, int32_t *__restrict__ status) {
+    const int64_t nslots = n * T * m;
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nslots;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = s / ((int64_t)T * m);
+        const int t = (int)((s / m) % T);
+        const int k = (int)(s % m);
+        if (k >= counts[i * T + t]) { rev[s] = -1; continue; }
+        const int64_t fb = frame_start ? frame_start[atom_frame[i]] : 0;
+        const int64_t j = fb + list[s] - 1;
+        const int ti = type_map[i];
+        const int target = (int)(i - fb) + 1;
+        const int sh = shift[s];
+        const int want = pack_shift(-((sh & 255) - 128), -(((sh >> 8) & 255) - 128), -(((sh >> 16) & 255) - 128));
+        const int64_t base = (j * T + ti) * (int64_t)m;
+        const int cnt = counts[j * T + ti];
+        int lo = 0, hi = cnt;          // first entry with list >= target (buckets are sorted)
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (list[base + mid] < target) lo = mid + 1; else hi = mid;
+        }
+        int found = -1;
+        for (int q = lo; q < cnt && list[base + q] == target; q++)
+            if (shift[base + q] == want) { found = q; break; }
+        if (found < 0) {
+            rev[s] = -1;
+            atomicCAS(status, 0, (int)(i + 1));
+        } else {
+            rev[s] = (int32_t)(base + found);
+        }
+    }
+}
+
 bool small_box_host(const double *L, double rc) {
     for (int d = 0; d < 3; d++) {
         const double len = sqrt(L[3 * d] * L[3 * d] + L[3 * d + 1] * L[3 * d + 1] + L[3 * d + 2] * L[3 * d + 2]);
@@ -450,4 +491,19 @@ MATPL_EXPORT int matpl_dp_neighbor_cell(const double *lattice_host, const double
                        frac_wrapped, atom_cell, cell_start, cell_atoms, type_map, rc_type, ntypes, m, rc_m, n, list,
                        dR, counts, shift, status);
     return check_launch("cell_kernel");
+}
+
+MATPL_EXPORT int matpl_dp_neighbor_reverse(const int32_t *list, const int32_t *shift, const int32_t *counts,
+                                           const int32_t *type_map, const int64_t *frame_start,
+                                           const int32_t *atom_frame, int ntypes, int m, int64_t n, int32_t *rev,
+                                           int32_t *status, void *stream) {
+    MATPL_CHECK_ARG(ntypes >= 1 && m >= 1 && n >= 0, "dp_neighbor_reverse: bad sizes");
+    MATPL_CHECK_ARG(n * ntypes * (int64_t)m < (int64_t)0x7fffffff, "dp_neighbor_reverse: too many slots for int32");
+    if (n == 0) return OK;
+    MATPL_CHECK_ARG(list && shift && counts && type_map && rev && status, "dp_neighbor_reverse: null pointer");
+    MATPL_CHECK_ARG((frame_start == nullptr) == (atom_frame == nullptr),
+                    "dp_neighbor_reverse: frame_start and atom_frame go together");
+    hipLaunchKernelGGL(reverse_kernel, dim3(grid_cap(n * ntypes * m)), dim3(256), 0, (hipStream_t)stream, list, shift,
+                       counts, type_map, frame_start, atom_frame, ntypes, m, n, rev, status);
+    return check_launch("reverse_kernel");
 }

@@ -159,10 +159,17 @@ class FusedDP:
         self.fw = pack_fused(w, precision, self.device)
         self.fit = _Fitting(w, self.device)
         self.precision = precision
+        self.events = None      # when a list: (start, end) HIP events around each backward launch
 
     def __call__(self, pos: torch.Tensor, lattice: torch.Tensor, type_map: torch.Tensor, nl_list: torch.Tensor,
                  nl_shift: torch.Tensor, nl_counts: torch.Tensor, mnl: int, atom_frame: Optional[torch.Tensor] = None,
-                 frame_start: Optional[torch.Tensor] = None, nframes: int = 1, type_ranges=None):
+                 frame_start: Optional[torch.Tensor] = None, nframes: int = 1, type_ranges=None,
+                 rev: Optional[torch.Tensor] = None):
+        """Returns (Ei [N], F [N,3] float32, V [nframes, 9] float64, status).
+
+        With ``rev`` (neighbor.reverse_map) the neighbor forces are gathered
+        without atomics; otherwise they are scattered with atomics.
+        """
         cfg = self.cfg
         N = pos.shape[0]
         T = cfg.ntypes
@@ -180,7 +187,21 @@ class FusedDP:
         _lib.call("matpl_dp_fused_forward", *args, _ptr(A), _ptr(status), _stream())
         Ei, dA = self.fit(A, type_map, type_ranges)
         dA = dA.contiguous()
-        F = torch.zeros((N, 3), dtype=torch.float32, device=dev)
         V = torch.zeros((nframes, 9), dtype=torch.float64, device=dev)
-        _lib.call("matpl_dp_fused_backward", *args, _ptr(dA), _ptr(F), _ptr(V), _ptr(status), _stream())
+        if rev is not None:
+            F = torch.empty((N, 3), dtype=torch.float32, device=dev)
+            fslot = torch.empty((N * T * mnl, 4), dtype=torch.float32, device=dev)
+            fcenter = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        else:
+            F = torch.zeros((N, 3), dtype=torch.float32, device=dev)
+            fslot = fcenter = None
+        if self.events is not None:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+        _lib.call("matpl_dp_fused_backward", *args, _ptr(dA), _ptr(rev), _ptr(fslot), _ptr(fcenter), _ptr(F), _ptr(V),
+                  _ptr(status), _stream())
+        if self.events is not None:
+            e1.record()
+            self.events.append((e0, e1))
         return Ei, F, V, status

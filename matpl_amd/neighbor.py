@@ -106,6 +106,21 @@ def find_neighbore_batch(lattices, fracs: List, type_maps: List, ntypes: int, m:
     return out.check() if check else out
 
 
+def reverse_map(nl: NeighborList, type_map: torch.Tensor, m: int, frame_start=None, atom_frame=None) -> torch.Tensor:
+    """rev [N, T*m]: for each used entry, the index of its partner entry (j -> i, opposite image).
+
+    Raises if the list is not symmetric (syncs); the fused force gather needs it.
+    """
+    n, S = nl.list.shape
+    T = S // m
+    rev = torch.empty((n, S), dtype=torch.int32, device=nl.list.device)
+    status = torch.zeros(1, dtype=torch.int32, device=nl.list.device)
+    tm = type_map.to(torch.int32).contiguous()
+    _lib.call("matpl_dp_neighbor_reverse", _ptr(nl.list), _ptr(nl.shift), _ptr(nl.counts), _ptr(tm),
+              _ptr(frame_start), _ptr(atom_frame), T, m, n, _ptr(rev), _ptr(status), _stream())
+    return rev, status
+
+
 def image_dr(lst: torch.Tensor, dR: torch.Tensor) -> torch.Tensor:
     """ImageDR = [|dR|, dx, dy, dz], |dR| only where list > 0 (dpuni_data_loader.py:669-673)."""
     r = torch.sqrt((dR * dR).sum(-1)) * (lst > 0)

@@ -12,7 +12,7 @@ import oracle
 from matpl_amd import structures as st
 from matpl_amd.dp_fused import FusedDP
 from matpl_amd.dpnet import DPConfig, DPWeights
-from matpl_amd.neighbor import find_neighbore_gpu
+from matpl_amd.neighbor import find_neighbore_gpu, reverse_map
 
 pytestmark = pytest.mark.gpu
 
@@ -24,7 +24,7 @@ def _setup(frame, T, m, seed):
     return DPWeights.random(cfg, seed=seed, davg=davg, dstd=dstd)
 
 
-def _fused(frame, w, precision, dev, rc_nl=None, mnl=None):
+def _fused(frame, w, precision, dev, rc_nl=None, mnl=None, gather=True):
     cfg = w.cfg
     rc_nl = rc_nl or cfg.rcut
     mnl = mnl or cfg.m
@@ -34,7 +34,11 @@ def _fused(frame, w, precision, dev, rc_nl=None, mnl=None):
     pos = (nl.frac_wrapped @ L).contiguous()
     model = FusedDP(w, precision, dev)
     tm = torch.as_tensor(frame.types, device=dev)
-    Ei, F, V, status = model(pos, L.reshape(1, 9).contiguous(), tm, nl.list, nl.shift, nl.counts, mnl)
+    rev = None
+    if gather:
+        rev, rst = reverse_map(nl, tm, mnl)
+        assert int(rst.item()) == 0
+    Ei, F, V, status = model(pos, L.reshape(1, 9).contiguous(), tm, nl.list, nl.shift, nl.counts, mnl, rev=rev)
     torch.cuda.synchronize()
     assert int(status.item()) == 0
     return Ei.double().cpu().numpy(), F.double().cpu().numpy(), V[0].cpu().numpy().reshape(3, 3)
@@ -95,3 +99,29 @@ def test_fused_overflow_flag(dev):
     _, _, _, status = model((nl.frac_wrapped @ L).contiguous(), L.reshape(1, 9).contiguous(),
                             torch.as_tensor(f.types, device=dev), nl.list, nl.shift, nl.counts, 120)
     assert int(status.item()) != 0
+
+
+@pytest.mark.parametrize("name", ["cu108", "hfo2_324"])
+def test_reverse_map_is_involution(dev, name):
+    maker, T, m = CASES[name]
+    f = maker()
+    nl = find_neighbore_gpu(f.lattice, f.frac(), f.types, T, m, 6.0, device=dev, want_dR=True, want_shift=True)
+    rev, st_ = reverse_map(nl, torch.as_tensor(f.types, device=dev), m)
+    assert int(st_.item()) == 0
+    r = rev.reshape(-1).cpu().numpy()
+    used = np.nonzero(r >= 0)[0]
+    assert len(used) == int((nl.list > 0).sum().item())
+    np.testing.assert_array_equal(r[r[used]], used)                 # rev(rev(s)) = s
+    dR = nl.dR.reshape(-1, 3).cpu().numpy()
+    np.testing.assert_array_equal(dR[r[used]], -dR[used])             # partner has the opposite displacement
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_gather_equals_atomic_scatter(dev, precision):
+    f = st.fluorite_hfo2((4, 4, 4), seed=8)
+    w = _setup(f, 2, 120, seed=9)
+    Ei0, F0, V0 = _fused(f, w, precision, dev, gather=False)
+    Ei1, F1, V1 = _fused(f, w, precision, dev, gather=True)
+    np.testing.assert_allclose(Ei1, Ei0, atol=0)
+    np.testing.assert_allclose(F1, F0, atol=2e-6 * max(1.0, np.abs(F0).max()))
+    np.testing.assert_allclose(V1, V0, rtol=1e-9, atol=1e-9)
