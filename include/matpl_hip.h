@@ -183,6 +183,22 @@ int matpl_dp_fused_backward(const double *pos, const double *lattice, const int3
                             float *fslot, float *fcenter, float *force, double *virial, int32_t *status,
                             void *stream);
 
+/*
+ * Fitting net of the fused path on the matrix cores: A [N][4][E] ->
+ * Ei [N] and dA = dEtot/dA [N][4][E], per tile of 32 atoms of one centre
+ * type (tiles [ntiles][3] = type, first atom, count). DR = A^T A[:, :16]
+ * is formed on the fly (M2 = 16, E <= 32); hidden width <= 64, nlayers
+ * (1..4) tanh layers, the first plain and the rest residual with resnet_dt,
+ * then a linear output. Weights packed by matpl_amd/dp_fitting.py (bf16 or
+ * fp32 per `bf16`). Replaces calculate_Ei / FittingNet.forward and its
+ * autograd backward (src/model/dp_dp.py:304-415,
+ * src/model/dp_embedding.py:96-163).
+ */
+int matpl_dp_fitting(const float *A, const int32_t *tiles, int ntiles, const void *w0f, const void *w0b,
+                     const void *whf, const void *whb, const float *vec, int64_t type_stride_w0f,
+                     int64_t type_stride_w0b, int64_t type_stride_wh, int64_t type_stride_vec, int E, int nlayers,
+                     int bf16, float *Ei, float *dA, void *stream);
+
 /* ------------------------------------------------------------ MD helpers */
 /*
  * Velocity Verlet for MD steps driven by the fused path (units A, fs, eV,

@@ -1,0 +1,313 @@
+// DP fitting net on the matrix cores: A [N][4][E] -> Ei [N] and dEi/dA.
+//
+// Reference: DP.calculate_Ei / calculate_xyz_scater (src/model/dp_dp.py:304-415)
+// and FittingNet.forward (src/model/dp_embedding.py:140-163):
+//   DR[p*M2 + q] = sum_w A[w][p] A[w][q]        (p < E, q < M2)
+//   x1 = tanh(DR W0 + b0);  x_{l+1} = x_l + rdt_l * tanh(x_l W_l + b_l)  (l >= 1)
+//   Ei = x_L . w_out + b_out
+// and the backward dEi/dA of that chain (the reference gets it from autograd).
+//
+// One wavefront owns a tile of 32 atoms of one centre type. Activations are
+// [feature x atom] tiles in the 32x32 accumulator layout (lane l: atom
+// n = l % 32, register r of block b: feature 32 b + f(r, h), f(r,h) =
+// 8 (r/4) + 4 h + r%4, h = l / 32), so a layer's output registers are the next
+// layer's B operand with no data movement; hidden widths are padded to 64
+// (two blocks). DR is formed on the fly in the B-operand layout of the first
+// layer (K-step s of 16 = DR row p = s when M2 = 16), and dDR is contracted into
+// dA block by block without being stored.
+#include "common.h"
+
+namespace matpl {
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int E_MAX = 32;
+constexpr int M2 = 16;
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
+
+__device__ __forceinline__ constexpr int feat(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+
+// Per-type packed parameters (see matpl_amd/dp_fitting.py for the host packer).
+//   w0f  [2 ob][64 lanes][Dh]   first layer, lane's K indices in order
+//   w0b  [nb0 blk][64 lanes][32] first layer backward (rows = DR index)
+//   whf  [L-1][2 ob][64][32]    hidden layers forward
+//   whb  [L-1][2 ob][64][32]    hidden layers backward
+//   vec  [L][3 (b, rdt, -)][2 blk][2 h][16] + out weights [2][2][16] + b_out
+struct FitParams {
+    const float *A;          // [N][4][E]
+    float *dA;               // [N][4][E]
+    float *Ei;               // [N]
+    const int32_t *tiles;    // [ntiles][3] = (type, first atom, count)
+    const void *w0f, *w0b, *whf, *whb;
+    const float *vec;
+    int64_t type_stride_w0f, type_stride_w0b, type_stride_wh, type_stride_vec;  // elements per type
+    int ntiles, E, D, Dh, nb0, L;
+};
+
+template <bool BF16>
+__device__ __forceinline__ void mma(floatx16 &acc, const void *w, int idx, const float *x) {
+    // idx: chunk index; BF16: 8 values per chunk, FP32: 1 value per chunk
+    if constexpr (BF16) {
+        const bf16x8 a = reinterpret_cast<const bf16x8 *>(w)[idx];
+        bf16x8 b;
+#pragma unroll
+        for (int q = 0; q < 8; q++) b[q] = (__bf16)x[q];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    } else {
+        const float a = reinterpret_cast<const float *>(w)[idx];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, x[0], acc, 0, 0, 0);
+    }
+}
+
+// out[ob] (+)= W^T x for a 64 -> 64 layer given the packed [2 ob][64][32] array.
+template <bool BF16>
+__device__ __forceinline__ void layer64(const void *w, int lane, const float x[2][16], floatx16 out[2]) {
+    constexpr int CH = BF16 ? 4 : 32;          // chunks per 64-wide input
+    constexpr int PER = BF16 ? 8 : 1;
+#pragma unroll
+    for (int ob = 0; ob < 2; ob++) {
+        const char *base = reinterpret_cast<const char *>(w) + ((size_t)ob * 64 + lane) * 32 * (BF16 ? 2 : 4);
+#pragma unroll
+        for (int c = 0; c < CH; c++) {
+            const int v = c * PER;                // first input register of the chunk
+            mma<BF16>(out[ob], base, c, &x[v / 16][v % 16]);
+        }
+    }
+}
+
+__device__ __forceinline__ const float *vptr(const float *vec, int layer, int which, int blk, int h) {
+    return vec + ((((size_t)layer * 3 + which) * 2 + blk) * 2 + h) * 16;
+}
+
+template <bool BF16, int L>
+__global__ void __launch_bounds__(WAVE)
+dp_fitting_kernel(FitParams p) {
+    const int lane = threadIdx.x;
+    const int n = lane & 31, h = lane >> 5;
+    __shared__ float As[32][4][E_MAX];       // A of the tile's atoms
+    __shared__ float dAs[32][4][E_MAX];      // dA accumulation
+    for (int tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
+        const int t = p.tiles[3 * tile], a0 = p.tiles[3 * tile + 1], cnt = p.tiles[3 * tile + 2];
+        const int E = p.E;
+        // stage A and clear dA
+        for (int x = lane; x < 32 * 4 * E_MAX; x += WAVE) {
+            const int an = x / (4 * E_MAX), w = (x / E_MAX) % 4, c = x % E_MAX;
+            As[an][w][c] = (an < cnt && c < E) ? p.A[((int64_t)(a0 + an) * 4 + w) * E + c] : 0.0f;
+            dAs[an][w][c] = 0.0f;
+        }
+        __syncthreads();
+        const void *w0f = reinterpret_cast<const char *>(p.w0f) + (size_t)t * p.type_stride_w0f * (BF16 ? 2 : 4);
+        const void *w0b = reinterpret_cast<const char *>(p.w0b) + (size_t)t * p.type_stride_w0b * (BF16 ? 2 : 4);
+        const char *whf = reinterpret_cast<const char *>(p.whf) + (size_t)t * p.type_stride_wh * (BF16 ? 2 : 4);
+        const char *whb = reinterpret_cast<const char *>(p.whb) + (size_t)t * p.type_stride_wh * (BF16 ? 2 : 4);
+        const float *vec = p.vec + (size_t)t * p.type_stride_vec;
+        const size_t lstride = (size_t)2 * 64 * 32 * (BF16 ? 2 : 4);
+
+        // ---- layer 0: x1 = tanh(DR W0 + b0)
+        floatx16 acc[2];
+#pragma unroll
+        for (int ob = 0; ob < 2; ob++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[ob][r] = vptr(vec, 0, 0, ob, h)[r];
+        if constexpr (BF16) {
+            // K-step s covers DR rows p = s (M2 = 16), this lane's 8 columns q = 8h..8h+7
+            float aq[4][8];
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+#pragma unroll
+                for (int q = 0; q < 8; q++) aq[w][q] = As[n][w][8 * h + q];
+            for (int s = 0; s < E; s++) {
+                float drv[8];
+                const float ap0 = As[n][0][s], ap1 = As[n][1][s], ap2 = As[n][2][s], ap3 = As[n][3][s];
+#pragma unroll
+                for (int q = 0; q < 8; q++) drv[q] = ap0 * aq[0][q] + ap1 * aq[1][q] + ap2 * aq[2][q] + ap3 * aq[3][q];
+#pragma unroll
+                for (int ob = 0; ob < 2; ob++)
+                    mma<true>(acc[ob], reinterpret_cast<const char *>(w0f) + ((size_t)ob * 64 + lane) * p.Dh * 2, s,
+                              drv);
+            }
+        } else {
+            // K-step s covers DR index k = 2 s + h
+            for (int s = 0; s < p.D / 2; s++) {
+                const int k = 2 * s + h;
+                const int pp = k / M2, qq = k % M2;
+                float drv = As[n][0][pp] * As[n][0][qq] + As[n][1][pp] * As[n][1][qq] +
+                            As[n][2][pp] * As[n][2][qq] + As[n][3][pp] * As[n][3][qq];
+#pragma unroll
+                for (int ob = 0; ob < 2; ob++)
+                    mma<false>(acc[ob], reinterpret_cast<const char *>(w0f) + ((size_t)ob * 64 + lane) * p.Dh * 4, s,
+                               &drv);
+            }
+        }
+        float tk[L][2][16];         // tanh outputs per layer (for the backward)
+        float x[2][16];
+#pragma unroll
+        for (int ob = 0; ob < 2; ob++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) { tk[0][ob][r] = tanhf(acc[ob][r]); x[ob][r] = tk[0][ob][r]; }
+        // ---- hidden residual layers
+#pragma unroll
+        for (int l = 1; l < L; l++) {
+#pragma unroll
+            for (int ob = 0; ob < 2; ob++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[ob][r] = vptr(vec, l, 0, ob, h)[r];
+            layer64<BF16>(whf + (l - 1) * lstride, lane, x, acc);
+#pragma unroll
+            for (int ob = 0; ob < 2; ob++) {
+                const float *rdt = vptr(vec, l, 1, ob, h);
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    tk[l][ob][r] = tanhf(acc[ob][r]);
+                    x[ob][r] += rdt[r] * tk[l][ob][r];
+                }
+            }
+        }
+        // ---- output
+        const float *wout = vec + (size_t)L * 3 * 2 * 2 * 16;        // [2 blk][2 h][16]
+        const float bout = wout[2 * 2 * 16];
+        float e = 0.0f;
+#pragma unroll
+        for (int ob = 0; ob < 2; ob++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) e += x[ob][r] * wout[(ob * 2 + h) * 16 + r];
+        e += __shfl_xor(e, 32, 64);
+        if (h == 0 && n < cnt) p.Ei[a0 + n] = e + bout;
+        // ---- backward: dx = w_out
+        float dx[2][16];
+#pragma unroll
+        for (int ob = 0; ob < 2; ob++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) dx[ob][r] = wout[(ob * 2 + h) * 16 + r];
+#pragma unroll
+        for (int l = L - 1; l >= 1; l--) {
+            float dz[2][16];
+#pragma unroll
+            for (int ob = 0; ob < 2; ob++) {
+                const float *rdt = vptr(vec, l, 1, ob, h);
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    dz[ob][r] = dx[ob][r] * rdt[r] * (1.0f - tk[l][ob][r] * tk[l][ob][r]);
+                    acc[ob][r] = dx[ob][r];
+                }
+            }
+            layer64<BF16>(whb + (l - 1) * lstride, lane, dz, acc);
+#pragma unroll
+            for (int ob = 0; ob < 2; ob++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) dx[ob][r] = acc[ob][r];
+        }
+        float dz0[2][16];
+#pragma unroll
+        for (int ob = 0; ob < 2; ob++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) dz0[ob][r] = dx[ob][r] * (1.0f - tk[0][ob][r] * tk[0][ob][r]);
+        // ---- layer 0 backward, block by block: dDR rows 32 blk + f(r,h)
+        //   k = p*16 + q: r < 8 -> p = 2 blk, q = f(r,h); r >= 8 -> p = 2 blk + 1, q = f(r-8,h)
+        float aq2[4][8];            // A[w][f(r,h)] for r < 8
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+#pragma unroll
+            for (int r = 0; r < 8; r++) aq2[w][r] = As[n][w][feat(r, h)];
+        float d2[4][8];             // dA[w][q] (second term), q = f(r,h)
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+#pragma unroll
+            for (int r = 0; r < 8; r++) d2[w][r] = 0.0f;
+        for (int blk = 0; blk < p.nb0; blk++) {
+            floatx16 g;
+#pragma unroll
+            for (int r = 0; r < 16; r++) g[r] = 0.0f;
+            const char *wb = reinterpret_cast<const char *>(w0b) + ((size_t)blk * 64 + lane) * 32 * (BF16 ? 2 : 4);
+            constexpr int CH = BF16 ? 4 : 32, PER = BF16 ? 8 : 1;
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                const int v = c * PER;
+                mma<BF16>(g, wb, c, &dz0[v / 16][v % 16]);
+            }
+            const int p0 = 2 * blk, p1 = 2 * blk + 1;
+            float s0[4] = {0.f, 0.f, 0.f, 0.f}, s1[4] = {0.f, 0.f, 0.f, 0.f};
+            const float ap0[4] = {p0 < E ? As[n][0][p0] : 0.f, p0 < E ? As[n][1][p0] : 0.f,
+                                  p0 < E ? As[n][2][p0] : 0.f, p0 < E ? As[n][3][p0] : 0.f};
+            const float ap1[4] = {p1 < E ? As[n][0][p1] : 0.f, p1 < E ? As[n][1][p1] : 0.f,
+                                  p1 < E ? As[n][2][p1] : 0.f, p1 < E ? As[n][3][p1] : 0.f};
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+#pragma unroll
+                for (int w = 0; w < 4; w++) {
+                    s0[w] += g[r] * aq2[w][r];
+                    s1[w] += g[r + 8] * aq2[w][r];
+                    d2[w][r] += g[r] * ap0[w] + g[r + 8] * ap1[w];
+                }
+            }
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                s0[w] += __shfl_xor(s0[w], 32, 64);
+                s1[w] += __shfl_xor(s1[w], 32, 64);
+            }
+            if (h == 0 && p0 < E) {
+#pragma unroll
+                for (int w = 0; w < 4; w++) dAs[n][w][p0] += s0[w];
+            }
+            if (h == 1 && p1 < E) {
+#pragma unroll
+                for (int w = 0; w < 4; w++) dAs[n][w][p1] += s1[w];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int q = feat(r, h);
+#pragma unroll
+            for (int w = 0; w < 4; w++) dAs[n][w][q] += d2[w][r];
+        }
+        __syncthreads();
+        for (int x2 = lane; x2 < 32 * 4 * E_MAX; x2 += WAVE) {
+            const int an = x2 / (4 * E_MAX), w = (x2 / E_MAX) % 4, c = x2 % E_MAX;
+            if (an < cnt && c < E) p.dA[((int64_t)(a0 + an) * 4 + w) * E + c] = dAs[an][w][c];
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+}  // namespace matpl
+
+using namespace matpl;
+
+MATPL_EXPORT int matpl_dp_fitting(const float *A, const int32_t *tiles, int ntiles, const void *w0f, const void *w0b,
+                                  const void *whf, const void *whb, const float *vec, int64_t type_stride_w0f,
+                                  int64_t type_stride_w0b, int64_t type_stride_wh, int64_t type_stride_vec, int E,
+                                  int nlayers, int bf16, float *Ei, float *dA, void *stream) {
+    MATPL_CHECK_ARG(E >= 1 && E <= E_MAX, "dp_fitting: E must be in [1,%d]", E_MAX);
+    MATPL_CHECK_ARG(nlayers >= 1 && nlayers <= 4, "dp_fitting: hidden layers must be in [1,4]");
+    if (ntiles <= 0) return OK;
+    MATPL_CHECK_ARG(A && tiles && w0f && w0b && vec && Ei && dA && (nlayers == 1 || (whf && whb)),
+                    "dp_fitting: null pointer");
+    FitParams p{};
+    p.A = A; p.dA = dA; p.Ei = Ei; p.tiles = tiles;
+    p.w0f = w0f; p.w0b = w0b; p.whf = whf; p.whb = whb; p.vec = vec;
+    p.type_stride_w0f = type_stride_w0f; p.type_stride_w0b = type_stride_w0b;
+    p.type_stride_wh = type_stride_wh; p.type_stride_vec = type_stride_vec;
+    p.ntiles = ntiles; p.E = E; p.D = E * M2; p.Dh = (E * M2) / 2; p.nb0 = (E * M2 + 31) / 32; p.L = nlayers;
+    const int grid = ntiles < 8192 ? ntiles : 8192;
+    hipStream_t st = (hipStream_t)stream;
+#define MATPL_FIT_CASE(LL)                                                                              \
+    case LL:                                                                                            \
+        if (bf16) hipLaunchKernelGGL((dp_fitting_kernel<true, LL>), dim3(grid), dim3(WAVE), 0, st, p);  \
+        else hipLaunchKernelGGL((dp_fitting_kernel<false, LL>), dim3(grid), dim3(WAVE), 0, st, p);      \
+        break;
+    switch (nlayers) {
+        MATPL_FIT_CASE(1)
+        MATPL_FIT_CASE(2)
+        MATPL_FIT_CASE(3)
+        MATPL_FIT_CASE(4)
+        default:
+            set_error("dp_fitting: %d hidden layers not compiled (1..4)", nlayers);
+            return ERR_ARGUMENT;
+    }
+#undef MATPL_FIT_CASE
+    return check_launch("dp_fitting_kernel");
+}

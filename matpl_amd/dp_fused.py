@@ -23,6 +23,7 @@ import torch
 
 from . import _lib
 from .calcops import _ptr, _stream
+from .dp_fitting import FittingMFMA
 from .dpnet import DPWeights
 
 NVEC, NMAT, NTI = 7, 4, 16
@@ -152,12 +153,15 @@ class FusedDP:
     Atoms must be grouped by type (type ranges are contiguous).
     """
 
-    def __init__(self, w: DPWeights, precision: str = "bf16", device="cuda"):
+    def __init__(self, w: DPWeights, precision: str = "bf16", device="cuda", fitting: str = "mfma"):
+        """fitting: "mfma" (hand-written matrix-core kernel) or "torch" (FP32 torch + autograd)."""
         self.w = w
         self.cfg = w.cfg
         self.device = torch.device(device)
         self.fw = pack_fused(w, precision, self.device)
         self.fit = _Fitting(w, self.device)
+        self.fitting = fitting
+        self._mfma_fit = {}
         self.precision = precision
         self.events = None      # when a list: (start, end) HIP events around each backward launch
 
@@ -185,7 +189,13 @@ class FusedDP:
                 _ptr(nl_counts), _ptr(type_map), _ptr(self.fw.vec), _ptr(self.fw.mat), _ptr(self.fw.tinfo), N, T, mnl,
                 cfg.m, self.fw.E, int(self.fw.bf16), float(cfg.rcut), float(cfg.rcut_smooth), scale)
         _lib.call("matpl_dp_fused_forward", *args, _ptr(A), _ptr(status), _stream())
-        Ei, dA = self.fit(A, type_map, type_ranges)
+        if self.fitting == "mfma":
+            key = tuple(type_ranges)
+            if key not in self._mfma_fit:
+                self._mfma_fit[key] = FittingMFMA(self.w, type_ranges, self.precision, dev)
+            Ei, dA = self._mfma_fit[key](A)
+        else:
+            Ei, dA = self.fit(A, type_map, type_ranges)
         dA = dA.contiguous()
         V = torch.zeros((nframes, 9), dtype=torch.float64, device=dev)
         if rev is not None:

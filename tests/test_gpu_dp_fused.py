@@ -125,3 +125,21 @@ def test_gather_equals_atomic_scatter(dev, precision):
     np.testing.assert_allclose(Ei1, Ei0, atol=0)
     np.testing.assert_allclose(F1, F0, atol=2e-6 * max(1.0, np.abs(F0).max()))
     np.testing.assert_allclose(V1, V0, rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-6), ("bf16", 3e-3)])
+def test_fitting_kernel_matches_torch_autograd(dev, precision, tol):
+    """Matrix-core fitting kernel (Ei, dEtot/dA) vs the FP32 torch fitting net with autograd."""
+    from matpl_amd.dp_fitting import FittingMFMA
+    from matpl_amd.dp_fused import _Fitting
+    cfg = DPConfig(ntypes=2, m=120)
+    w = DPWeights.random(cfg, seed=31)
+    rng = np.random.default_rng(4)
+    N = 203                                              # ragged: tiles of 32 plus a partial tile per type
+    ranges = [(0, 77), (77, N)]
+    A = torch.as_tensor(rng.normal(size=(N, 4, 25)) * 0.3, dtype=torch.float32, device=dev)
+    Ei_ref, dA_ref = _Fitting(w, dev)(A, None, ranges)
+    Ei, dA = FittingMFMA(w, ranges, precision, dev)(A)
+    torch.cuda.synchronize()
+    assert torch.allclose(Ei, Ei_ref, atol=tol * float(Ei_ref.abs().max()), rtol=0)
+    assert torch.allclose(dA, dA_ref, atol=tol * float(dA_ref.abs().max()), rtol=0)
