@@ -199,6 +199,80 @@ int matpl_dp_fitting(const float *A, const int32_t *tiles, int ntiles, const voi
                      int64_t type_stride_w0b, int64_t type_stride_wh, int64_t type_stride_vec, int E, int nlayers,
                      int bf16, float *Ei, float *dA, void *stream);
 
+/* ------------------------------------------------------------ NEP operators */
+/*
+ * Neighbor search over replicated boxes (num_cell[f][3] replicas of the
+ * original cell, then the minimum image of the expanded box, as the
+ * reference's expand_box + gpu_find_neighbor_list). Frames are packed:
+ * frame_start[nframes+1], atom_frame[n]; box/box_orig [nframes][9] (box =
+ * expanded lattice), pos [n][3] Cartesian. mode 0: neighbor lists
+ * NL_r[n][maxr], NL_a[n][maxa] (global atom index, -1 padded), Ri [n][max][3
+ * or 4 with rij4] = [(r,) x, y, z], NN counts; *status = 1+atom if an atom
+ * has more than maxr/maxa neighbors. mode 1: NN_r/NN_a[n] counts only.
+ * mode 2: per-type counts NN_r/NN_a[n][ntypes] (type_map int64 [n]).
+ * Replaces launch_calculate_neighbor / launch_calculate_maxneigh
+ * (src/op/include/calculate_nepneighbor.h:1-42).
+ */
+int matpl_nep_neighbor(const double *box, const double *box_orig, const int64_t *num_cell,
+                       const int64_t *frame_start, const int32_t *atom_frame, const int64_t *type_map,
+                       const double *pos, double rc_r, double rc_a, int64_t maxr, int64_t maxa, int64_t n, int mode,
+                       int ntypes, int rij4, int64_t *NN_r, int64_t *NL_r, int64_t *NN_a, int64_t *NL_a, double *Ri_r,
+                       double *Ri_a, int32_t *status, void *stream);
+
+/*
+ * NEP features of one family and their derivatives; f64 selects double,
+ * else float. angular = 0: radial features q[n] = sum_j sum_k c[ti][tj][n][k]
+ * f_k(r_ij); angular = 1: angular features from s_nlm sums (lmax l3 of the
+ * 3-body, l4 > 0 adds the 4-body, l5 > 0 the 5-body term). c [T][T][nm][nb],
+ * Ri [N][S][4] = [r, x, y, z], NL [N][S] (-1 padded), atom_map [N];
+ * features written to feats[i*ld + col0 + ...].
+ *  mode 0: feats (+ s_save [N][nm*24] for angular);
+ *  mode 1: gRi [N][S][4] = dL/dRi and gc += dL/dc for gfeat = dL/dfeats;
+ *  mode 2: second order for gg = upstream grad of gRi: ggfeat [N][ld] and gc.
+ * Replaces launch_calculate_nepfeat / _grad / _secondgradout(_c2) and
+ * launch_calculate_nepmbfeat / _grad / _secondgradout(_c3)
+ * (src/op/include/calculate_nepfeat.h:46-175).
+ */
+int matpl_nep_feat(int f64, int angular, int mode, const void *c, const void *Ri, const int64_t *NL,
+                   const int64_t *atom_map, int64_t N, int S, int T, int nm, int nb, int ld, int col0, int l3, int l4,
+                   int l5, double rc, void *feats, void *s_save, const void *gfeat, const void *gg, void *gRi,
+                   void *gc, void *ggfeat, void *stream);
+
+/*
+ * Neighbor scatter F[NL[i][k]] += dE[i][k] . Ri_d[i][k] (a 4x3 contraction)
+ * into F [N][3], which holds the centre terms -sum_k dE[i][k] . Ri_d[i][k]
+ * on entry. dE [N][S][4], Ri_d [N][S][4][3]. Replaces
+ * launch_calculate_nepforce (src/op/include/calculate_nepfeat.h:1).
+ */
+int matpl_nep_force(int f64, const int64_t *NL, const void *dE, const void *Ri_d, void *F, int64_t N, int S,
+                    void *stream);
+
+/*
+ * Per-frame virial [nframes][9] and per-atom virial [N][9] from the same
+ * pair terms and Rij [N][S][3]; frames are atom ranges frame_start[f] ..
+ * frame_start[f+1]. Replaces launch_calculate_nepvirial
+ * (src/op/include/calculate_nepfeat.h:21).
+ */
+int matpl_nep_virial(int f64, const int64_t *NL, const void *dE, const void *Rij, const void *Ri_d,
+                     const int64_t *frame_start, int64_t nframes, int64_t N, int S, void *virial, void *atom_virial,
+                     void *stream);
+
+/*
+ * Backward of the whole force (centre + neighbor terms) with respect to dE:
+ * g[i][k] = Ri_d[i][k] . (gF[NL[i][k]] - gF[i]), g [N][S][4] from gF [N][3].
+ * Replaces launch_calculate_nepforce_grad (src/op/include/calculate_nepfeat.h:11).
+ */
+int matpl_nep_force_grad(int f64, const int64_t *NL, const void *Ri_d, const void *gF, void *g, int64_t N, int S,
+                         void *stream);
+
+/*
+ * Backward of matpl_nep_virial with respect to dE, with gV [nframes][9]
+ * applied to each atom's own frame (atom_frame [N]). Replaces
+ * launch_calculate_nepvirial_grad (src/op/include/calculate_nepfeat.h:35).
+ */
+int matpl_nep_virial_grad(int f64, const int64_t *NL, const void *Rij, const void *Ri_d, const void *gV,
+                          const int32_t *atom_frame, void *g, int64_t N, int S, void *stream);
+
 /* ------------------------------------------------------------ MD helpers */
 /*
  * Velocity Verlet for MD steps driven by the fused path (units A, fs, eV,

@@ -45,6 +45,13 @@ SIGNATURES = {
     "matpl_dp_fused_backward": (I32, [P] * 11 + [I64, I32, I32, I32, I32, I32, F64, F64, F64] + [P] * 8),
     "matpl_dp_neighbor_reverse": (I32, [P, P, P, P, P, P, I32, I32, I64, P, P, P]),
     "matpl_dp_fitting": (I32, [P, P, I32, P, P, P, P, P, I64, I64, I64, I64, I32, I32, I32, P, P, P]),
+    # NEP operators
+    "matpl_nep_neighbor": (I32, [P] * 7 + [F64, F64, I64, I64, I64, I32, I32, I32] + [P] * 8),
+    "matpl_nep_feat": (I32, [I32, I32, I32, P, P, P, P, I64] + [I32] * 10 + [F64] + [P] * 8),
+    "matpl_nep_force": (I32, [I32, P, P, P, P, I64, I32, P]),
+    "matpl_nep_virial": (I32, [I32, P, P, P, P, P, I64, I64, I32, P, P, P]),
+    "matpl_nep_force_grad": (I32, [I32, P, P, P, P, I64, I32, P]),
+    "matpl_nep_virial_grad": (I32, [I32, P, P, P, P, P, P, I64, I32, P]),
     # MD helpers
     "matpl_md_kick": (I32, [P, P, P, P, F64, I64, I32, P]),
     "matpl_md_affine": (I32, [P, P, P, I64, P]),

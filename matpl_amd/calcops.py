@@ -184,5 +184,26 @@ def register() -> None:
     lib.define("calculateVirial(Tensor list_neigh, Tensor dE, Tensor Rij, Tensor Ri_d, Tensor nghost_tensor) -> Tensor[]")
     lib.impl("calculateForce", calculateForce, "CompositeImplicitAutograd")
     lib.impl("calculateVirial", calculateVirial, "CompositeImplicitAutograd")
+    from . import nep_ops
+    # NEP schemas as bound in src/op/src/CalcOps_bind.cpp:10-16 (signatures src/op/include/CalcOps.h:168-318)
+    lib.define("calculateNepFeat(Tensor coeff2, Tensor d12_radial, Tensor NL_radial, Tensor atom_map, Tensor feats, "
+               "float rcut_radial, int feat_multi_nums) -> Tensor[]")
+    lib.define("calculateNepMbFeat(Tensor coeff3, Tensor d12_angluar, Tensor NL_angluar, Tensor atom_map, "
+               "Tensor feats, int feat_2b_num, int lmax_3, int lmax_4, int lmax_5, float rcut_angluar) -> Tensor[]")
+    lib.define("calculateNepForce(Tensor list_neigh, Tensor dE, Tensor Ri_d, Tensor F) -> Tensor[]")
+    lib.define("calculateNepVirial(Tensor list_neigh, Tensor dE, Tensor Rij, Tensor Ri_d, Tensor num_atom) "
+               "-> Tensor[]")
+    lib.define("calculate_maxneigh(Tensor num_atoms, Tensor box, Tensor box_orig, Tensor num_cell, Tensor position, "
+               "float cutoff_2b, float cutoff_3b, int atom_type_num, Tensor atom_type_map, bool with_type=False) "
+               "-> Tensor[]")
+    lib.define("calculate_neighbor(Tensor num_atoms, Tensor atom_type_map, Tensor atom_types, Tensor box, "
+               "Tensor box_orig, Tensor num_cell, Tensor position, float cutoff_2b, float cutoff_3b, "
+               "int max_NN_radial, int max_NN_angular, bool with_rij=False) -> Tensor[]")
+    lib.define("calculate_descriptor(Tensor weight_radial, Tensor weight_angular, Tensor Ri_radial, "
+               "Tensor NL_radial, Tensor atom_type_map, float cutoff_radial, float cutoff_angular, "
+               "int max_NN_radial, int lmax_3, int lmax_4, int lmax_5) -> Tensor[]")
+    for name in ("calculateNepFeat", "calculateNepMbFeat", "calculateNepForce", "calculateNepVirial",
+                 "calculate_maxneigh", "calculate_neighbor", "calculate_descriptor"):
+        lib.impl(name, getattr(nep_ops, name), "CompositeImplicitAutograd")
     _LIBS.append(lib)
     _REGISTERED = True

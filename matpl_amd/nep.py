@@ -1,0 +1,197 @@
+"""NEP model on the HIP operators (the GPU branch of the reference's NEP).
+
+Mirrors src/model/nep_net.py:293-648 (device != cpu): features from
+CalcOps_cuda.calculateNepFeat / calculateNepMbFeat, per-type fitting nets,
+dE = dEtot/dRi by autograd (create_graph, so force losses train the
+parameters), Ri_d from [r, x, y, z] (nep_net.py:440-468), forces and virial
+from calculateNepForce / calculateNepVirial, returned negated as the
+reference does.
+
+Host-side input packing (``pack_frames``) restates expand_box
+(src/pre_data/nep_data_loader.py:220-254): replicate each cell num_cell times
+along each vector so that the expanded box is at least 2 rc thick, keep the
+inverse of the expanded box (column-major vectors) in box[9:18].
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import calcops, nep_ops
+from .nepnet import NEPWeights
+
+
+def _det(l):
+    return (l[0] * (l[4] * l[8] - l[5] * l[7]) - l[1] * (l[3] * l[8] - l[5] * l[6]) +
+            l[2] * (l[3] * l[7] - l[4] * l[6]))
+
+
+def _area(u, v):
+    cx = u[1] * v[2] - u[2] * v[1]
+    cy = u[2] * v[0] - u[0] * v[2]
+    cz = u[0] * v[1] - u[1] * v[0]
+    return math.sqrt(cx * cx + cy * cy + cz * cz)
+
+
+def expand_box(lattice, cutoff):
+    """(num_cell [3], box [18], box_orig [9]) for one frame; lattice rows are the cell vectors."""
+    l = [float(x) for x in np.asarray(lattice, np.float64).T.reshape(9)]
+    a, b, c = l[0::3], l[1::3], l[2::3]
+    det = _det(l)
+    vol = abs(det)
+    nc = [int(math.ceil(2.0 * cutoff / (vol / _area(b, c)))), int(math.ceil(2.0 * cutoff / (vol / _area(c, a)))),
+          int(math.ceil(2.0 * cutoff / (vol / _area(a, b))))]
+    box = [0.0] * 18
+    for k in range(3):
+        for d in range(3):
+            box[3 * k + d] = l[3 * k + d] * float(nc[d])
+    box[9] = box[4] * box[8] - box[5] * box[7]
+    box[10] = box[2] * box[7] - box[1] * box[8]
+    box[11] = box[1] * box[5] - box[2] * box[4]
+    box[12] = box[5] * box[6] - box[3] * box[8]
+    box[13] = box[0] * box[8] - box[2] * box[6]
+    box[14] = box[2] * box[3] - box[0] * box[5]
+    box[15] = box[3] * box[7] - box[4] * box[6]
+    box[16] = box[1] * box[6] - box[0] * box[7]
+    box[17] = box[0] * box[4] - box[1] * box[3]
+    det *= float(nc[0] * nc[1] * nc[2])
+    for n in range(9, 18):
+        box[n] /= det
+    return np.array(nc, np.int64), np.array(box), np.array(l)
+
+
+def pack_frames(frames, cutoff_radial, device="cuda"):
+    """Packed NEP inputs (num_atoms, box, box_orig, num_cell, position, atom_type_map) as device tensors."""
+    nums, boxes, origs, ncs = [], [], [], []
+    for f in frames:
+        nc, box, lat = expand_box(f.lattice, cutoff_radial)
+        nums.append(f.natoms); boxes.append(box); origs.append(lat); ncs.append(nc)
+    t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
+    return dict(num_atoms=t(np.array(nums), torch.int64), box=t(np.concatenate(boxes), torch.float64),
+                box_orig=t(np.concatenate(origs), torch.float64), num_cell=t(np.concatenate(ncs), torch.int64),
+                position=t(np.concatenate([f.pos for f in frames]), torch.float64),
+                atom_type_map=t(np.concatenate([f.types for f in frames]), torch.int64))
+
+
+def neighbors(packed, rc_r, rc_a, ntypes, max_r=None, max_a=None):
+    """calculate_maxneigh then calculate_neighbor with Rij (as the reference's NEP data path,
+    nep_data_loader.py / nep_net.py), lists sized by the largest count unless given."""
+    p = packed
+    if max_r is None or max_a is None:
+        NNr, NNa = nep_ops.calculate_maxneigh(p["num_atoms"], p["box"], p["box_orig"], p["num_cell"], p["position"],
+                                              rc_r, rc_a, ntypes, p["atom_type_map"], False)
+        max_r = max_r or max(int(NNr.max().item()) if NNr.numel() else 1, 1)
+        max_a = max_a or max(int(NNa.max().item()) if NNa.numel() else 1, 1)
+    NN_r, NN_a, NL_r, NL_a, Ri_r, Ri_a = nep_ops.calculate_neighbor(
+        p["num_atoms"], p["atom_type_map"], p["atom_type_map"], p["box"], p["box_orig"], p["num_cell"], p["position"],
+        rc_r, rc_a, max_r, max_a, True)
+    return dict(NN_r=NN_r, NN_a=NN_a, NL_r=NL_r, NL_a=NL_a, Ri_r=Ri_r, Ri_a=Ri_a)
+
+
+def ri_d(Ri):
+    """d[r, x, y, z]/d(dR) [N, S, 4, 3] (nep_net.py:440-468): row 0 = dR/r, rows 1-3 = identity."""
+    N, S = Ri.shape[:2]
+    out = torch.zeros((N, S, 4, 3), dtype=Ri.dtype, device=Ri.device)
+    r = Ri[..., 0]
+    m = (r.abs() > 1e-5).unsqueeze(-1)
+    out[:, :, 0, :] = torch.where(m, Ri[..., 1:4] / torch.where(m, r.unsqueeze(-1), torch.ones_like(r).unsqueeze(-1)),
+                                  torch.zeros_like(Ri[..., 1:4]))
+    eye = torch.eye(3, dtype=Ri.dtype, device=Ri.device)
+    out[:, :, 1:4, :] = eye * m.unsqueeze(-1).to(Ri.dtype)
+    return out
+
+
+class _FitNet(torch.nn.Module):
+    """NEP fitting net (src/model/nep_fitting.py:40-163): tanh first layer, residual hidden layers
+    (with resnet_dt), linear output."""
+
+    def __init__(self, layers, dtype, device):
+        super().__init__()
+        self.W = torch.nn.ParameterList()
+        self.b = torch.nn.ParameterList()
+        self.rdt = []
+        for W, b, r in layers:
+            self.W.append(torch.nn.Parameter(torch.as_tensor(W, dtype=dtype, device=device)))
+            self.b.append(torch.nn.Parameter(torch.as_tensor(b, dtype=dtype, device=device)))
+            self.rdt.append(None if r is None else torch.nn.Parameter(torch.as_tensor(r, dtype=dtype, device=device)))
+        for k, r in enumerate(self.rdt):
+            if r is not None:
+                self.register_parameter(f"rdt{k}", r)
+
+    def forward(self, x):
+        L = len(self.W)
+        for l in range(L):
+            z = x @ self.W[l] + self.b[l]
+            if l == L - 1:
+                return z
+            h = torch.tanh(z)
+            x = h if l == 0 else (h * self.rdt[l] if self.rdt[l] is not None else h) + x
+        return x
+
+
+class NEPTorch(torch.nn.Module):
+    """NEP.forward on the HIP operators: (Etot [F,1], Ei [N], Force [N,3], Virial [F,9])."""
+
+    def __init__(self, w: NEPWeights, dtype=torch.float64, device="cuda"):
+        super().__init__()
+        calcops.register()
+        cfg = w.cfg
+        self.cfg = cfg
+        self.c2 = torch.nn.Parameter(torch.as_tensor(w.c2, dtype=dtype, device=device))
+        self.c3 = torch.nn.Parameter(torch.as_tensor(w.c3, dtype=dtype, device=device))
+        self.register_buffer("q_scaler", torch.as_tensor(w.q_scaler, dtype=dtype, device=device))
+        self.fit = torch.nn.ModuleList([_FitNet(layers, dtype, device) for layers in w.fit])
+        self.dtype = dtype
+
+    def features(self, Ri_r, NL_r, Ri_a, NL_a, atom_type_map):
+        cfg = self.cfg
+        l3, l4, l5 = cfg.l_max
+        ops = torch.ops.CalcOps_cuda
+        q = ops.calculateNepFeat(self.c2, Ri_r, NL_r, atom_type_map, Ri_r.new_empty(0), cfg.rcut_radial,
+                                 cfg.n_angular_feats)[0]
+        if l3 > 0:
+            qa = ops.calculateNepMbFeat(self.c3, Ri_a, NL_a, atom_type_map, Ri_a.new_empty(0), cfg.nr, l3, l4, l5,
+                                        cfg.rcut_angular)[0]
+            q = torch.cat([q, qa], -1)
+        return q
+
+    def energies(self, feats, atom_type_map):
+        x = feats * self.q_scaler
+        Ei = torch.zeros(feats.shape[0], dtype=feats.dtype, device=feats.device)
+        for t, net in enumerate(self.fit):
+            idx = torch.nonzero(atom_type_map == t).reshape(-1)
+            if idx.numel():
+                Ei = Ei.index_put((idx,), net(x[idx]).reshape(-1))
+        return Ei
+
+    def forward(self, NN_r, NL_r, Ri_r, NN_a, NL_a, Ri_a, num_atom, atom_type_map, is_calc_f=True,
+                create_graph=None):
+        Ri_r = Ri_r.to(self.dtype).detach().requires_grad_()
+        Ri_a = Ri_a.to(self.dtype).detach().requires_grad_()
+        l3 = self.cfg.l_max[0]
+        feats = self.features(Ri_r, NL_r, Ri_a, NL_a, atom_type_map)
+        Ei = self.energies(feats, atom_type_map)
+        sizes = num_atom.reshape(-1).tolist()
+        Etot = torch.stack([e.sum() for e in Ei.split(sizes)]).unsqueeze(-1)
+        if not is_calc_f:
+            return Etot, Ei, None, None
+        if create_graph is None:
+            create_graph = torch.is_grad_enabled() and any(p.requires_grad for p in self.parameters())
+        ins = [Ri_r, Ri_a] if l3 > 0 else [Ri_r]
+        g = torch.autograd.grad([Etot.sum()], ins, create_graph=create_graph, allow_unused=True)
+        ops = torch.ops.CalcOps_cuda
+        Force = Virial = None
+        for Ri, NL, dE in zip(ins, (NL_r, NL_a), g):
+            if dE is None:
+                continue
+            N, S = NL.shape
+            d = ri_d(Ri.detach())
+            centre = -torch.matmul(dE.reshape(N, 1, S * 4), d.reshape(N, S * 4, 3)).squeeze(-2)
+            F = ops.calculateNepForce(NL, dE, d, centre)[0]
+            V = ops.calculateNepVirial(NL, dE, Ri.detach()[..., 1:].contiguous(), d, num_atom)[0]
+            Force = F if Force is None else Force + F
+            Virial = V if Virial is None else Virial + V
+        return Etot, Ei, -Force, -Virial

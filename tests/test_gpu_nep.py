@@ -1,0 +1,278 @@
+"""NEP operators on the GPU against the FP64 oracle (oracle/nep_oracle.c, oracle/nep.py).
+
+Neighbor lists and counts bit-exact; features, forces and virials to 1e-10
+relative in float64 and 2e-5 in float32; first- and second-order operator
+gradients by gradcheck / gradgradcheck; the force-loss parameter gradient of
+the whole model against the oracle's torch double backward.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle.nep as on
+from matpl_amd import nep as mn
+from matpl_amd import nep_ops
+from matpl_amd import structures as st
+from matpl_amd.nepnet import NEPConfig, NEPWeights
+
+pytestmark = pytest.mark.gpu
+
+
+def _frames():
+    return [st.zincblende_sic((2, 2, 2), seed=4), st.agausic_frames(nframes=2, nmin=30, nmax=50, seed=5)[0],
+            st.zincblende_sic((3, 3, 3), seed=3)]
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max(initial=0.0) / max(np.abs(b).max(initial=0.0), 1e-30))
+
+
+def test_neighbor_bit_exact(dev):
+    frames = _frames()
+    pk = on.pack_frames(frames, 6.0)
+    want = on.neighbor(pk, 6.0, 4.0, 200, 60)
+    gp = mn.pack_frames(frames, 6.0, dev)
+    for k in ("box", "box_orig", "num_cell"):
+        assert np.array_equal(gp[k].cpu().numpy().reshape(-1), np.asarray(pk[k]).reshape(-1)), k
+    got = nep_ops.calculate_neighbor(gp["num_atoms"], gp["atom_type_map"], gp["atom_type_map"], gp["box"],
+                                     gp["box_orig"], gp["num_cell"], gp["position"], 6.0, 4.0, 200, 60, True)
+    for name, t in zip(("NN_r", "NN_a", "NL_r", "NL_a", "Ri_r", "Ri_a"), got):
+        assert np.array_equal(t.cpu().numpy(), want[name]), name
+    # without rij: [x, y, z]
+    got3 = nep_ops.calculate_neighbor(gp["num_atoms"], gp["atom_type_map"], gp["atom_type_map"], gp["box"],
+                                      gp["box_orig"], gp["num_cell"], gp["position"], 6.0, 4.0, 200, 60, False)
+    assert np.array_equal(got3[4].cpu().numpy(), want["Ri_r"][..., 1:])
+    for with_type in (False, True):
+        r, a = on.maxneigh(pk, 6.0, 4.0, 3, with_type)
+        g = nep_ops.calculate_maxneigh(gp["num_atoms"], gp["box"], gp["box_orig"], gp["num_cell"], gp["position"],
+                                       6.0, 4.0, 3, gp["atom_type_map"], with_type)
+        assert np.array_equal(g[0].cpu().numpy(), r) and np.array_equal(g[1].cpu().numpy(), a)
+
+
+def test_neighbor_overflow_raises(dev):
+    gp = mn.pack_frames([st.zincblende_sic((2, 2, 2), seed=4)], 6.0, dev)
+    with pytest.raises(OverflowError):
+        nep_ops.calculate_neighbor(gp["num_atoms"], gp["atom_type_map"], gp["atom_type_map"], gp["box"],
+                                   gp["box_orig"], gp["num_cell"], gp["position"], 6.0, 4.0, 20, 60, True)
+
+
+@pytest.mark.parametrize("lmax", [(4, 2, 0), (4, 2, 1), (4, 0, 0), (2, 0, 0)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_features_match_oracle(dev, lmax, dtype):
+    frames = _frames()
+    cfg = NEPConfig(ntypes=3, l_max=list(lmax))
+    w = NEPWeights.random(cfg, seed=11)
+    pk = on.pack_frames(frames, 6.0)
+    nb = on.neighbor(pk, 6.0, 4.0, 200, 60)
+    am = pk["atom_type_map"]
+    qr = on.radial_feats(w.c2, torch.as_tensor(nb["Ri_r"]), nb["NL_r"], am, 6.0).numpy()
+    qa = on.angular_feats(w.c3, torch.as_tensor(nb["Ri_a"]), nb["NL_a"], am, 4.0, *lmax).numpy()
+    T = lambda a, dt=dtype: torch.as_tensor(a, dtype=dt, device=dev)
+    gr = nep_ops.calculateNepFeat(T(w.c2), T(nb["Ri_r"]), T(nb["NL_r"], torch.int64), T(am, torch.int64), None,
+                                  6.0, 0)[0]
+    ga = nep_ops.calculateNepMbFeat(T(w.c3), T(nb["Ri_a"]), T(nb["NL_a"], torch.int64), T(am, torch.int64), None,
+                                    cfg.nr, *lmax, 4.0)[0]
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    assert gr.dtype == dtype and ga.dtype == dtype
+    assert _rel(gr.cpu(), qr) < tol
+    assert _rel(ga.cpu(), qa) < tol
+
+
+def test_descriptor_matches_oracle(dev):
+    frames = _frames()[:2]
+    cfg = NEPConfig(ntypes=3, l_max=[4, 2, 1])
+    w = NEPWeights.random(cfg, seed=12)
+    pk = on.pack_frames(frames, 6.0)
+    r = on.nep_evaluate(frames, w)
+    gp = mn.pack_frames(frames, 6.0, dev)
+    nb = nep_ops.calculate_neighbor(gp["num_atoms"], gp["atom_type_map"], gp["atom_type_map"], gp["box"],
+                                    gp["box_orig"], gp["num_cell"], gp["position"], 6.0, 6.0, 200, 200, False)
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev)
+    q = nep_ops.calculate_descriptor(T(w.c2), T(w.c3), nb[4], nb[2], gp["atom_type_map"], 6.0, 4.0, 200, 4, 2, 1)[0]
+    assert _rel(q.cpu(), r["q"]) < 1e-11
+
+
+def _rand_inputs(dev, N=6, S=7, T=2, nm=3, nb=5, angular=False, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    NL = torch.randint(0, N, (N, S), generator=g)
+    cnt = torch.randint(S // 2, S + 1, (N,), generator=g)
+    NL[torch.arange(S)[None, :] >= cnt[:, None]] = -1
+    xyz = torch.randn(N, S, 3, generator=g, dtype=torch.float64)
+    xyz = xyz / xyz.norm(dim=-1, keepdim=True) * (1.0 + 2.5 * torch.rand(N, S, 1, generator=g, dtype=torch.float64))
+    r = xyz.norm(dim=-1, keepdim=True) * (1 + 0.01 * torch.randn(N, S, 1, generator=g, dtype=torch.float64))
+    Ri = torch.cat([r, xyz], -1)
+    Ri[NL < 0] = 0
+    am = torch.randint(0, T, (N,), generator=g)
+    c = torch.randn(T, T, nm, nb, generator=g, dtype=torch.float64)
+    return (c.to(dev), Ri.to(dev), NL.to(dev), am.to(dev))
+
+
+def test_radial_gradcheck(dev):
+    c, Ri, NL, am = _rand_inputs(dev)
+    c.requires_grad_(); Ri.requires_grad_()
+    f = lambda c_, R_: nep_ops.NepFeatRadial.apply(c_, R_, NL, am, 4.0)
+    assert torch.autograd.gradcheck(f, (c, Ri), eps=1e-6, atol=1e-7, rtol=1e-6)
+    # second order as force training uses it: d(gRi)/d(upstream feature grad, c)
+    g = torch.randn(NL.shape[0], c.shape[2], dtype=torch.float64, device=dev, requires_grad=True)
+    h = lambda g_, c_: nep_ops.NepFeatRadialGrad.apply(g_, c_, Ri.detach(), NL, am, 4.0)[1]
+    assert torch.autograd.gradcheck(h, (g, c), eps=1e-6, atol=1e-7, rtol=1e-6)
+
+
+@pytest.mark.parametrize("lmax", [(4, 2, 1), (4, 2, 0), (3, 0, 0)])
+def test_angular_gradcheck(dev, lmax):
+    c, Ri, NL, am = _rand_inputs(dev, seed=1)
+    c.requires_grad_(); Ri.requires_grad_()
+    f = lambda c_, R_: nep_ops.NepFeatAngular.apply(c_, R_, NL, am, 4.0, *lmax)
+    assert torch.autograd.gradcheck(f, (c, Ri), eps=1e-6, atol=1e-7, rtol=1e-6)
+    nf = nep_ops._nangular(c.shape[2], *lmax)
+    g = torch.randn(NL.shape[0], nf, dtype=torch.float64, device=dev, requires_grad=True)
+
+    def h(g_, c_):
+        R = Ri.detach()
+        s = torch.empty((NL.shape[0], c.shape[2] * 24), dtype=torch.float64, device=dev)
+        nep_ops._feat_call(0, True, c_.detach(), R, NL, am, 4.0, lmax, feats=torch.empty(NL.shape[0], nf,
+                           dtype=torch.float64, device=dev), s_save=s)
+        return nep_ops.NepFeatAngularGrad.apply(g_, c_, R, NL, am, s, 4.0, *lmax)[1]
+
+    # the saved s sums are recomputed for every perturbed c, so the check sees d/dc through them too
+    assert torch.autograd.gradcheck(h, (g, c), eps=1e-6, atol=1e-7, rtol=1e-6)
+
+
+def test_force_virial_ops_and_grads(dev):
+    _, Ri, NL, _ = _rand_inputs(dev, N=9, S=8, seed=2)
+    N, S = NL.shape
+    g = torch.Generator().manual_seed(3)
+    dE = torch.randn(N, S, 4, generator=g, dtype=torch.float64).to(dev) * (NL >= 0).unsqueeze(-1)
+    Rid = mn.ri_d(Ri)
+    num_atom = torch.tensor([4, 5], device=dev)
+    centre = -torch.matmul(dE.reshape(N, 1, -1), Rid.reshape(N, -1, 3)).squeeze(-2)
+    F = nep_ops.calculateNepForce(NL, dE, Rid, centre)[0]
+    V, AV = nep_ops.calculateNepVirial(NL, dE, Ri[..., 1:].contiguous(), Rid, num_atom)
+    # direct restatement on the CPU
+    dEc, Ridc, NLc, Ric = dE.cpu(), Rid.cpu(), NL.cpu(), Ri.cpu()
+    pair = torch.einsum("nsk,nska->nsa", dEc, Ridc)
+    Fw = -pair.sum(1)
+    AVw = torch.zeros(N, 9, dtype=torch.float64)
+    for i in range(N):
+        for s in range(S):
+            j = int(NLc[i, s])
+            if j < 0:
+                continue
+            Fw[j] += pair[i, s]
+            AVw[j] += torch.outer(pair[i, s], Ric[i, s, 1:]).reshape(9)
+    Vw = torch.stack([AVw[:4].sum(0), AVw[4:].sum(0)])
+    assert torch.allclose(F.cpu(), Fw, rtol=1e-12, atol=1e-12)
+    assert torch.allclose(AV.cpu(), AVw, rtol=1e-12, atol=1e-12)
+    assert torch.allclose(V.cpu(), Vw, rtol=1e-12, atol=1e-12)
+    # backward wrt dE: the op's gradient covers centre + neighbor terms
+    dE.requires_grad_()
+    f = lambda d: nep_ops.NepForce.apply(NL, d, Rid, torch.zeros(N, 3, dtype=torch.float64, device=dev)) - \
+        torch.matmul(d.reshape(N, 1, -1), Rid.reshape(N, -1, 3)).squeeze(-2).detach()
+    gF = torch.randn(N, 3, dtype=torch.float64, device=dev)
+    (got,) = torch.autograd.grad(f(dE), dE, gF)
+    want = torch.zeros(N, S, 4, dtype=torch.float64)
+    for i in range(N):
+        for s in range(S):
+            j = int(NLc[i, s])
+            if j < 0:
+                continue
+            want[i, s] = Ridc[i, s] @ (gF.cpu()[j] - gF.cpu()[i])
+    assert torch.allclose(got.cpu(), want, rtol=1e-12, atol=1e-12)
+    gV = torch.randn(2, 9, dtype=torch.float64, device=dev)
+    (gotv,) = torch.autograd.grad(nep_ops.NepVirial.apply(NL, dE, Ri[..., 1:].contiguous(), Rid, num_atom)[0], dE,
+                                  gV)
+    wantv = torch.zeros(N, S, 4, dtype=torch.float64)
+    for i in range(N):
+        fr = 0 if i < 4 else 1
+        for s in range(S):
+            if int(NLc[i, s]) < 0:
+                continue
+            wantv[i, s] = Ridc[i, s] @ gV.cpu()[fr].reshape(3, 3) @ Ric[i, s, 1:]
+    assert torch.allclose(gotv.cpu(), wantv, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("lmax", [(4, 2, 0), (4, 2, 1)])
+def test_model_energy_force_virial(dev, lmax):
+    frames = _frames()
+    cfg = NEPConfig(ntypes=3, l_max=list(lmax))
+    w = NEPWeights.random(cfg, seed=13)
+    want = on.nep_evaluate(frames, w)
+    gp = mn.pack_frames(frames, cfg.rcut_radial, dev)
+    nb = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes)
+    model = mn.NEPTorch(w, torch.float64, dev)
+    Etot, Ei, F, V = model(nb["NN_r"], nb["NL_r"], nb["Ri_r"], nb["NN_a"], nb["NL_a"], nb["Ri_a"], gp["num_atoms"],
+                           gp["atom_type_map"], create_graph=False)
+    assert _rel(Etot.detach().cpu().reshape(-1), want["E"]) < 1e-12
+    assert _rel(Ei.detach().cpu(), want["Ei"]) < 1e-12
+    assert _rel(F.detach().cpu(), want["F"]) < 1e-10
+    assert _rel(V.detach().cpu(), want["V"]) < 1e-10
+
+
+def test_model_float32(dev):
+    frames = _frames()
+    cfg = NEPConfig(ntypes=3, l_max=[4, 2, 0])
+    w = NEPWeights.random(cfg, seed=14)
+    want = on.nep_evaluate(frames, w)
+    gp = mn.pack_frames(frames, cfg.rcut_radial, dev)
+    nb = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes)
+    model = mn.NEPTorch(w, torch.float32, dev)
+    Etot, Ei, F, V = model(nb["NN_r"], nb["NL_r"], nb["Ri_r"], nb["NN_a"], nb["NL_a"], nb["Ri_a"], gp["num_atoms"],
+                           gp["atom_type_map"], create_graph=False)
+    assert _rel(Etot.detach().cpu().reshape(-1), want["E"]) < 1e-5
+    f_rmse = float(np.sqrt(np.mean((F.detach().cpu().double().numpy() - want["F"]) ** 2)))
+    assert f_rmse < 1e-4, f_rmse
+
+
+def test_force_loss_training_gradient(dev):
+    """d(sum F^2 + E^2)/d(c2, c3, fitting params) through the double-backward chain vs the oracle."""
+    frames = [st.zincblende_sic((2, 2, 2), seed=4)]
+    cfg = NEPConfig(ntypes=2, l_max=[4, 2, 1])
+    w = NEPWeights.random(cfg, seed=15)
+    pk = on.pack_frames(frames, cfg.rcut_radial)
+    onb = on.neighbor(pk, 6.0, 4.0, 120, 40)
+    am = pk["atom_type_map"]
+    # oracle: torch FP64 CPU with parameters as leaves
+    c2 = torch.tensor(w.c2, requires_grad=True)
+    c3 = torch.tensor(w.c3, requires_grad=True)
+    Ri_r = torch.as_tensor(onb["Ri_r"]).clone().requires_grad_()
+    Ri_a = torch.as_tensor(onb["Ri_a"]).clone().requires_grad_()
+    q = torch.cat([on.radial_feats(c2, Ri_r, onb["NL_r"], am, 6.0),
+                   on.angular_feats(c3, Ri_a, onb["NL_a"], am, 4.0, 4, 2, 1)], -1)
+    fit = [[(torch.tensor(W, requires_grad=True), torch.tensor(b, requires_grad=True),
+             None if r is None else torch.tensor(r, requires_grad=True)) for (W, b, r) in layers] for layers in w.fit]
+    x = q * torch.as_tensor(w.q_scaler)
+    amt = torch.as_tensor(am)
+    Ei = torch.zeros(q.shape[0], dtype=torch.float64)
+    for t in range(cfg.ntypes):
+        m = amt == t
+        Ei = Ei.index_put((m,), on.fit_forward(fit[t], x[m]).reshape(-1))
+    E = Ei.sum()
+    gr, ga = torch.autograd.grad(E, [Ri_r, Ri_a], create_graph=True)
+    loss_terms = []
+    for Ri, gg, NL in ((Ri_r, gr, onb["NL_r"]), (Ri_a, ga, onb["NL_a"])):
+        NLt = torch.as_tensor(NL)
+        valid = (NLt >= 0).unsqueeze(-1)
+        r = torch.where(valid[..., 0], Ri[..., 0].detach(), torch.ones_like(Ri[..., 0]))
+        pair = (gg[..., 0:1] * Ri[..., 1:4].detach() / r.unsqueeze(-1) + gg[..., 1:4]) * valid
+        F = pair.sum(1).index_add(0, NLt.clamp(min=0).reshape(-1), -pair.reshape(-1, 3))
+        loss_terms.append(F)
+    F_or = loss_terms[0] + loss_terms[1]
+    loss = (F_or ** 2).sum() + E ** 2
+    params_or = [c2, c3] + [p for layers in fit for (W, b, r) in layers for p in (W, b, r) if p is not None]
+    want = torch.autograd.grad(loss, params_or)
+    # HIP path
+    gp = mn.pack_frames(frames, cfg.rcut_radial, dev)
+    nb = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes, 120, 40)
+    model = mn.NEPTorch(w, torch.float64, dev)
+    Etot, _, Fg, _ = model(nb["NN_r"], nb["NL_r"], nb["Ri_r"], nb["NN_a"], nb["NL_a"], nb["Ri_a"], gp["num_atoms"],
+                           gp["atom_type_map"], create_graph=True)
+    assert _rel(Fg.detach().cpu(), F_or.detach()) < 1e-10
+    lg = (Fg ** 2).sum() + (Etot ** 2).sum()
+    params = [model.c2, model.c3]
+    for net in model.fit:
+        for l in range(len(net.W)):
+            params += [net.W[l], net.b[l]] + ([net.rdt[l]] if net.rdt[l] is not None else [])
+    got = torch.autograd.grad(lg, params)
+    for a, b in zip(got, want):
+        assert _rel(a.detach().cpu(), b) < 1e-9
