@@ -50,9 +50,30 @@ def _feat_call(mode, angular, c, Ri, NL, atom_map, rc, l=(0, 0, 0), feats=None, 
               _ptr(gg), _ptr(gRi), _ptr(gc), _ptr(ggfeat), _stream())
 
 
+def _expect(t, numel, what):
+    if t.numel() != numel:
+        raise ValueError(f"CalcOps_cuda NEP ops: {what} has {t.numel()} elements, expected {numel}")
+
+
+def _check_list(NL, natoms):
+    """Neighbor indices must address atoms of the packed batch (the kernels index with them)."""
+    if NL.numel() and int(NL.max().item()) >= natoms:
+        raise ValueError(f"CalcOps_cuda NEP ops: neighbor index {int(NL.max().item())} >= {natoms} atoms")
+
+
 def _prep(c, Ri, NL, atom_map):
     Ri = Ri.contiguous()
-    return (c.to(Ri.dtype).contiguous(), Ri, NL.to(torch.int64).contiguous(), atom_map.to(torch.int64).contiguous())
+    NL = NL.to(torch.int64).contiguous()
+    atom_map = atom_map.to(torch.int64).contiguous()
+    if c.dim() != 4 or c.shape[0] != c.shape[1]:
+        raise ValueError(f"CalcOps_cuda NEP ops: coefficients must be [T, T, n, k], got {tuple(c.shape)}")
+    if NL.dim() != 2 or atom_map.numel() != NL.shape[0]:
+        raise ValueError("CalcOps_cuda NEP ops: NL must be [N, S] with one atom_map entry per atom")
+    _expect(Ri, NL.numel() * 4, "Ri [N, S, 4]")
+    _check_list(NL, NL.shape[0])
+    if atom_map.numel() and int(atom_map.max().item()) >= c.shape[0]:
+        raise ValueError("CalcOps_cuda NEP ops: atom type beyond the coefficient table")
+    return (c.to(Ri.dtype).contiguous(), Ri, NL, atom_map)
 
 
 # ------------------------------------------------------------------ radial
@@ -148,6 +169,10 @@ class NepForce(torch.autograd.Function):
         Ri_d = Ri_d.to(dE.dtype).contiguous()
         out = F.to(dE.dtype).contiguous().clone()
         N, S = NL.shape
+        _expect(dE, N * S * 4, "dE [N, S, 4]")
+        _expect(Ri_d, N * S * 12, "Ri_d [N, S, 4, 3]")
+        _expect(out, N * 3, "F [N, 3]")
+        _check_list(NL, N)
         _lib.call("matpl_nep_force", _f64(dE), _ptr(NL), _ptr(dE), _ptr(Ri_d), _ptr(out), N, S, _stream())
         ctx.save_for_backward(NL, Ri_d)
         ctx.dE_shape = dE.shape
@@ -181,6 +206,12 @@ class NepVirial(torch.autograd.Function):
         Rij = Rij.to(dE.dtype).contiguous()
         Ri_d = Ri_d.to(dE.dtype).contiguous()
         N, S = NL.shape
+        _expect(dE, N * S * 4, "dE [N, S, 4]")
+        _expect(Ri_d, N * S * 12, "Ri_d [N, S, 4, 3]")
+        _expect(Rij, N * S * 3, "Rij [N, S, 3]")
+        _check_list(NL, N)
+        if int(num_atom.sum().item()) != N:
+            raise ValueError(f"calculateNepVirial: num_atom sums to {int(num_atom.sum().item())}, list has {N} atoms")
         start, af = _frame_index(num_atom, dE.device)
         nf = start.numel() - 1
         virial = torch.empty((nf, 9), dtype=dE.dtype, device=dE.device)
@@ -208,6 +239,11 @@ class NepVirial(torch.autograd.Function):
 # ------------------------------------------------------------------ neighbor search
 def _nep_geo(num_atoms, box, box_orig, num_cell, position):
     dev = position.device
+    nf = num_atoms.numel()
+    if int(num_atoms.sum().item()) != position.shape[0] or position.numel() != 3 * position.shape[0]:
+        raise ValueError("NEP neighbor search: position must be [sum(num_atoms), 3]")
+    if box.numel() != 18 * nf or box_orig.numel() != 9 * nf or num_cell.numel() != 3 * nf:
+        raise ValueError("NEP neighbor search: box [F*18], box_orig [F*9], num_cell [F*3] expected")
     start, af = _frame_index(num_atoms, dev)
     return (start, af, box.to(torch.float64).contiguous(), box_orig.to(torch.float64).contiguous(),
             num_cell.to(torch.int64).contiguous(), position.to(torch.float64).contiguous())
