@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", default=None, help="override cells, e.g. 8,8,8")
+    ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a captured HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU decomposition is "
@@ -115,25 +116,46 @@ def main():
     md.check()
     for _ in range(args.warmup):
         md.step()
-    torch.cuda.synchronize()
-    md.check()
-    log("warmup done")
-    # timed region; HIP events on the launch stream around the backward kernel stage
-    ev = []
-    md.model.events = ev
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    while md.nstep % md.rebuild_every:
         md.step()
     torch.cuda.synchronize()
+    md.check()
+    use_graph = not args.no_graph
+    if use_graph:
+        md.capture()
+    log(f"warmup done ({md.nstep} steps){', captured a 10-step HIP graph' if use_graph else ''}")
+
+    def run(k):
+        done = 0
+        while done < k:
+            if md.graph is not None and md.nstep % md.rebuild_every == 0 and k - done >= md.rebuild_every:
+                done += md.step()
+            else:
+                md._step()
+                done += 1
+
+    # timed region
+    ev = []
+    if not use_graph:   # eager: HIP events on the launch stream around every backward launch
+        md.model.events = ev
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(args.steps)
+    torch.cuda.synchronize()
     t1 = time.perf_counter()
-    md.model.events = None
     md.check()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = N * args.steps / (t1 - t0)
     log(f"fused {args.precision}: {ms_per_step:.3f} ms/step, {value:.4g} atom-steps/s")
+    if use_graph:
+        # kernel durations: one more block of eager steps with events around the backward launches
+        md.model.events = ev
+        for _ in range(md.rebuild_every):
+            md._step()
+        torch.cuda.synchronize()
+    md.model.events = None
 
-    # roofline of the dominant (backward) kernel from the events recorded in the timed region
+    # roofline of the dominant (backward) kernel from the recorded events
     bwd_ms = [a.elapsed_time(b) for a, b in ev]
     from matpl_amd.neighbor import find_neighbore_gpu
     nl_rc = find_neighbore_gpu(frame.lattice, frame.frac(), frame.types, 2, w.cfg.m, w.cfg.rcut, device=dev,
@@ -182,7 +204,8 @@ def main():
         "config": {"workload": f"DP se_e2_a velocity-Verlet MD, fluorite HfO2 {reps[0]}x{reps[1]}x{reps[2]} cells "
                                f"({N} atoms), rcut 6.0 A, m 120/type, NL skin 0.5 A rebuilt every 10 steps, "
                                "E+F+V every step",
-                   "natoms": N, "precision": args.precision + " nets / fp32 descriptors / fp64 min-image+reductions"},
+                   "natoms": N, "precision": args.precision + " nets / fp32 descriptors / fp64 min-image+reductions",
+                   "hip_graph": use_graph},
         "fp64_dropin": fp64,
         "speedup_vs_fp64_dropin": round(value / fp64["value"], 2) if fp64 else None,
         "force_rmse_vs_fp64_eV_per_A": force_rmse,

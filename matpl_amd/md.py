@@ -24,8 +24,8 @@ from .neighbor import find_neighbore_gpu, image_dr, reverse_map
 from .structures import MASS, Frame
 
 
-def _affine(x: torch.Tensor, M: np.ndarray) -> torch.Tensor:
-    out = torch.empty_like(x)
+def _affine(x: torch.Tensor, M: np.ndarray, out=None) -> torch.Tensor:
+    out = torch.empty_like(x) if out is None else out
     Mh = np.ascontiguousarray(M, np.float64).reshape(9)
     _lib.call("matpl_md_affine", _ptr(x), _ptr(out), Mh.ctypes.data, x.shape[0], _stream())
     return out
@@ -58,18 +58,23 @@ class DPMD:
         self.compute()
 
     def rebuild(self):
+        """Neighbor list at rc + skin from the current positions; positions re-wrapped into the cell.
+        After the first call every result goes into the same buffers (stream-capturable)."""
         frac = _affine(self.pos, self.Linv)
+        first = not hasattr(self, "nl")
         self.nl = find_neighbore_gpu(self.L, frac, self.types, self.cfg.ntypes, self.mnl, self.rc_nl,
-                                     device=self.dev, want_dR=False, want_shift=True, check=False)
-        self.pos = _affine(self.nl.frac_wrapped, self.L)
-        self.rev, self.rev_status = reverse_map(self.nl, self.types, self.mnl)
+                                     device=self.dev, want_dR=False, want_shift=True, check=False,
+                                     out=None if first else self.nl)
+        _affine(self.nl.frac_wrapped, self.L, out=self.pos)
+        self.rev, self.rev_status = reverse_map(self.nl, self.types, self.mnl,
+                                                out=None if first else (self.rev, self.rev_status))
 
     def compute(self):
         self.Ei, self.F, self.V, self.status = self.model(self.pos, self.L_dev, self.types, self.nl.list,
                                                           self.nl.shift, self.nl.counts, self.mnl,
                                                           type_ranges=self.type_ranges, rev=self.rev)
 
-    def step(self):
+    def _step(self):
         _lib.call("matpl_md_kick", _ptr(self.vel), _ptr(self.pos), _ptr(self.F), _ptr(self.inv_mass), self.dt,
                   self.N, 1, _stream())
         self.nstep += 1
@@ -78,6 +83,43 @@ class DPMD:
         self.compute()
         _lib.call("matpl_md_kick", _ptr(self.vel), None, _ptr(self.F), _ptr(self.inv_mass), self.dt, self.N, 0,
                   _stream())
+
+    def step(self):
+        """One velocity-Verlet step; with a captured graph and at a block boundary, one replay
+        advances ``rebuild_every`` steps at once (returns the number of steps taken)."""
+        if self.graph is not None and self.nstep % self.rebuild_every == 0:
+            self.graph.replay()
+            self.nstep += self.rebuild_every
+            return self.rebuild_every
+        self._step()
+        return 1
+
+    graph = None
+
+    def capture(self):
+        """Captures ``rebuild_every`` MD steps (the last one with the neighbor-list rebuild) in one
+        HIP graph. State that crosses replays (positions, velocities, neighbor list, reverse map,
+        forces, energies, virial, status flags) lives in fixed buffers. Must be called at a block
+        boundary (nstep % rebuild_every == 0) after at least one eager step."""
+        if self.nstep % self.rebuild_every != 0:
+            raise RuntimeError("capture at a neighbor-list rebuild boundary")
+        static = dict(F=self.F, Ei=self.Ei, V=self.V, status=self.status)
+        nstep0 = self.nstep
+        s = torch.cuda.Stream(device=self.dev)
+        s.wait_stream(torch.cuda.current_stream(self.dev))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                for _ in range(self.rebuild_every):
+                    self._step()
+                for k, t in static.items():
+                    t.copy_(getattr(self, k))
+        torch.cuda.current_stream(self.dev).wait_stream(s)
+        for k, t in static.items():
+            setattr(self, k, t)
+        # capture records work without running it: state is unchanged
+        self.nstep = nstep0
+        self.graph = g
 
     def check(self):
         """Raises if a neighbor bucket overflowed (syncs the stream)."""

@@ -38,35 +38,57 @@ class NeighborList:
         return self
 
 
+# small device-side constants (cutoffs per type, one-frame index arrays) are made once and
+# reused, so that repeated calls - including under stream capture - do no host-to-device copies
+_CONST = {}
+
+
+def _const(key, make):
+    t = _CONST.get(key)
+    if t is None:
+        t = _CONST[key] = make()
+    return t
+
+
 def _rc_type_tensor(rc_type, ntypes, rc_m, device):
     if rc_type is None:
         rc_type = np.full(ntypes, rc_m)
-    return torch.as_tensor(np.asarray(rc_type, np.float64), device=device)
+    a = np.asarray(rc_type, np.float64)
+    return _const(("rct", str(device), a.tobytes()), lambda: torch.as_tensor(a, device=device))
 
 
 def find_neighbore_gpu(lattice, frac, type_map, ntypes: int, m: int, rc_m: float, rc_type=None,
                        want_dR=True, want_shift=False, device="cuda", force_brute=False,
-                       check=True) -> NeighborList:
-    """One frame. lattice [3,3] (host array; rows are cell vectors); frac [N,3]."""
+                       check=True, out: Optional[NeighborList] = None) -> NeighborList:
+    """One frame. lattice [3,3] (host array; rows are cell vectors); frac [N,3].
+
+    ``out``: preallocated result buffers (same shapes) to write into.
+    """
     lat = np.ascontiguousarray(np.asarray(lattice, np.float64).reshape(3, 3))
     frac_t = torch.as_tensor(frac, dtype=torch.float64, device=device).contiguous()
     tm = torch.as_tensor(type_map, dtype=torch.int32, device=device).contiguous()
     n = frac_t.shape[0]
     S = ntypes * m
     rct = _rc_type_tensor(rc_type, ntypes, rc_m, device)
-    out = NeighborList(
-        list=torch.empty((n, S), dtype=torch.int32, device=device),
-        dR=torch.empty((n, S, 3), dtype=torch.float64, device=device) if want_dR else None,
-        counts=torch.empty((n, ntypes), dtype=torch.int32, device=device),
-        shift=torch.empty((n, S), dtype=torch.int32, device=device) if want_shift else None,
-        frac_wrapped=torch.empty((n, 3), dtype=torch.float64, device=device),
-        status=torch.zeros(1, dtype=torch.int32, device=device),
-    )
+    if out is None:
+        out = NeighborList(
+            list=torch.empty((n, S), dtype=torch.int32, device=device),
+            dR=torch.empty((n, S, 3), dtype=torch.float64, device=device) if want_dR else None,
+            counts=torch.empty((n, ntypes), dtype=torch.int32, device=device),
+            shift=torch.empty((n, S), dtype=torch.int32, device=device) if want_shift else None,
+            frac_wrapped=torch.empty((n, 3), dtype=torch.float64, device=device),
+            status=torch.zeros(1, dtype=torch.int32, device=device),
+        )
+    else:
+        if out.list.shape != (n, S) or (want_dR and out.dR is None) or (want_shift and out.shift is None):
+            raise ValueError("find_neighbore_gpu: out buffers do not match")
+        out.status.zero_()
     small = _lib.query("matpl_dp_neighbor_is_small_box", lat.ctypes.data, float(rc_m))
     if small or force_brute:
-        lat_d = torch.as_tensor(lat.reshape(1, 9), device=device)
-        fs = torch.tensor([0, n], dtype=torch.int64, device=device)
-        af = torch.zeros(n, dtype=torch.int32, device=device)
+        dk = str(device)
+        lat_d = _const(("lat", dk, lat.tobytes()), lambda: torch.as_tensor(lat.reshape(1, 9), device=device))
+        fs = _const(("fs", dk, n), lambda: torch.tensor([0, n], dtype=torch.int64, device=device))
+        af = _const(("af", dk, n), lambda: torch.zeros(n, dtype=torch.int32, device=device))
         _lib.call("matpl_dp_neighbor_brute", _ptr(lat_d), _ptr(frac_t), _ptr(fs), _ptr(af), _ptr(tm), _ptr(rct),
                   ntypes, m, float(rc_m), 1, n, _ptr(out.frac_wrapped), _ptr(out.list), _ptr(out.dR),
                   _ptr(out.counts), _ptr(out.shift), _ptr(out.status), _stream())
@@ -106,15 +128,19 @@ def find_neighbore_batch(lattices, fracs: List, type_maps: List, ntypes: int, m:
     return out.check() if check else out
 
 
-def reverse_map(nl: NeighborList, type_map: torch.Tensor, m: int, frame_start=None, atom_frame=None) -> torch.Tensor:
-    """rev [N, T*m]: for each used entry, the index of its partner entry (j -> i, opposite image).
-
-    Raises if the list is not symmetric (syncs); the fused force gather needs it.
+def reverse_map(nl: NeighborList, type_map: torch.Tensor, m: int, frame_start=None, atom_frame=None, out=None):
+    """(rev [N, T*m], status [1]): for each used entry, the index of its partner entry (j -> i,
+    opposite image); status != 0 if some entry has no partner (the list is not symmetric; the fused
+    force gather needs it). ``out``: (rev, status) buffers to write into.
     """
     n, S = nl.list.shape
     T = S // m
-    rev = torch.empty((n, S), dtype=torch.int32, device=nl.list.device)
-    status = torch.zeros(1, dtype=torch.int32, device=nl.list.device)
+    if out is None:
+        rev = torch.empty((n, S), dtype=torch.int32, device=nl.list.device)
+        status = torch.zeros(1, dtype=torch.int32, device=nl.list.device)
+    else:
+        rev, status = out
+        status.zero_()
     tm = type_map.to(torch.int32).contiguous()
     _lib.call("matpl_dp_neighbor_reverse", _ptr(nl.list), _ptr(nl.shift), _ptr(nl.counts), _ptr(tm),
               _ptr(frame_start), _ptr(atom_frame), T, m, n, _ptr(rev), _ptr(status), _stream())
