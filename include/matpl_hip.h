@@ -199,6 +199,27 @@ int matpl_dp_fitting(const float *A, const int32_t *tiles, int ntiles, const voi
                      int64_t type_stride_w0b, int64_t type_stride_wh, int64_t type_stride_vec, int E, int nlayers,
                      int bf16, float *Ei, float *dA, void *stream);
 
+/* ------------------------------------------------------------ DP compress */
+/*
+ * calculateCompress: G[s][i] = polynomial of degree coe_num-1 in f2[s] with
+ * coefficients coef[s][i][0..coe_num-1], highest power first (coe_num 4 or
+ * 6). f2 [n], coef [n][L][coe_num], G [n][L]. Replaces
+ * CalculateCompressFuncs::forward (src/op/src/CalcOps.cpp:331-342) ->
+ * compress_calc (src/op/kernel/calculateCompress.cu).
+ */
+int matpl_dp_compress_f32(const float *f2, const float *coef, int64_t n, int L, int coe_num, float *G, void *stream);
+int matpl_dp_compress_f64(const double *f2, const double *coef, int64_t n, int L, int coe_num, double *G,
+                          void *stream);
+/*
+ * Backward wrt f2: gf2[s] = sum_i gG[s][i] dG[s][i]/df2 (the reference forms
+ * dG/df2 in compress_calc_grad, src/op/kernel/calculateCompress_grad.cu, and
+ * the product and sum in torch, CalcOps.cpp:344-357).
+ */
+int matpl_dp_compress_grad_f32(const float *f2, const float *coef, const float *gG, int64_t n, int L, int coe_num,
+                               float *gf2, void *stream);
+int matpl_dp_compress_grad_f64(const double *f2, const double *coef, const double *gG, int64_t n, int L,
+                               int coe_num, double *gf2, void *stream);
+
 /* ------------------------------------------------------------ NEP operators */
 /*
  * Neighbor search over replicated boxes (num_cell[f][3] replicas of the

@@ -45,6 +45,11 @@ SIGNATURES = {
     "matpl_dp_fused_backward": (I32, [P] * 11 + [I64, I32, I32, I32, I32, I32, F64, F64, F64] + [P] * 8),
     "matpl_dp_neighbor_reverse": (I32, [P, P, P, P, P, P, I32, I32, I64, P, P, P]),
     "matpl_dp_fitting": (I32, [P, P, I32, P, P, P, P, P, I64, I64, I64, I64, I32, I32, I32, P, P, P]),
+    # DP compress
+    "matpl_dp_compress_f32": (I32, [P, P, I64, I32, I32, P, P]),
+    "matpl_dp_compress_f64": (I32, [P, P, I64, I32, I32, P, P]),
+    "matpl_dp_compress_grad_f32": (I32, [P, P, P, I64, I32, I32, P, P]),
+    "matpl_dp_compress_grad_f64": (I32, [P, P, P, I64, I32, I32, P, P]),
     # NEP operators
     "matpl_nep_neighbor": (I32, [P] * 7 + [F64, F64, I64, I64, I64, I32, I32, I32] + [P] * 8),
     "matpl_nep_feat": (I32, [I32, I32, I32, P, P, P, P, I64] + [I32] * 10 + [F64] + [P] * 8),

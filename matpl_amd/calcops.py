@@ -1,9 +1,11 @@
 """``torch.ops.CalcOps_cuda``: the reference operator library, re-implemented.
 
 Registers the reference's op names and schemas (src/op/src/CalcOps_bind.cpp:6-17,
-signatures src/op/include/CalcOps.h:35-359) on top of the C-ABI library, plus
-the build's own ops in the ``matpl_amd`` namespace (neighbor search,
-environment matrix, statistics, fused DP evaluation).
+signatures src/op/include/CalcOps.h:35-359) on top of the C-ABI library: the
+DP force/virial ops here, calculateCompress (matpl_amd.compress) and the NEP
+ops (matpl_amd.nep_ops). The build's own entry points (cell-list neighbor
+search, environment matrix, statistics, fused DP evaluation) are plain Python
+functions in matpl_amd.neighbor / matpl_amd.dp_fused.
 
 Conventions kept from the reference (src/op/src/CalcOps.cpp):
   * ops allocate and return their outputs as a list of tensors;
@@ -184,7 +186,9 @@ def register() -> None:
     lib.define("calculateVirial(Tensor list_neigh, Tensor dE, Tensor Rij, Tensor Ri_d, Tensor nghost_tensor) -> Tensor[]")
     lib.impl("calculateForce", calculateForce, "CompositeImplicitAutograd")
     lib.impl("calculateVirial", calculateVirial, "CompositeImplicitAutograd")
-    from . import nep_ops
+    from . import compress, nep_ops
+    lib.define("calculateCompress(Tensor f2, Tensor coefficient) -> Tensor[]")
+    lib.impl("calculateCompress", compress.calculateCompress, "CompositeImplicitAutograd")
     # NEP schemas as bound in src/op/src/CalcOps_bind.cpp:10-16 (signatures src/op/include/CalcOps.h:168-318)
     lib.define("calculateNepFeat(Tensor coeff2, Tensor d12_radial, Tensor NL_radial, Tensor atom_map, Tensor feats, "
                "float rcut_radial, int feat_multi_nums) -> Tensor[]")

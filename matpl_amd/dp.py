@@ -15,6 +15,7 @@ from typing import List, Optional, Tuple
 import torch
 
 from . import calcops
+from .compress import calculateCompress
 from .dpnet import DPWeights
 from .neighbor import env_mat
 
@@ -57,11 +58,14 @@ class _Net(torch.nn.Module):
 
 
 class DPTorch(torch.nn.Module):
-    def __init__(self, w: DPWeights, dtype=torch.float64, device="cuda"):
+    def __init__(self, w: DPWeights, dtype=torch.float64, device="cuda", compress=None):
+        """compress: a matpl_amd.compress.CompressTable; the embedding nets are then evaluated from
+        the table with CalcOps_cuda.calculateCompress (dp_dp.py:350-355, 572-601)."""
         super().__init__()
         cfg = w.cfg
         self.cfg = cfg
         self.dtype = dtype
+        self.compress = compress
         T = cfg.ntypes
         self.emb = torch.nn.ModuleList([_Net(w.emb[p], 0, dtype, device) for p in range(T * T)])
         self.fit = torch.nn.ModuleList([_Net(w.fit[t], 1, dtype, device) for t in range(T)])
@@ -90,7 +94,11 @@ class DPTorch(torch.nn.Module):
             A = None
             for tj in present:
                 Rt = Ri[:, idx, tj * m:(tj + 1) * m]              # [B, n, m, 4]
-                G = self.emb[ti * T + tj](Rt[..., 0:1])            # [B, n, m, E]
+                if self.compress is None:
+                    G = self.emb[ti * T + tj](Rt[..., 0:1])        # [B, n, m, E]
+                else:
+                    f2, coef = self.compress.lookup(Rt[..., 0], ti * T + tj)
+                    G = calculateCompress(f2, coef)[0].reshape(Rt.shape[:-1] + (-1,))
                 part = Rt.transpose(-2, -1) @ G                     # [B, n, 4, E]
                 A = part if A is None else A + part
             A = A / (m * len(present))
