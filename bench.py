@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", default=None, help="override cells, e.g. 8,8,8")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a captured HIP graph")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MiB",
+                    help="after the run, copy a buffer of this size 3 times (a known-traffic kernel for "
+                         "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU decomposition is "
@@ -195,6 +198,15 @@ def main():
         log(f"fp64 drop-in: {fp64_ms:.2f} ms/step")
         del ref
         torch.cuda.empty_cache()
+
+    if args.copy_calib:
+        a = torch.empty(args.copy_calib << 18, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {args.copy_calib} MiB read + written")
+        del a, b
 
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
