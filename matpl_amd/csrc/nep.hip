@@ -238,7 +238,8 @@ nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int
         const int64_t n2 = c0 + lane;
         const bool valid = n2 < N2;
         int cr = 0, ca = 0;
-        const int t2 = valid ? (int)type_map[n2] : 0;
+        // type_map is only given (and only read) for the per-type counts of mode 2
+        const int t2 = (valid && mode == 2) ? (int)type_map[n2] : 0;
         // pass 1: counts
         if (valid) {
             for (int64_t rep = 0; rep < nrep; rep++) {

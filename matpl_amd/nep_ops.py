@@ -253,6 +253,8 @@ def calculate_maxneigh(num_atoms, box, box_orig, num_cell, position, cutoff_2b, 
                        atom_type_map, with_type=False):
     start, af, box, bo, nc, pos = _nep_geo(num_atoms, box, box_orig, num_cell, position)
     N = pos.shape[0]
+    if with_type and (atom_type_map.numel() != N or (N and int(atom_type_map.max().item()) >= atom_type_num)):
+        raise ValueError("calculate_maxneigh: atom_type_map must give each atom a type < atom_type_num")
     shape = (N, atom_type_num) if with_type else (N,)
     NN_r = torch.empty(shape, dtype=torch.int64, device=pos.device)
     NN_a = torch.empty(shape, dtype=torch.int64, device=pos.device)

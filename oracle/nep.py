@@ -96,6 +96,8 @@ def neighbor(packed, rc_r, rc_a, maxr, maxa):
 
 
 def maxneigh(packed, rc_r, rc_a, ntypes, with_type):
+    if with_type and np.asarray(packed["atom_type_map"]).max(initial=-1) >= ntypes:
+        raise ValueError("maxneigh: atom type >= ntypes")
     L = _setup()
     n = int(packed["num_atoms"].sum())
     shape = (n, ntypes) if with_type else (n,)

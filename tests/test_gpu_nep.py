@@ -44,9 +44,9 @@ def test_neighbor_bit_exact(dev):
                                       gp["box_orig"], gp["num_cell"], gp["position"], 6.0, 4.0, 200, 60, False)
     assert np.array_equal(got3[4].cpu().numpy(), want["Ri_r"][..., 1:])
     for with_type in (False, True):
-        r, a = on.maxneigh(pk, 6.0, 4.0, 3, with_type)
+        r, a = on.maxneigh(pk, 6.0, 4.0, 4, with_type)
         g = nep_ops.calculate_maxneigh(gp["num_atoms"], gp["box"], gp["box_orig"], gp["num_cell"], gp["position"],
-                                       6.0, 4.0, 3, gp["atom_type_map"], with_type)
+                                       6.0, 4.0, 4, gp["atom_type_map"], with_type)
         assert np.array_equal(g[0].cpu().numpy(), r) and np.array_equal(g[1].cpu().numpy(), a)
 
 
@@ -61,7 +61,7 @@ def test_neighbor_overflow_raises(dev):
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_features_match_oracle(dev, lmax, dtype):
     frames = _frames()
-    cfg = NEPConfig(ntypes=3, l_max=list(lmax))
+    cfg = NEPConfig(ntypes=4, l_max=list(lmax))
     w = NEPWeights.random(cfg, seed=11)
     pk = on.pack_frames(frames, 6.0)
     nb = on.neighbor(pk, 6.0, 4.0, 200, 60)
@@ -81,7 +81,7 @@ def test_features_match_oracle(dev, lmax, dtype):
 
 def test_descriptor_matches_oracle(dev):
     frames = _frames()[:2]
-    cfg = NEPConfig(ntypes=3, l_max=[4, 2, 1])
+    cfg = NEPConfig(ntypes=4, l_max=[4, 2, 1])
     w = NEPWeights.random(cfg, seed=12)
     pk = on.pack_frames(frames, 6.0)
     r = on.nep_evaluate(frames, w)
@@ -195,7 +195,7 @@ def test_force_virial_ops_and_grads(dev):
 @pytest.mark.parametrize("lmax", [(4, 2, 0), (4, 2, 1)])
 def test_model_energy_force_virial(dev, lmax):
     frames = _frames()
-    cfg = NEPConfig(ntypes=3, l_max=list(lmax))
+    cfg = NEPConfig(ntypes=4, l_max=list(lmax))
     w = NEPWeights.random(cfg, seed=13)
     want = on.nep_evaluate(frames, w)
     gp = mn.pack_frames(frames, cfg.rcut_radial, dev)
@@ -211,7 +211,7 @@ def test_model_energy_force_virial(dev, lmax):
 
 def test_model_float32(dev):
     frames = _frames()
-    cfg = NEPConfig(ntypes=3, l_max=[4, 2, 0])
+    cfg = NEPConfig(ntypes=4, l_max=[4, 2, 0])
     w = NEPWeights.random(cfg, seed=14)
     want = on.nep_evaluate(frames, w)
     gp = mn.pack_frames(frames, cfg.rcut_radial, dev)
