@@ -85,6 +85,53 @@ def cpu_baseline():
                       f"{nev} full E+F+V evaluations incl. neighbor search, FP64 C oracle with OpenMP"}
 
 
+def nep_leg(steps=20):
+    """NEP E+F+V on zincblende SiC 6x6x6 (1,728 atoms), radial/angular cutoff 6/4 A, n_max 4/4,
+    basis 12, l_max [4,2,0], 30 hidden units per type: one step = neighbor search + features +
+    fitting net + dE/dRi + force/virial ops, in float32 and in float64 (the drop-in path)."""
+    import torch
+    from matpl_amd import nep as mn
+    from matpl_amd import nep_ops
+    from matpl_amd import structures as st
+    from matpl_amd.nepnet import NEPConfig, NEPWeights
+    dev = torch.device("cuda:0")
+    f = st.zincblende_sic((6, 6, 6), a=4.36, sigma=0.08, seed=1001)
+    cfg = NEPConfig(ntypes=2, l_max=[4, 2, 0])
+    w = NEPWeights.random(cfg, seed=2021)
+    gp = mn.pack_frames([f], cfg.rcut_radial, dev)
+    NNr, NNa = nep_ops.calculate_maxneigh(
+        gp["num_atoms"], gp["box"], gp["box_orig"], gp["num_cell"], gp["position"], cfg.rcut_radial,
+        cfg.rcut_angular, 2, gp["atom_type_map"], False)
+    maxr, maxa = int(NNr.max().item()), int(NNa.max().item())
+    out = {"workload": f"NEP zincblende SiC 6x6x6 ({f.natoms} atoms), rc 6/4 A, n_max 4/4, basis 12, "
+                       f"l_max [4,2,0], 30 hidden, E+F+V incl. neighbor search", "natoms": f.natoms}
+    res = {}
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        model = mn.NEPTorch(w, dt, dev)
+
+        def step():
+            nb = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, 2, maxr, maxa)
+            return model(nb["NN_r"], nb["NL_r"], nb["Ri_r"], nb["NN_a"], nb["NL_a"], nb["Ri_a"], gp["num_atoms"],
+                         gp["atom_type_map"], create_graph=False)
+        for _ in range(3):
+            r = step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = step()
+        torch.cuda.synchronize()
+        dt_s = time.perf_counter() - t0
+        res[name] = r
+        out[name] = {"value": round(f.natoms * steps / dt_s, 1), "unit": "atom-steps/s",
+                     "ms_per_step": round(dt_s * 1e3 / steps, 3)}
+    F32, F64 = res["fp32"][2].detach().double(), res["fp64"][2].detach()
+    out["force_rmse_fp32_vs_fp64_eV_per_A"] = float(torch.sqrt(((F32 - F64) ** 2).mean()).item())
+    out["speedup_fp32_vs_fp64"] = round(out["fp32"]["value"] / out["fp64"]["value"], 2)
+    log(f"NEP: fp32 {out['fp32']['ms_per_step']} ms/step, fp64 {out['fp64']['ms_per_step']} ms/step, "
+        f"force RMSE {out['force_rmse_fp32_vs_fp64_eV_per_A']:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +143,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", default=None, help="override cells, e.g. 8,8,8")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a captured HIP graph")
+    ap.add_argument("--no-nep", action="store_true", help="skip the NEP (SiC 1,728 atoms) leg")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MiB",
                     help="after the run, copy a buffer of this size 3 times (a known-traffic kernel for "
                          "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -208,6 +256,7 @@ def main():
         log(f"copy calibration: 3 x {args.copy_calib} MiB read + written")
         del a, b
 
+    nep = None if args.no_nep else nep_leg()
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -223,6 +272,7 @@ def main():
         "force_rmse_vs_fp64_eV_per_A": force_rmse,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "nep": nep,
     }
     print(json.dumps(out), flush=True)
 
