@@ -96,6 +96,7 @@ struct FusedParams {
     int T, mnl, m;
     double rc2;                 // model cutoff^2 (FP64 test)
     float rmin, rmax, scale;    // scale = 1 / (m * type_nums)
+    float iw;                   // 1 / (rmax - rmin)
 };
 
 __device__ __forceinline__ float fast_tanh(float x) {
@@ -110,17 +111,16 @@ __device__ __forceinline__ float act(float x) {
     else return tanhf(x);
 }
 
-// s(r) and ds/dr, dp_dp.py:244-249 (FP32)
-__device__ __forceinline__ void switch_fn(float r, float rmin, float rmax, float &s, float &ds) {
+// s(r) and ds/dr, dp_dp.py:244-249 (FP32); iw = 1 / (rmax - rmin)
+__device__ __forceinline__ void switch_fn(float r, float rmin, float rmax, float iw, float &s, float &ds) {
+    const float ir = __builtin_amdgcn_rcpf(r);
     if (r < rmin) {
-        s = 1.0f / r;
+        s = ir;
         ds = -s * s;
     } else if (r < rmax) {
-        const float inv = 1.0f / (rmax - rmin);
-        const float u = (r - rmin) * inv;
+        const float u = (r - rmin) * iw;
         const float vv = u * u * u * (-6.0f * u * u + 15.0f * u - 10.0f) + 1.0f;
-        const float dvv = -30.0f * u * u * (u - 1.0f) * (u - 1.0f) * inv;
-        const float ir = 1.0f / r;
+        const float dvv = -30.0f * u * u * (u - 1.0f) * (u - 1.0f) * iw;
         s = vv * ir;
         ds = (dvv - s) * ir;
     } else {
@@ -129,49 +129,104 @@ __device__ __forceinline__ void switch_fn(float r, float rmin, float rmax, float
     }
 }
 
-// One matrix-core layer: acc (C-init) += W^T x, x given as 16 fp32 registers.
+// Weight blocks. Per type pair: NVEC vectors [2 halves][16] floats and NMAT
+// matrices of 64 lanes x 16 elements (bf16 or f32), pre-permuted for the
+// accumulator layout. When the weights of all pairs fit, each block stages
+// them in LDS once; the matrices are then stored as [chunk][lane][16 bytes]
+// so that every ds_read_b128 of a wavefront reads 64 consecutive 16-byte
+// chunks (no bank conflicts), and the vectors are read as broadcasts (all
+// lanes of a half read the same address).
 template <bool BF16>
-__device__ __forceinline__ void mfma_layer(const void *mat, int lane, const float x[16], floatx16 &acc) {
+struct MatLayout {
+    static constexpr int ES = BF16 ? 2 : 4;                 // element size
+    static constexpr int CPL = 16 * ES / 16;                // 16-byte chunks per lane
+    static constexpr int BLK = 64 * 16 * ES;                // bytes per matrix block
+};
+
+template <bool BF16, bool LW>
+__device__ __forceinline__ void mfma_layer(const unsigned char *blk, int lane, const float x[16], floatx16 &acc) {
     if constexpr (BF16) {
-        const bf16x8 *w = reinterpret_cast<const bf16x8 *>(mat) + lane * 2;
+        bf16x8 w0, w1;
+        if constexpr (LW) {
+            w0 = *reinterpret_cast<const bf16x8 *>(blk + lane * 16);
+            w1 = *reinterpret_cast<const bf16x8 *>(blk + 1024 + lane * 16);
+        } else {
+            const bf16x8 *w = reinterpret_cast<const bf16x8 *>(blk) + lane * 2;
+            w0 = w[0];
+            w1 = w[1];
+        }
         bf16x8 b0, b1;
 #pragma unroll
         for (int q = 0; q < 8; q++) { b0[q] = (__bf16)x[q]; b1[q] = (__bf16)x[8 + q]; }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[0], b0, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[1], b1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w0, b0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1, b1, acc, 0, 0, 0);
     } else {
-        const float *w = reinterpret_cast<const float *>(mat) + lane * 16;
+        float w[16];
+        if constexpr (LW) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const float4 v = *reinterpret_cast<const float4 *>(blk + (c * 64 + lane) * 16);
+                w[4 * c] = v.x; w[4 * c + 1] = v.y; w[4 * c + 2] = v.z; w[4 * c + 3] = v.w;
+            }
+        } else {
+            const float *wp = reinterpret_cast<const float *>(blk) + lane * 16;
+#pragma unroll
+            for (int r = 0; r < 16; r++) w[r] = wp[r];
+        }
 #pragma unroll
         for (int r = 0; r < 16; r++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(w[r], x[r], acc, 0, 0, 0);
     }
 }
 
 template <bool BF16>
-__device__ __forceinline__ const void *mat_ptr(const void *base, int pair, int which) {
-    const size_t per = (size_t)64 * 16 * (BF16 ? sizeof(__bf16) : sizeof(float));
-    return reinterpret_cast<const char *>(base) + ((size_t)pair * NMAT + which) * per;
+__device__ __forceinline__ const unsigned char *mat_ptr(const unsigned char *base, int pair, int which) {
+    return base + ((size_t)pair * NMAT + which) * MatLayout<BF16>::BLK;
 }
 
 __device__ __forceinline__ const float *vec_ptr(const float *base, int pair, int which, int h) {
     return base + (((size_t)pair * NVEC + which) * 2 + h) * 16;
 }
 
+// Bytes of LDS the staged weights of T*T pairs take.
+template <bool BF16>
+__host__ __device__ inline size_t weight_lds_bytes(int T) {
+    return (size_t)T * T * (NVEC * 32 * sizeof(float) + NMAT * MatLayout<BF16>::BLK);
+}
+
+// Copies every pair's weights from global memory into LDS (chunked matrix layout).
+template <bool BF16>
+__device__ void stage_weights(const FusedParams &p, float *svec, unsigned char *smat) {
+    using ML = MatLayout<BF16>;
+    const int npair = p.T * p.T;
+    const int nv = npair * NVEC * 32;
+    for (int x = threadIdx.x; x < nv; x += blockDim.x) svec[x] = p.vec[x];
+    const int nchunk = npair * NMAT * 64 * ML::CPL;
+    const uint4 *g = reinterpret_cast<const uint4 *>(p.mat);
+    uint4 *l = reinterpret_cast<uint4 *>(smat);
+    for (int u = threadIdx.x; u < nchunk; u += blockDim.x) {
+        const int blk = u / (64 * ML::CPL), rem = u - blk * (64 * ML::CPL);
+        const int ln = rem / ML::CPL, c = rem - ln * ML::CPL;
+        l[blk * 64 * ML::CPL + c * 64 + ln] = g[u];
+    }
+    __syncthreads();
+}
+
 // Embedding forward for one tile. s = embedding input of this lane's neighbor.
 // Out: h1 = tanh(layer1), t2, t3 = tanh of layers 2/3, G = output.
-template <int E, bool BF16>
-__device__ __forceinline__ void embed_fwd(const FusedParams &p, int pair, int lane, float s, float h1[16],
-                                          float t2[16], float t3[16], float G[16]) {
+template <int E, bool BF16, bool LW>
+__device__ __forceinline__ void embed_fwd(const float *vecb, const unsigned char *matb, int pair, int lane, float s,
+                                          float h1[16], float t2[16], float t3[16], float G[16]) {
     constexpr int NR = NReg<E>::value;
     const int h = lane >> 5;
-    const float *w1 = vec_ptr(p.vec, pair, V_W1, h), *b1 = vec_ptr(p.vec, pair, V_B1, h);
-    const float *b2 = vec_ptr(p.vec, pair, V_B2, h), *b3 = vec_ptr(p.vec, pair, V_B3, h);
-    const float *r2 = vec_ptr(p.vec, pair, V_R2, h), *r3 = vec_ptr(p.vec, pair, V_R3, h);
+    const float *w1 = vec_ptr(vecb, pair, V_W1, h), *b1 = vec_ptr(vecb, pair, V_B1, h);
+    const float *b2 = vec_ptr(vecb, pair, V_B2, h), *b3 = vec_ptr(vecb, pair, V_B3, h);
+    const float *r2 = vec_ptr(vecb, pair, V_R2, h), *r3 = vec_ptr(vecb, pair, V_R3, h);
 #pragma unroll
     for (int r = 0; r < 16; r++) h1[r] = (r < NR) ? act<BF16>(s * w1[r] + b1[r]) : 0.0f;
     floatx16 acc;
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = b2[r];
-    mfma_layer<BF16>(mat_ptr<BF16>(p.mat, pair, M_W2F), lane, h1, acc);
+    mfma_layer<BF16, LW>(mat_ptr<BF16>(matb, pair, M_W2F), lane, h1, acc);
     float h2[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -180,7 +235,7 @@ __device__ __forceinline__ void embed_fwd(const FusedParams &p, int pair, int la
     }
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = b3[r];
-    mfma_layer<BF16>(mat_ptr<BF16>(p.mat, pair, M_W3F), lane, h2, acc);
+    mfma_layer<BF16, LW>(mat_ptr<BF16>(matb, pair, M_W3F), lane, h2, acc);
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         t3[r] = (r < NR) ? act<BF16>(acc[r]) : 0.0f;
@@ -252,11 +307,11 @@ __device__ __forceinline__ bool compact(const FusedParams &p, const AtomGeo &g, 
 }
 
 // Normalized Ri of a compacted neighbor (FP32).
-__device__ __forceinline__ void ri_of(const float4 e, const float *ti, float rmin, float rmax, float Ri[4],
+__device__ __forceinline__ void ri_of(const float4 e, const float *ti, float rmin, float rmax, float iw, float Ri[4],
                                       float &r, float &s, float &ds) {
     r = sqrtf(e.x * e.x + e.y * e.y + e.z * e.z);
-    switch_fn(r, rmin, rmax, s, ds);
-    const float ir = 1.0f / r;
+    switch_fn(r, rmin, rmax, iw, s, ds);
+    const float ir = __builtin_amdgcn_rcpf(r);
     Ri[0] = (s - ti[TI_DAVG]) * ti[TI_ISTD];
     Ri[1] = (s * e.x * ir - ti[TI_DAVG + 1]) * ti[TI_ISTD + 1];
     Ri[2] = (s * e.y * ir - ti[TI_DAVG + 2]) * ti[TI_ISTD + 2];
@@ -270,14 +325,62 @@ __device__ __forceinline__ float half_sum(float v) {
     return v;
 }
 
-template <int E, bool BF16>
+// LDS layout of a fused block: [staged weights (LW only)][neighbor tiles of
+// each wave: T*mnl float4][per-wave bucket counts: MAXT ints][bwd only: per
+// wave dA of the current atom [32 features][4] floats, virial 9 doubles].
+struct BlockLds {
+    const float *vec;
+    const unsigned char *mat;
+    float4 *nb;
+    int *nvs;
+    float4 *dAs;
+    double *vacc;
+};
+
+template <bool BF16, bool LW, bool BWD>
+__device__ __forceinline__ BlockLds block_lds(const FusedParams &p, float4 *smem, int wid) {
+    unsigned char *base = reinterpret_cast<unsigned char *>(smem);
+    BlockLds b;
+    size_t off = 0;
+    if constexpr (LW) {
+        float *sv = reinterpret_cast<float *>(base);
+        unsigned char *sm = base + (size_t)p.T * p.T * NVEC * 32 * sizeof(float);
+        stage_weights<BF16>(p, sv, sm);
+        b.vec = sv;
+        b.mat = sm;
+        off = weight_lds_bytes<BF16>(p.T);
+    } else {
+        b.vec = p.vec;
+        b.mat = reinterpret_cast<const unsigned char *>(p.mat);
+    }
+    float4 *nball = reinterpret_cast<float4 *>(base + off);
+    b.nb = nball + (size_t)wid * p.T * p.mnl;
+    int *nvall = reinterpret_cast<int *>(nball + (size_t)FUSED_WAVES * p.T * p.mnl);
+    b.nvs = nvall + wid * MAXT;
+    float4 *dAall = reinterpret_cast<float4 *>(nvall + FUSED_WAVES * MAXT);
+    b.dAs = dAall + wid * 32;
+    b.vacc = reinterpret_cast<double *>(dAall + FUSED_WAVES * 32) + wid * 9;
+    (void)BWD;
+    return b;
+}
+
+template <bool BF16, bool LW, bool BWD>
+inline size_t fused_lds_bytes(int T, int mnl) {
+    size_t b = (LW ? weight_lds_bytes<BF16>(T) : 0) + sizeof(float4) * FUSED_WAVES * (size_t)T * mnl +
+               sizeof(int) * FUSED_WAVES * MAXT;
+    if (BWD) b += sizeof(float4) * FUSED_WAVES * 32 + sizeof(double) * FUSED_WAVES * 9;
+    return b;
+}
+
+template <int E, bool BF16, bool LW>
 __global__ void __launch_bounds__(FUSED_WAVES * WAVE)
 dp_fused_fwd_kernel(FusedParams p) {
     constexpr int NR = NReg<E>::value;
     extern __shared__ float4 smem[];
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-    float4 *nb = smem + (size_t)wid * p.T * p.mnl;
-    int *nvs = reinterpret_cast<int *>(smem + (size_t)FUSED_WAVES * p.T * p.mnl) + wid * MAXT;
+    const BlockLds L = block_lds<BF16, LW, false>(p, smem, wid);
+    float4 *nb = L.nb;
+    int *nvs = L.nvs;
     const int h = lane >> 5, n = lane & 31;
 
     for (int64_t i = (int64_t)blockIdx.x * FUSED_WAVES + wid; i < p.natoms; i += (int64_t)gridDim.x * FUSED_WAVES) {
@@ -296,11 +399,14 @@ dp_fused_fwd_kernel(FusedParams p) {
             const int pair = ti * p.T + t;
             for (int k0 = 0; k0 < nv; k0 += TILE) {
                 const int k = k0 + n;
+                // re-read the weights every tile instead of keeping them live across tiles
+                int tp = pair;
+                asm volatile("" : "+v"(tp));
                 float Ri[4] = {0.f, 0.f, 0.f, 0.f};
                 float r, s, ds;
-                if (k < nv) ri_of(nb[t * p.mnl + k], tinfo, p.rmin, p.rmax, Ri, r, s, ds);
+                if (k < nv) ri_of(nb[t * p.mnl + k], tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
                 float h1[16], t2[16], t3[16], G[16];
-                embed_fwd<E, BF16>(p, pair, lane, Ri[0], h1, t2, t3, G);
+                embed_fwd<E, BF16, LW>(L.vec, L.mat, tp, lane, Ri[0], h1, t2, t3, G);
                 const float live = (k < nv) ? 1.0f : 0.0f;
 #pragma unroll
                 for (int w = 0; w < 4; w++)
@@ -327,7 +433,7 @@ dp_fused_fwd_kernel(FusedParams p) {
                 float pad = 0.0f;
                 for (int t = 0; t < p.T; t++) {
                     const float npad = (float)(p.m - nvs[t]);
-                    pad += npad * vec_ptr(p.vec, ti * p.T + t, V_GPAD, h)[n];
+                    pad += npad * vec_ptr(L.vec, ti * p.T + t, V_GPAD, h)[n];
                 }
 #pragma unroll
                 for (int w = 0; w < 4; w++) out[w] += tinfo[TI_SPAD + w] * pad;
@@ -340,18 +446,19 @@ dp_fused_fwd_kernel(FusedParams p) {
     }
 }
 
-template <int E, bool BF16>
+template <int E, bool BF16, bool LW>
 __global__ void __launch_bounds__(FUSED_WAVES * WAVE)
 dp_fused_bwd_kernel(FusedParams p) {
     constexpr int NR = NReg<E>::value;
     extern __shared__ float4 smem[];
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-    float4 *nb = smem + (size_t)wid * p.T * p.mnl;
-    int *nvs = reinterpret_cast<int *>(smem + (size_t)FUSED_WAVES * p.T * p.mnl) + wid * MAXT;
+    const BlockLds L = block_lds<BF16, LW, true>(p, smem, wid);
+    float4 *nb = L.nb;
+    int *nvs = L.nvs;
+    float4 *dAs = L.dAs;        // [32 features] x (dA_0..dA_3) of the current atom, scale folded in
+    double *vacc = L.vacc;      // this wave's virial of frame vframe (written by lane 0)
     const int h = lane >> 5, n = lane & 31;
-    double vir[9];
-#pragma unroll
-    for (int v = 0; v < 9; v++) vir[v] = 0.0;
+    if (lane < 9) vacc[lane] = 0.0;
     int vframe = -1;
 
     for (int64_t i = (int64_t)blockIdx.x * FUSED_WAVES + wid; i < p.natoms; i += (int64_t)gridDim.x * FUSED_WAVES) {
@@ -360,50 +467,51 @@ dp_fused_bwd_kernel(FusedParams p) {
         const AtomGeo g = atom_geo(p, i);
         const int frame = p.atom_frame ? p.atom_frame[i] : 0;
         if (frame != vframe) {
-            // flush the per-lane virial of the previous frame
-            if (vframe >= 0) {
-#pragma unroll
-                for (int v = 0; v < 9; v++) {
-                    double x = wave_sum(vir[v]);
-                    if (lane == 0 && x != 0.0) atomic_add(p.virial + 9 * vframe + v, x);
-                    vir[v] = 0.0;
-                }
-            }
+            // flush this wave's virial of the previous frame
+            if (vframe >= 0 && lane < 9 && vacc[lane] != 0.0) atomic_add(p.virial + 9 * vframe + lane, vacc[lane]);
+            if (lane < 9) vacc[lane] = 0.0;
             vframe = frame;
         }
         compact(p, g, i, lane, nb, nvs, p.fslot);
-        __builtin_amdgcn_wave_barrier();
-        // dA for this lane's features, with the 1/(m*type_nums) scale folded in
-        float dAl[4][16];
-        const float *dAi = p.dA + (size_t)i * 4 * E;
-#pragma unroll
-        for (int rr = 0; rr < 16; rr++) {
-            const int f = feat(rr, h);
-#pragma unroll
-            for (int w = 0; w < 4; w++) dAl[w][rr] = (rr < NR && f < E) ? dAi[w * E + f] * p.scale : 0.0f;
+        {
+            const float *dAi = p.dA + (size_t)i * 4 * E;
+            for (int x = lane; x < 128; x += WAVE) {
+                const int f = x >> 2, w = x & 3;
+                reinterpret_cast<float *>(dAs)[x] = f < E ? dAi[w * E + f] * p.scale : 0.0f;
+            }
         }
+        __builtin_amdgcn_wave_barrier();
         float fc[3] = {0.f, 0.f, 0.f};     // centre-force accumulator (half 0 lanes)
+        float vf[9];                       // this atom's virial partial (half 0 lanes)
+#pragma unroll
+        for (int v = 0; v < 9; v++) vf[v] = 0.0f;
         for (int t = 0; t < p.T; t++) {
             const int nv = nvs[t];
             const int pair = ti * p.T + t;
-            const float *w1 = vec_ptr(p.vec, pair, V_W1, h);
-            const float *r2v = vec_ptr(p.vec, pair, V_R2, h), *r3v = vec_ptr(p.vec, pair, V_R3, h);
             for (int k0 = 0; k0 < nv; k0 += TILE) {
                 const int k = k0 + n;
+                // re-read the weights every tile instead of keeping them live across tiles
+                int tp = pair;
+                asm volatile("" : "+v"(tp));
+                const float *w1 = vec_ptr(L.vec, tp, V_W1, h);
+                const float *r2v = vec_ptr(L.vec, tp, V_R2, h), *r3v = vec_ptr(L.vec, tp, V_R3, h);
                 const bool live = k < nv;
                 float4 e = live ? nb[t * p.mnl + k] : make_float4(1.f, 0.f, 0.f, 0.f);
                 float Ri[4] = {0.f, 0.f, 0.f, 0.f};
                 float r = 1.f, s = 0.f, ds = 0.f;
-                if (live) ri_of(e, tinfo, p.rmin, p.rmax, Ri, r, s, ds);
+                if (live) ri_of(e, tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
                 float h1[16], t2[16], t3[16], G[16];
-                embed_fwd<E, BF16>(p, pair, lane, Ri[0], h1, t2, t3, G);
+                embed_fwd<E, BF16, LW>(L.vec, L.mat, tp, lane, Ri[0], h1, t2, t3, G);
                 // dG = dA^T Ri, dRi = dA G (partial over this half's features)
                 float dG[16], dRi[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int rr = 0; rr < 16; rr++) {
-                    dG[rr] = dAl[0][rr] * Ri[0] + dAl[1][rr] * Ri[1] + dAl[2][rr] * Ri[2] + dAl[3][rr] * Ri[3];
-#pragma unroll
-                    for (int w = 0; w < 4; w++) dRi[w] += dAl[w][rr] * G[rr];
+                    const float4 d = rr < NR ? dAs[feat(rr, h)] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    dG[rr] = d.x * Ri[0] + d.y * Ri[1] + d.z * Ri[2] + d.w * Ri[3];
+                    dRi[0] += d.x * G[rr];
+                    dRi[1] += d.y * G[rr];
+                    dRi[2] += d.z * G[rr];
+                    dRi[3] += d.w * G[rr];
                 }
                 // layer 3: G = h2 + r3 t3 -> dh2 = dG + W3 (dG r3 (1-t3^2))
                 float dz[16];
@@ -413,11 +521,11 @@ dp_fused_bwd_kernel(FusedParams p) {
                     dz[rr] = dG[rr] * r3v[rr] * (1.0f - t3[rr] * t3[rr]);
                     acc[rr] = dG[rr];
                 }
-                mfma_layer<BF16>(mat_ptr<BF16>(p.mat, pair, M_W3B), lane, dz, acc);
+                mfma_layer<BF16, LW>(mat_ptr<BF16>(L.mat, tp, M_W3B), lane, dz, acc);
                 // layer 2: h2 = h1 + r2 t2 -> dh1 = dh2 + W2 (dh2 r2 (1-t2^2))
 #pragma unroll
                 for (int rr = 0; rr < 16; rr++) dz[rr] = acc[rr] * r2v[rr] * (1.0f - t2[rr] * t2[rr]);
-                mfma_layer<BF16>(mat_ptr<BF16>(p.mat, pair, M_W2B), lane, dz, acc);
+                mfma_layer<BF16, LW>(mat_ptr<BF16>(L.mat, tp, M_W2B), lane, dz, acc);
                 // layer 1: h1 = tanh(s w1 + b1) -> ds = sum_f dh1 (1-h1^2) w1
                 float dsum = 0.0f;
 #pragma unroll
@@ -427,7 +535,7 @@ dp_fused_bwd_kernel(FusedParams p) {
                 for (int w = 0; w < 4; w++) dRi[w] += __shfl_xor(dRi[w], 32, 64);
                 // force of this slot: f_a = sum_w dE/dRi_w * Ri_d[w][a], Ri_d = -dfeat/dstd
                 if (live && h == 0) {
-                    const float ir = 1.0f / r;
+                    const float ir = __builtin_amdgcn_rcpf(r);
                     const float x[3] = {e.x, e.y, e.z};
                     float gw[4];
 #pragma unroll
@@ -452,7 +560,7 @@ dp_fused_bwd_kernel(FusedParams p) {
                     for (int a = 0; a < 3; a++) {
                         fc[a] -= f[a];
 #pragma unroll
-                        for (int c = 0; c < 3; c++) vir[3 * a + c] += (double)(f[a] * x[c]);
+                        for (int c = 0; c < 3; c++) vf[3 * a + c] += f[a] * x[c];
                     }
                 }
             }
@@ -465,20 +573,21 @@ dp_fused_bwd_kernel(FusedParams p) {
                 else atomic_add(p.force + 3 * i + a, x);
             }
         }
+        // the atom's virial (FP32 over its ~100 slots) joins the wave's FP64 frame sum
+#pragma unroll
+        for (int v = 0; v < 9; v++) {
+            const float x = wave_sum(vf[v]);
+            if (lane == 0) vacc[v] += (double)x;
+        }
         __builtin_amdgcn_wave_barrier();
     }
     // virial: wave sums -> one set of atomics per block and frame
-    __shared__ double vred[FUSED_WAVES][9];
     __shared__ int vfr[FUSED_WAVES];
-#pragma unroll
-    for (int v = 0; v < 9; v++) {
-        const double x = wave_sum(vir[v]);
-        if (lane == 0) vred[wid][v] = x;
-    }
     if (lane == 0) vfr[wid] = vframe;
     __syncthreads();
     if (threadIdx.x < 9) {
         const int v = threadIdx.x;
+        const double *vall = vacc - wid * 9;     // wave 0's row
         double acc = 0.0;
         int fr = -1;
         for (int w = 0; w < FUSED_WAVES; w++) {
@@ -488,7 +597,7 @@ dp_fused_bwd_kernel(FusedParams p) {
                 fr = vfr[w];
                 acc = 0.0;
             }
-            acc += vred[w][v];
+            acc += vall[w * 9 + v];
         }
         if (fr >= 0 && acc != 0.0) atomic_add(p.virial + 9 * fr + v, acc);
     }
@@ -522,20 +631,30 @@ gather_kernel(const int32_t *__restrict__ rev, const int32_t *__restrict__ count
     }
 }
 
-template <int E, bool BF16>
-int launch(FusedParams &p, bool fwd, hipStream_t st) {
-    const size_t lds = sizeof(float4) * FUSED_WAVES * (size_t)p.T * p.mnl + sizeof(int) * FUSED_WAVES * MAXT;
+template <int E, bool BF16, bool LW>
+int launch_lw(FusedParams &p, bool fwd, hipStream_t st) {
+    const size_t lds = fwd ? fused_lds_bytes<BF16, LW, false>(p.T, p.mnl) : fused_lds_bytes<BF16, LW, true>(p.T, p.mnl);
     MATPL_CHECK_ARG(lds <= 64 * 1024, "dp_fused: T*mnl too large for LDS (%zu bytes)", lds);
     // persistent-style grid: a few blocks per CU, each strides over atoms
     int64_t blocks = (p.natoms + FUSED_WAVES - 1) / FUSED_WAVES;
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
     if (fwd) {
-        hipLaunchKernelGGL((dp_fused_fwd_kernel<E, BF16>), dim3((unsigned)blocks), dim3(FUSED_WAVES * WAVE), lds, st, p);
+        hipLaunchKernelGGL((dp_fused_fwd_kernel<E, BF16, LW>), dim3((unsigned)blocks), dim3(FUSED_WAVES * WAVE), lds,
+                           st, p);
         return check_launch("dp_fused_fwd_kernel");
     }
-    hipLaunchKernelGGL((dp_fused_bwd_kernel<E, BF16>), dim3((unsigned)blocks), dim3(FUSED_WAVES * WAVE), lds, st, p);
+    hipLaunchKernelGGL((dp_fused_bwd_kernel<E, BF16, LW>), dim3((unsigned)blocks), dim3(FUSED_WAVES * WAVE), lds, st,
+                       p);
     return check_launch("dp_fused_bwd_kernel");
+}
+
+// Weights go to LDS when they and the neighbor tiles fit in 64 KB per block
+// (2 blocks = 8 waves per CU); otherwise they are read from global memory.
+template <int E, bool BF16>
+int launch(FusedParams &p, bool fwd, hipStream_t st) {
+    if (fused_lds_bytes<BF16, true, true>(p.T, p.mnl) <= 64 * 1024) return launch_lw<E, BF16, true>(p, fwd, st);
+    return launch_lw<E, BF16, false>(p, fwd, st);
 }
 
 int dispatch(FusedParams &p, int E, int bf16, bool fwd, hipStream_t st) {
@@ -568,6 +687,8 @@ MATPL_EXPORT int matpl_dp_fused_forward(const double *pos, const double *lattice
     p.vec = vec; p.mat = mat; p.tinfo = tinfo; p.A = A; p.status = status;
     p.natoms = natoms; p.T = ntypes; p.mnl = mnl; p.m = m;
     p.rc2 = rcut * rcut; p.rmin = (float)rcut_smooth; p.rmax = (float)rcut; p.scale = (float)scale;
+    MATPL_CHECK_ARG(rcut > rcut_smooth, "dp_fused: rcut must exceed rcut_smooth");
+    p.iw = (float)(1.0 / (rcut - rcut_smooth));
     MATPL_CHECK_ARG(natoms == 0 || A, "dp_fused_forward: null A");
     return dispatch(p, E, bf16, true, (hipStream_t)stream);
 }
@@ -585,6 +706,8 @@ MATPL_EXPORT int matpl_dp_fused_backward(const double *pos, const double *lattic
     p.vec = vec; p.mat = mat; p.tinfo = tinfo; p.dA = dA; p.force = force; p.virial = virial; p.status = status;
     p.natoms = natoms; p.T = ntypes; p.mnl = mnl; p.m = m;
     p.rc2 = rcut * rcut; p.rmin = (float)rcut_smooth; p.rmax = (float)rcut; p.scale = (float)scale;
+    MATPL_CHECK_ARG(rcut > rcut_smooth, "dp_fused: rcut must exceed rcut_smooth");
+    p.iw = (float)(1.0 / (rcut - rcut_smooth));
     MATPL_CHECK_ARG(natoms == 0 || (dA && force && virial), "dp_fused_backward: null output");
     if (rev) {
         MATPL_CHECK_ARG(fslot && fcenter && aligned16(fslot), "dp_fused_backward: gather mode needs fslot/fcenter");
