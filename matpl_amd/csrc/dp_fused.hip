@@ -714,9 +714,16 @@ MATPL_EXPORT int matpl_dp_fused_backward(const double *pos, const double *lattic
         p.fslot = reinterpret_cast<float4 *>(fslot);
         p.fcenter = fcenter;
     }
-    int rc = dispatch(p, E, bf16, false, (hipStream_t)stream);
-    if (rc || !rev || natoms == 0) return rc;
+    return dispatch(p, E, bf16, false, (hipStream_t)stream);
+}
+
+MATPL_EXPORT int matpl_dp_fused_gather(const int32_t *rev, const int32_t *counts, const float *fslot,
+                                       const float *fcenter, float *force, int ntypes, int mnl, int64_t natoms,
+                                       void *stream) {
+    if (natoms == 0) return OK;
+    MATPL_CHECK_ARG(rev && counts && fslot && fcenter && force && aligned16(fslot), "dp_fused_gather: bad pointer");
+    MATPL_CHECK_ARG(ntypes >= 1 && mnl >= 1, "dp_fused_gather: bad sizes");
     hipLaunchKernelGGL(gather_kernel, dim3(grid_cap(natoms, 4)), dim3(256), 0, (hipStream_t)stream, rev, counts,
-                       p.fslot, fcenter, force, ntypes, mnl, natoms);
+                       reinterpret_cast<const float4 *>(fslot), fcenter, force, ntypes, mnl, natoms);
     return check_launch("gather_kernel");
 }
