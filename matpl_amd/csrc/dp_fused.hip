@@ -94,6 +94,7 @@ struct FusedParams {
     int32_t *status;            // 1 + first atom whose in-cutoff count exceeds m
     int64_t natoms;
     int T, mnl, m;
+    int cap;                    // LDS tile capacity per type = min(m, mnl)
     double rc2;                 // model cutoff^2 (FP64 test)
     float rmin, rmax, scale;    // scale = 1 / (m * type_nums)
     float iw;                   // 1 / (rmax - rmin)
@@ -272,7 +273,7 @@ __device__ __forceinline__ int64_t neighbor_disp(const FusedParams &p, const Ato
     return j;
 }
 
-// Compacts the in-cutoff neighbors of atom i into LDS: nb[t*mnl + c] =
+// Compacts the in-cutoff neighbors of atom i into LDS: nb[t*cap + c] =
 // (dx, dy, dz, bucket slot k as float bits); nv[t] = count. Used bucket
 // entries beyond the cutoff get a zero per-slot force when fslot is given.
 // Returns false on overflow.
@@ -297,10 +298,11 @@ __device__ __forceinline__ bool compact(const FusedParams &p, const AtomGeo &g, 
             }
             const uint64_t bal = __ballot(ok);
             const int idx = n + __popcll(bal & ((1ull << lane) - 1ull));
-            if (ok) nb[t * p.mnl + idx] = e;
+            if (ok && idx < p.cap) nb[t * p.cap + idx] = e;
             n += __popcll(bal);
         }
-        if (lane == 0) nv[t] = n;
+        // more than m in-cutoff neighbors is an error the caller reports; only cap of them are used
+        if (lane == 0) nv[t] = n < p.cap ? n : p.cap;
         if (n > p.m) ok_all = false;
     }
     return ok_all;
@@ -324,6 +326,37 @@ __device__ __forceinline__ float half_sum(float v) {
     for (int off = 1; off < 32; off <<= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
+
+// The three tanh-layer gates r (1 - t^2) of a tile, kept from the forward
+// recompute until the backward uses them. In BF16 mode they are stored as
+// packed bf16 pairs (the backward rounds its MFMA operands to bf16 anyway),
+// which halves the registers they hold.
+template <bool PACK>
+struct Gate16;
+template <>
+struct Gate16<true> {
+    uint32_t v[8];
+    __device__ __forceinline__ void set(const float g[16]) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const __bf16 a = (__bf16)g[2 * j], b = (__bf16)g[2 * j + 1];
+            v[j] = (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+        }
+    }
+    __device__ __forceinline__ float get(int r) const {
+        const uint32_t w = v[r >> 1];
+        return (r & 1) ? __uint_as_float(w & 0xffff0000u) : __uint_as_float(w << 16);
+    }
+};
+template <>
+struct Gate16<false> {
+    float v[16];
+    __device__ __forceinline__ void set(const float g[16]) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = g[r];
+    }
+    __device__ __forceinline__ float get(int r) const { return v[r]; }
+};
 
 // LDS layout of a fused block: [staged weights (LW only)][neighbor tiles of
 // each wave: T*mnl float4][per-wave bucket counts: MAXT ints][bwd only: per
@@ -354,8 +387,8 @@ __device__ __forceinline__ BlockLds block_lds(const FusedParams &p, float4 *smem
         b.mat = reinterpret_cast<const unsigned char *>(p.mat);
     }
     float4 *nball = reinterpret_cast<float4 *>(base + off);
-    b.nb = nball + (size_t)wid * p.T * p.mnl;
-    int *nvall = reinterpret_cast<int *>(nball + (size_t)FUSED_WAVES * p.T * p.mnl);
+    b.nb = nball + (size_t)wid * p.T * p.cap;
+    int *nvall = reinterpret_cast<int *>(nball + (size_t)FUSED_WAVES * p.T * p.cap);
     b.nvs = nvall + wid * MAXT;
     float4 *dAall = reinterpret_cast<float4 *>(nvall + FUSED_WAVES * MAXT);
     b.dAs = dAall + wid * 32;
@@ -365,8 +398,8 @@ __device__ __forceinline__ BlockLds block_lds(const FusedParams &p, float4 *smem
 }
 
 template <bool BF16, bool LW, bool BWD>
-inline size_t fused_lds_bytes(int T, int mnl) {
-    size_t b = (LW ? weight_lds_bytes<BF16>(T) : 0) + sizeof(float4) * FUSED_WAVES * (size_t)T * mnl +
+inline size_t fused_lds_bytes(int T, int cap) {
+    size_t b = (LW ? weight_lds_bytes<BF16>(T) : 0) + sizeof(float4) * FUSED_WAVES * (size_t)T * cap +
                sizeof(int) * FUSED_WAVES * MAXT;
     if (BWD) b += sizeof(float4) * FUSED_WAVES * 32 + sizeof(double) * FUSED_WAVES * 9;
     return b;
@@ -404,7 +437,7 @@ dp_fused_fwd_kernel(FusedParams p) {
                 asm volatile("" : "+v"(tp));
                 float Ri[4] = {0.f, 0.f, 0.f, 0.f};
                 float r, s, ds;
-                if (k < nv) ri_of(nb[t * p.mnl + k], tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
+                if (k < nv) ri_of(nb[t * p.cap + k], tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
                 float h1[16], t2[16], t3[16], G[16];
                 embed_fwd<E, BF16, LW>(L.vec, L.mat, tp, lane, Ri[0], h1, t2, t3, G);
                 const float live = (k < nv) ? 1.0f : 0.0f;
@@ -496,12 +529,26 @@ dp_fused_bwd_kernel(FusedParams p) {
                 const float *w1 = vec_ptr(L.vec, tp, V_W1, h);
                 const float *r2v = vec_ptr(L.vec, tp, V_R2, h), *r3v = vec_ptr(L.vec, tp, V_R3, h);
                 const bool live = k < nv;
-                float4 e = live ? nb[t * p.mnl + k] : make_float4(1.f, 0.f, 0.f, 0.f);
+                float4 e = live ? nb[t * p.cap + k] : make_float4(1.f, 0.f, 0.f, 0.f);
                 float Ri[4] = {0.f, 0.f, 0.f, 0.f};
                 float r = 1.f, s = 0.f, ds = 0.f;
                 if (live) ri_of(e, tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
-                float h1[16], t2[16], t3[16], G[16];
-                embed_fwd<E, BF16, LW>(L.vec, L.mat, tp, lane, Ri[0], h1, t2, t3, G);
+                float G[16];
+                Gate16<BF16> g1, g2, g3;
+                {
+                    float h1[16], t2[16], t3[16];
+                    embed_fwd<E, BF16, LW>(L.vec, L.mat, tp, lane, Ri[0], h1, t2, t3, G);
+                    float q[16];
+#pragma unroll
+                    for (int rr = 0; rr < 16; rr++) q[rr] = rr < NR ? r3v[rr] * (1.0f - t3[rr] * t3[rr]) : 0.0f;
+                    g3.set(q);
+#pragma unroll
+                    for (int rr = 0; rr < 16; rr++) q[rr] = rr < NR ? r2v[rr] * (1.0f - t2[rr] * t2[rr]) : 0.0f;
+                    g2.set(q);
+#pragma unroll
+                    for (int rr = 0; rr < 16; rr++) q[rr] = rr < NR ? w1[rr] * (1.0f - h1[rr] * h1[rr]) : 0.0f;
+                    g1.set(q);
+                }
                 // dG = dA^T Ri, dRi = dA G (partial over this half's features)
                 float dG[16], dRi[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -518,18 +565,18 @@ dp_fused_bwd_kernel(FusedParams p) {
                 floatx16 acc;
 #pragma unroll
                 for (int rr = 0; rr < 16; rr++) {
-                    dz[rr] = dG[rr] * r3v[rr] * (1.0f - t3[rr] * t3[rr]);
+                    dz[rr] = dG[rr] * g3.get(rr);
                     acc[rr] = dG[rr];
                 }
                 mfma_layer<BF16, LW>(mat_ptr<BF16>(L.mat, tp, M_W3B), lane, dz, acc);
                 // layer 2: h2 = h1 + r2 t2 -> dh1 = dh2 + W2 (dh2 r2 (1-t2^2))
 #pragma unroll
-                for (int rr = 0; rr < 16; rr++) dz[rr] = acc[rr] * r2v[rr] * (1.0f - t2[rr] * t2[rr]);
+                for (int rr = 0; rr < 16; rr++) dz[rr] = acc[rr] * g2.get(rr);
                 mfma_layer<BF16, LW>(mat_ptr<BF16>(L.mat, tp, M_W2B), lane, dz, acc);
                 // layer 1: h1 = tanh(s w1 + b1) -> ds = sum_f dh1 (1-h1^2) w1
                 float dsum = 0.0f;
 #pragma unroll
-                for (int rr = 0; rr < NR; rr++) dsum += acc[rr] * (1.0f - h1[rr] * h1[rr]) * w1[rr];
+                for (int rr = 0; rr < NR; rr++) dsum += acc[rr] * g1.get(rr);
                 dRi[0] += dsum;
 #pragma unroll
                 for (int w = 0; w < 4; w++) dRi[w] += __shfl_xor(dRi[w], 32, 64);
@@ -633,7 +680,7 @@ gather_kernel(const int32_t *__restrict__ rev, const int32_t *__restrict__ count
 
 template <int E, bool BF16, bool LW>
 int launch_lw(FusedParams &p, bool fwd, hipStream_t st) {
-    const size_t lds = fwd ? fused_lds_bytes<BF16, LW, false>(p.T, p.mnl) : fused_lds_bytes<BF16, LW, true>(p.T, p.mnl);
+    const size_t lds = fwd ? fused_lds_bytes<BF16, LW, false>(p.T, p.cap) : fused_lds_bytes<BF16, LW, true>(p.T, p.cap);
     MATPL_CHECK_ARG(lds <= 64 * 1024, "dp_fused: T*mnl too large for LDS (%zu bytes)", lds);
     // persistent-style grid: a few blocks per CU, each strides over atoms
     int64_t blocks = (p.natoms + FUSED_WAVES - 1) / FUSED_WAVES;
@@ -653,13 +700,14 @@ int launch_lw(FusedParams &p, bool fwd, hipStream_t st) {
 // (2 blocks = 8 waves per CU); otherwise they are read from global memory.
 template <int E, bool BF16>
 int launch(FusedParams &p, bool fwd, hipStream_t st) {
-    if (fused_lds_bytes<BF16, true, true>(p.T, p.mnl) <= 64 * 1024) return launch_lw<E, BF16, true>(p, fwd, st);
+    if (fused_lds_bytes<BF16, true, true>(p.T, p.cap) <= 64 * 1024) return launch_lw<E, BF16, true>(p, fwd, st);
     return launch_lw<E, BF16, false>(p, fwd, st);
 }
 
 int dispatch(FusedParams &p, int E, int bf16, bool fwd, hipStream_t st) {
     MATPL_CHECK_ARG(p.T >= 1 && p.T <= MAXT, "dp_fused: ntypes must be in [1,%d]", MAXT);
     MATPL_CHECK_ARG(p.mnl >= 1 && p.m >= 1, "dp_fused: bad bucket sizes");
+    p.cap = p.m < p.mnl ? p.m : p.mnl;
     if (p.natoms == 0) return OK;
     MATPL_CHECK_ARG(p.pos && p.lattice && p.list && p.shift && p.counts && p.type_map && p.vec && p.mat && p.tinfo &&
                         p.status, "dp_fused: null pointer");
