@@ -328,9 +328,9 @@ __device__ __forceinline__ float half_sum(float v) {
 }
 
 // The three tanh-layer gates r (1 - t^2) of a tile, kept from the forward
-// recompute until the backward uses them. In BF16 mode they are stored as
-// packed bf16 pairs (the backward rounds its MFMA operands to bf16 anyway),
-// which halves the registers they hold.
+// recompute until the backward uses them: FP32, or packed bf16 pairs (half
+// the registers; measured slower on gfx950 at no register saving for E = 25,
+// so the kernels use FP32).
 template <bool PACK>
 struct Gate16;
 template <>
@@ -534,7 +534,7 @@ dp_fused_bwd_kernel(FusedParams p) {
                 float r = 1.f, s = 0.f, ds = 0.f;
                 if (live) ri_of(e, tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
                 float G[16];
-                Gate16<BF16> g1, g2, g3;
+                Gate16<false> g1, g2, g3;
                 {
                     float h1[16], t2[16], t3[16];
                     embed_fwd<E, BF16, LW>(L.vec, L.mat, tp, lane, Ri[0], h1, t2, t3, G);
