@@ -46,6 +46,43 @@ struct FitParams {
     int ntiles, E, D, Dh, nb0, L;
 };
 
+// tanh: in BF16 mode 1 - 2 / (2^(2x log2 e) + 1) with the hardware exp2 and
+// reciprocal (relative error ~1e-7, far below the bf16 operand rounding);
+// in FP32 mode the library tanhf.
+template <bool BF16>
+__device__ __forceinline__ float act(float x) {
+    if constexpr (BF16) {
+        const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+        return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+    } else {
+        return tanhf(x);
+    }
+}
+
+// Per-layer tanh gates (1 - t^2) [rdt] kept for the backward: packed bf16
+// pairs in BF16 mode (the backward rounds its MFMA operands to bf16 anyway),
+// FP32 otherwise.
+template <bool PACK>
+struct Gates16;
+template <>
+struct Gates16<true> {
+    uint32_t v[8];
+    __device__ __forceinline__ void set(int r, float g0, float g1) {
+        const __bf16 a = (__bf16)g0, b = (__bf16)g1;
+        v[r >> 1] = (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+    }
+    __device__ __forceinline__ float get(int r) const {
+        const uint32_t w = v[r >> 1];
+        return (r & 1) ? __uint_as_float(w & 0xffff0000u) : __uint_as_float(w << 16);
+    }
+};
+template <>
+struct Gates16<false> {
+    float v[16];
+    __device__ __forceinline__ void set(int r, float g0, float g1) { v[r] = g0; v[r + 1] = g1; }
+    __device__ __forceinline__ float get(int r) const { return v[r]; }
+};
+
 template <bool BF16>
 __device__ __forceinline__ void mma(floatx16 &acc, const void *w, int idx, const float *x) {
     // idx: chunk index; BF16: 8 values per chunk, FP32: 1 value per chunk
@@ -86,16 +123,21 @@ __global__ void __launch_bounds__(WAVE)
 dp_fitting_kernel(FitParams p) {
     const int lane = threadIdx.x;
     const int n = lane & 31, h = lane >> 5;
-    __shared__ float As[32][4][E_MAX];       // A of the tile's atoms
-    __shared__ float dAs[32][4][E_MAX];      // dA accumulation
+    // A and the dA accumulation of the tile's 32 atoms; one row of 4E values per
+    // atom, row stride 4E + 1 (odd) so that lanes reading the same element of
+    // different atoms hit different banks
+    extern __shared__ float fit_smem[];
+    const int E = p.E, RS = 4 * E + 1;
+    float *As = fit_smem, *dAs = fit_smem + 32 * RS;
+#define AS(a, w, c) As[(a) * RS + (w) * E + (c)]
+#define DAS(a, w, c) dAs[(a) * RS + (w) * E + (c)]
     for (int tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
         const int t = p.tiles[3 * tile], a0 = p.tiles[3 * tile + 1], cnt = p.tiles[3 * tile + 2];
-        const int E = p.E;
         // stage A and clear dA
-        for (int x = lane; x < 32 * 4 * E_MAX; x += WAVE) {
-            const int an = x / (4 * E_MAX), w = (x / E_MAX) % 4, c = x % E_MAX;
-            As[an][w][c] = (an < cnt && c < E) ? p.A[((int64_t)(a0 + an) * 4 + w) * E + c] : 0.0f;
-            dAs[an][w][c] = 0.0f;
+        for (int x = lane; x < 32 * 4 * E; x += WAVE) {
+            const int an = x / (4 * E), wc = x - an * 4 * E;
+            As[an * RS + wc] = an < cnt ? p.A[(int64_t)(a0 + an) * 4 * E + wc] : 0.0f;
+            dAs[an * RS + wc] = 0.0f;
         }
         __syncthreads();
         const void *w0f = reinterpret_cast<const char *>(p.w0f) + (size_t)t * p.type_stride_w0f * (BF16 ? 2 : 4);
@@ -117,10 +159,10 @@ dp_fitting_kernel(FitParams p) {
 #pragma unroll
             for (int w = 0; w < 4; w++)
 #pragma unroll
-                for (int q = 0; q < 8; q++) aq[w][q] = As[n][w][8 * h + q];
+                for (int q = 0; q < 8; q++) aq[w][q] = AS(n, w, 8 * h + q);
             for (int s = 0; s < E; s++) {
                 float drv[8];
-                const float ap0 = As[n][0][s], ap1 = As[n][1][s], ap2 = As[n][2][s], ap3 = As[n][3][s];
+                const float ap0 = AS(n, 0, s), ap1 = AS(n, 1, s), ap2 = AS(n, 2, s), ap3 = AS(n, 3, s);
 #pragma unroll
                 for (int q = 0; q < 8; q++) drv[q] = ap0 * aq[0][q] + ap1 * aq[1][q] + ap2 * aq[2][q] + ap3 * aq[3][q];
 #pragma unroll
@@ -133,20 +175,25 @@ dp_fitting_kernel(FitParams p) {
             for (int s = 0; s < p.D / 2; s++) {
                 const int k = 2 * s + h;
                 const int pp = k / M2, qq = k % M2;
-                float drv = As[n][0][pp] * As[n][0][qq] + As[n][1][pp] * As[n][1][qq] +
-                            As[n][2][pp] * As[n][2][qq] + As[n][3][pp] * As[n][3][qq];
+                float drv = AS(n, 0, pp) * AS(n, 0, qq) + AS(n, 1, pp) * AS(n, 1, qq) +
+                            AS(n, 2, pp) * AS(n, 2, qq) + AS(n, 3, pp) * AS(n, 3, qq);
 #pragma unroll
                 for (int ob = 0; ob < 2; ob++)
                     mma<false>(acc[ob], reinterpret_cast<const char *>(w0f) + ((size_t)ob * 64 + lane) * p.Dh * 4, s,
                                &drv);
             }
         }
-        float tk[L][2][16];         // tanh outputs per layer (for the backward)
+        Gates16<BF16> gk[L][2];     // gates per layer: (1 - t^2) for layer 0, rdt (1 - t^2) after
         float x[2][16];
 #pragma unroll
         for (int ob = 0; ob < 2; ob++)
 #pragma unroll
-            for (int r = 0; r < 16; r++) { tk[0][ob][r] = tanhf(acc[ob][r]); x[ob][r] = tk[0][ob][r]; }
+            for (int r = 0; r < 16; r += 2) {
+                const float ta = act<BF16>(acc[ob][r]), tb = act<BF16>(acc[ob][r + 1]);
+                x[ob][r] = ta;
+                x[ob][r + 1] = tb;
+                gk[0][ob].set(r, 1.0f - ta * ta, 1.0f - tb * tb);
+            }
         // ---- hidden residual layers
 #pragma unroll
         for (int l = 1; l < L; l++) {
@@ -159,9 +206,11 @@ dp_fitting_kernel(FitParams p) {
             for (int ob = 0; ob < 2; ob++) {
                 const float *rdt = vptr(vec, l, 1, ob, h);
 #pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    tk[l][ob][r] = tanhf(acc[ob][r]);
-                    x[ob][r] += rdt[r] * tk[l][ob][r];
+                for (int r = 0; r < 16; r += 2) {
+                    const float ta = act<BF16>(acc[ob][r]), tb = act<BF16>(acc[ob][r + 1]);
+                    x[ob][r] += rdt[r] * ta;
+                    x[ob][r + 1] += rdt[r + 1] * tb;
+                    gk[l][ob].set(r, rdt[r] * (1.0f - ta * ta), rdt[r + 1] * (1.0f - tb * tb));
                 }
             }
         }
@@ -186,10 +235,9 @@ dp_fitting_kernel(FitParams p) {
             float dz[2][16];
 #pragma unroll
             for (int ob = 0; ob < 2; ob++) {
-                const float *rdt = vptr(vec, l, 1, ob, h);
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
-                    dz[ob][r] = dx[ob][r] * rdt[r] * (1.0f - tk[l][ob][r] * tk[l][ob][r]);
+                    dz[ob][r] = dx[ob][r] * gk[l][ob].get(r);
                     acc[ob][r] = dx[ob][r];
                 }
             }
@@ -203,19 +251,14 @@ dp_fitting_kernel(FitParams p) {
 #pragma unroll
         for (int ob = 0; ob < 2; ob++)
 #pragma unroll
-            for (int r = 0; r < 16; r++) dz0[ob][r] = dx[ob][r] * (1.0f - tk[0][ob][r] * tk[0][ob][r]);
+            for (int r = 0; r < 16; r++) dz0[ob][r] = dx[ob][r] * gk[0][ob].get(r);
         // ---- layer 0 backward, block by block: dDR rows 32 blk + f(r,h)
         //   k = p*16 + q: r < 8 -> p = 2 blk, q = f(r,h); r >= 8 -> p = 2 blk + 1, q = f(r-8,h)
         float aq2[4][8];            // A[w][f(r,h)] for r < 8
 #pragma unroll
         for (int w = 0; w < 4; w++)
 #pragma unroll
-            for (int r = 0; r < 8; r++) aq2[w][r] = As[n][w][feat(r, h)];
-        float d2[4][8];             // dA[w][q] (second term), q = f(r,h)
-#pragma unroll
-        for (int w = 0; w < 4; w++)
-#pragma unroll
-            for (int r = 0; r < 8; r++) d2[w][r] = 0.0f;
+            for (int r = 0; r < 8; r++) aq2[w][r] = AS(n, w, feat(r, h));
         for (int blk = 0; blk < p.nb0; blk++) {
             floatx16 g;
 #pragma unroll
@@ -229,17 +272,19 @@ dp_fitting_kernel(FitParams p) {
             }
             const int p0 = 2 * blk, p1 = 2 * blk + 1;
             float s0[4] = {0.f, 0.f, 0.f, 0.f}, s1[4] = {0.f, 0.f, 0.f, 0.f};
-            const float ap0[4] = {p0 < E ? As[n][0][p0] : 0.f, p0 < E ? As[n][1][p0] : 0.f,
-                                  p0 < E ? As[n][2][p0] : 0.f, p0 < E ? As[n][3][p0] : 0.f};
-            const float ap1[4] = {p1 < E ? As[n][0][p1] : 0.f, p1 < E ? As[n][1][p1] : 0.f,
-                                  p1 < E ? As[n][2][p1] : 0.f, p1 < E ? As[n][3][p1] : 0.f};
+            const float ap0[4] = {p0 < E ? AS(n, 0, p0) : 0.f, p0 < E ? AS(n, 1, p0) : 0.f,
+                                  p0 < E ? AS(n, 2, p0) : 0.f, p0 < E ? AS(n, 3, p0) : 0.f};
+            const float ap1[4] = {p1 < E ? AS(n, 0, p1) : 0.f, p1 < E ? AS(n, 1, p1) : 0.f,
+                                  p1 < E ? AS(n, 2, p1) : 0.f, p1 < E ? AS(n, 3, p1) : 0.f};
 #pragma unroll
             for (int r = 0; r < 8; r++) {
+                // dA[w][q] of the q-side (q = f(r,h)) goes straight to this lane's LDS row
+                const int q = feat(r, h);
 #pragma unroll
                 for (int w = 0; w < 4; w++) {
                     s0[w] += g[r] * aq2[w][r];
                     s1[w] += g[r + 8] * aq2[w][r];
-                    d2[w][r] += g[r] * ap0[w] + g[r + 8] * ap1[w];
+                    DAS(n, w, q) += g[r] * ap0[w] + g[r + 8] * ap1[w];
                 }
             }
 #pragma unroll
@@ -249,27 +294,23 @@ dp_fitting_kernel(FitParams p) {
             }
             if (h == 0 && p0 < E) {
 #pragma unroll
-                for (int w = 0; w < 4; w++) dAs[n][w][p0] += s0[w];
+                for (int w = 0; w < 4; w++) DAS(n, w, p0) += s0[w];
             }
             if (h == 1 && p1 < E) {
 #pragma unroll
-                for (int w = 0; w < 4; w++) dAs[n][w][p1] += s1[w];
+                for (int w = 0; w < 4; w++) DAS(n, w, p1) += s1[w];
             }
         }
         __syncthreads();
-#pragma unroll
-        for (int r = 0; r < 8; r++) {
-            const int q = feat(r, h);
-#pragma unroll
-            for (int w = 0; w < 4; w++) dAs[n][w][q] += d2[w][r];
-        }
         __syncthreads();
-        for (int x2 = lane; x2 < 32 * 4 * E_MAX; x2 += WAVE) {
-            const int an = x2 / (4 * E_MAX), w = (x2 / E_MAX) % 4, c = x2 % E_MAX;
-            if (an < cnt && c < E) p.dA[((int64_t)(a0 + an) * 4 + w) * E + c] = dAs[an][w][c];
+        for (int x2 = lane; x2 < 32 * 4 * E; x2 += WAVE) {
+            const int an = x2 / (4 * E), wc = x2 - an * 4 * E;
+            if (an < cnt) p.dA[(int64_t)(a0 + an) * 4 * E + wc] = dAs[an * RS + wc];
         }
         __syncthreads();
     }
+#undef AS
+#undef DAS
 }
 
 }  // namespace
@@ -281,7 +322,7 @@ MATPL_EXPORT int matpl_dp_fitting(const float *A, const int32_t *tiles, int ntil
                                   const void *whf, const void *whb, const float *vec, int64_t type_stride_w0f,
                                   int64_t type_stride_w0b, int64_t type_stride_wh, int64_t type_stride_vec, int E,
                                   int nlayers, int bf16, float *Ei, float *dA, void *stream) {
-    MATPL_CHECK_ARG(E >= 1 && E <= E_MAX, "dp_fitting: E must be in [1,%d]", E_MAX);
+    MATPL_CHECK_ARG(E >= M2 && E <= E_MAX, "dp_fitting: E must be in [%d,%d]", M2, E_MAX);
     MATPL_CHECK_ARG(nlayers >= 1 && nlayers <= 4, "dp_fitting: hidden layers must be in [1,4]");
     if (ntiles <= 0) return OK;
     MATPL_CHECK_ARG(A && tiles && w0f && w0b && vec && Ei && dA && (nlayers == 1 || (whf && whb)),
@@ -293,11 +334,12 @@ MATPL_EXPORT int matpl_dp_fitting(const float *A, const int32_t *tiles, int ntil
     p.type_stride_wh = type_stride_wh; p.type_stride_vec = type_stride_vec;
     p.ntiles = ntiles; p.E = E; p.D = E * M2; p.Dh = (E * M2) / 2; p.nb0 = (E * M2 + 31) / 32; p.L = nlayers;
     const int grid = ntiles < 8192 ? ntiles : 8192;
+    const size_t lds = sizeof(float) * 2 * 32 * (4 * E + 1);
     hipStream_t st = (hipStream_t)stream;
 #define MATPL_FIT_CASE(LL)                                                                              \
     case LL:                                                                                            \
-        if (bf16) hipLaunchKernelGGL((dp_fitting_kernel<true, LL>), dim3(grid), dim3(WAVE), 0, st, p);  \
-        else hipLaunchKernelGGL((dp_fitting_kernel<false, LL>), dim3(grid), dim3(WAVE), 0, st, p);      \
+        if (bf16) hipLaunchKernelGGL((dp_fitting_kernel<true, LL>), dim3(grid), dim3(WAVE), lds, st, p);\
+        else hipLaunchKernelGGL((dp_fitting_kernel<false, LL>), dim3(grid), dim3(WAVE), lds, st, p);\
         break;
     switch (nlayers) {
         MATPL_FIT_CASE(1)
