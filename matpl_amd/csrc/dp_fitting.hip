@@ -59,9 +59,9 @@ __device__ __forceinline__ float act(float x) {
     }
 }
 
-// Per-layer tanh gates (1 - t^2) [rdt] kept for the backward: packed bf16
-// pairs in BF16 mode (the backward rounds its MFMA operands to bf16 anyway),
-// FP32 otherwise.
+// Per-layer tanh gates (1 - t^2) [rdt] kept for the backward, FP32 (packed
+// bf16 pairs cost dA accuracy beyond the bf16-operand level and saved no
+// registers here, so the kernel uses FP32).
 template <bool PACK>
 struct Gates16;
 template <>
@@ -183,7 +183,7 @@ dp_fitting_kernel(FitParams p) {
                                &drv);
             }
         }
-        Gates16<BF16> gk[L][2];     // gates per layer: (1 - t^2) for layer 0, rdt (1 - t^2) after
+        Gates16<false> gk[L][2];    // gates per layer: (1 - t^2) for layer 0, rdt (1 - t^2) after
         float x[2][16];
 #pragma unroll
         for (int ob = 0; ob < 2; ob++)
