@@ -32,8 +32,11 @@ def hipcc() -> str:
 
 
 def cflags():
+    # -amdgpu-mfma-vgpr-form: matrix-core accumulators in ordinary VGPRs (no AGPR round trips
+    # between the MFMA layers and the activations around them)
     return ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-fvisibility=hidden",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-mllvm", "-amdgpu-mfma-vgpr-form",
             "-Wno-unused-result", "-I", CSRC]
 
 

@@ -100,16 +100,16 @@ struct FusedParams {
     float iw;                   // 1 / (rmax - rmin)
 };
 
-__device__ __forceinline__ float fast_tanh(float x) {
-    // tanh x = 1 - 2 / (2^(2x log2 e) + 1); saturates correctly at +-inf
-    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
-}
+// In BF16 mode the forward weights and biases of the embedding net come
+// pre-multiplied by TANH_K = 2 log2(e) (matpl_amd/dp_fused.py), so a layer's
+// pre-activation y = TANH_K z and tanh z = 1 - 2 / (2^y + 1): one hardware
+// exp2 and one reciprocal, saturating correctly at +-inf.
+constexpr float TANH_K = 2.8853900817779268f;
 
 template <bool BF16>
-__device__ __forceinline__ float act(float x) {
-    if constexpr (BF16) return fast_tanh(x);
-    else return tanhf(x);
+__device__ __forceinline__ float act(float y) {
+    if constexpr (BF16) return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(y) + 1.0f);
+    else return tanhf(y);
 }
 
 // s(r) and ds/dr, dp_dp.py:244-249 (FP32); iw = 1 / (rmax - rmin)
@@ -577,6 +577,7 @@ dp_fused_bwd_kernel(FusedParams p) {
                 float dsum = 0.0f;
 #pragma unroll
                 for (int rr = 0; rr < NR; rr++) dsum += acc[rr] * g1.get(rr);
+                if constexpr (BF16) dsum *= 1.0f / TANH_K;     // w1 is stored pre-scaled
                 dRi[0] += dsum;
 #pragma unroll
                 for (int w = 0; w < 4; w++) dRi[w] += __shfl_xor(dRi[w], 32, 64);

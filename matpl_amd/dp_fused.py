@@ -27,6 +27,7 @@ from .dp_fitting import FittingMFMA
 from .dpnet import DPWeights
 
 NVEC, NMAT, NTI = 7, 4, 16
+TANH_K = 2.0 / np.log(2.0)      # 2 log2(e): BF16-mode pre-activation scale (see dp_fused.hip)
 V_W1, V_B1, V_B2, V_B3, V_R2, V_R3, V_GPAD = range(NVEC)
 M_W2F, M_W3F, M_W2B, M_W3B = range(NMAT)
 
@@ -102,6 +103,13 @@ def pack_fused(w: DPWeights, precision: str = "bf16", device="cuda") -> FusedWei
                     mat[p, M_W3F, lane, r] = W3[f, i]
                     mat[p, M_W2B, lane, r] = W2[i, f]
                     mat[p, M_W3B, lane, r] = W3[i, f]
+    if bf16:
+        # tanh(z) is evaluated as 1 - 2 / (2^y + 1) with y = TANH_K z: pre-scale the forward weights and
+        # biases of every tanh layer (the backward matrices and the padding vector stay unscaled)
+        for k in (V_W1, V_B1, V_B2, V_B3):
+            vec[:, k] *= TANH_K
+        mat[:, M_W2F] *= TANH_K
+        mat[:, M_W3F] *= TANH_K
     mt = torch.as_tensor(mat, device=device)
     return FusedWeights(torch.as_tensor(vec, device=device), mt.to(torch.bfloat16) if bf16 else mt,
                         torch.as_tensor(tinfo, device=device), E, bf16)
