@@ -160,10 +160,11 @@ int matpl_dp_calc_stat(const int32_t *list, const double *dR, const int32_t *typ
  * virial [F][9] (float64, accumulated: zero it first) the
  * calculateVirial-convention virial. With rev (from
  * matpl_dp_neighbor_reverse) the neighbor forces are gathered without
- * atomics: the backward writes the per-slot neighbor forces fslot
- * [N][T][mnl][4] float (16-byte aligned) and the centre terms fcenter [N][3],
- * and matpl_dp_fused_gather then overwrites force; with rev == NULL the
- * backward adds them to force with atomics (force zeroed first).
+ * atomics: the backward writes the force of each slot (i -> j) into j's
+ * entry naming i, fslot [N][T][mnl][4] float (16-byte aligned), and the
+ * centre terms fcenter [N][3], and matpl_dp_fused_gather then overwrites
+ * force; with rev == NULL the backward adds them to force with atomics
+ * (force zeroed first).
  * Together they replace DP.calculate_Ri + calculate_xyz_scater + the
  * autograd dE/dRi + calculateForce/calculateVirial of the reference
  * (src/model/dp_dp.py:217-469, src/model/dp_embedding.py:40-94); the fitting
@@ -183,11 +184,11 @@ int matpl_dp_fused_backward(const double *pos, const double *lattice, const int3
                             double rcut, double rcut_smooth, double scale, const float *dA, const int32_t *rev,
                             float *fslot, float *fcenter, float *force, double *virial, int32_t *status,
                             void *stream);
-/* force[i] = fcenter[i] + sum of fslot[rev[s]] over the used slots s of atom i
+/* force[i] = fcenter[i] + sum of fslot over the used entries of atom i's list
  * (counts [N][T]), in a fixed order (deterministic, no atomics). No reference
  * counterpart (the reference scatters with atomics, calculateForce.cu:92). */
-int matpl_dp_fused_gather(const int32_t *rev, const int32_t *counts, const float *fslot, const float *fcenter,
-                          float *force, int ntypes, int mnl, int64_t natoms, void *stream);
+int matpl_dp_fused_gather(const int32_t *counts, const float *fslot, const float *fcenter, float *force, int ntypes,
+                          int mnl, int64_t natoms, void *stream);
 
 /*
  * Fitting net of the fused path on the matrix cores: A [N][4][E] ->

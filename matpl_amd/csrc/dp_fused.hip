@@ -20,11 +20,11 @@
 //         dE/dRi per neighbor, contracts it with the analytic dRi/dx
 //         (dp_dp.py:266-299) and produces the virial V[a][c] += f_a dR_c
 //         (FP64 per frame; calculateVirial.cu:80) and the forces. With a
-//         reverse-slot map (neighbor list symmetric) each slot's neighbor
-//         force is stored in its list slot and gather_kernel sums, for every
-//         atom, its centre term and the slots that point back at it, in a
-//         fixed order and without atomics (device-scope atomics leave the
-//         per-XCD L2 on this part and throttle the scatter); without one,
+//         reverse-slot map (neighbor list symmetric) the force of slot
+//         (i -> j) is stored in j's entry naming i, and gather_kernel sums,
+//         for every atom, its centre term and its own entries, contiguously,
+//         in a fixed order and without atomics (device-scope atomics leave
+//         the per-XCD L2 on this part and throttle the scatter); without one,
 //         FP32 atomics at j are used.
 //
 // Matrix-core layout. Activations live as [feature x neighbor] tiles in the
@@ -81,6 +81,7 @@ struct FusedParams {
     const int32_t *list;        // [N][T][mnl] 1-based within frame, 0 = empty
     const int32_t *shift;       // [N][T][mnl] packed image
     const int32_t *counts;      // [N][T] entries used per bucket
+    const int32_t *rev;         // [N][T][mnl] partner slot of each used entry (bwd gather mode)
     const int32_t *type_map;    // [N]
     const float *vec;           // [T*T][NVEC][2][16]
     const void *mat;            // [T*T][NMAT][64][16] bf16 or f32
@@ -294,7 +295,10 @@ __device__ __forceinline__ bool compact(const FusedParams &p, const AtomGeo &g, 
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 ok = r2 < p.rc2 && r2 > 1e-10;
                 e = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(k));
-                if (!ok && fslot) fslot[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (!ok && fslot) {
+                    const int r = p.rev[slot];       // -1 if the list is not symmetric (reported by its status)
+                    if (r >= 0) fslot[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
             }
             const uint64_t bal = __ballot(ok);
             const int idx = n + __popcll(bal & ((1ull << lane) - 1ull));
@@ -320,11 +324,32 @@ __device__ __forceinline__ void ri_of(const float4 e, const float *ti, float rmi
     Ri[3] = (s * e.z * ir - ti[TI_DAVG + 3]) * ti[TI_ISTD + 3];
 }
 
-// Sum over the 32 lanes of each half-wave.
+// Sum over the 32 lanes of each half-wave, without LDS: DPP sums within each
+// row of 16 lanes (quad_perm, row_half_mirror, row_mirror), row_bcast:15 adds
+// row 0 into row 1 and row 2 into row 3, and the two half totals (lanes 31
+// and 63) are read back as scalars.
 __device__ __forceinline__ float half_sum(float v) {
-#pragma unroll
-    for (int off = 1; off < 32; off <<= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += __builtin_amdgcn_update_dpp(0.0f, v, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_update_dpp(0.0f, v, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
+    v += __builtin_amdgcn_update_dpp(0.0f, v, 0x141, 0xF, 0xF, false);   // row_half_mirror
+    v += __builtin_amdgcn_update_dpp(0.0f, v, 0x140, 0xF, 0xF, false);   // row_mirror
+    v += __builtin_amdgcn_update_dpp(0.0f, v, 0x142, 0xA, 0xF, false);   // row_bcast:15 -> rows 1, 3
+    const float s0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+    const float s1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    return (threadIdx.x & 32) ? s1 : s0;
+}
+
+// Value of lane l ^ 32 (the same lane of the other half-wave), via v_permlane32_swap.
+__device__ __forceinline__ float other_half(float v) {
+    const unsigned u = __float_as_uint(v);
+    const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    return __uint_as_float((threadIdx.x & 32) ? r[0] : r[1]);
+}
+
+// Sum over all 64 lanes, without LDS.
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+    v = half_sum(v);
+    return v + other_half(v);
 }
 
 // The three tanh-layer gates r (1 - t^2) of a tile, kept from the forward
@@ -580,7 +605,7 @@ dp_fused_bwd_kernel(FusedParams p) {
                 if constexpr (BF16) dsum *= 1.0f / TANH_K;     // w1 is stored pre-scaled
                 dRi[0] += dsum;
 #pragma unroll
-                for (int w = 0; w < 4; w++) dRi[w] += __shfl_xor(dRi[w], 32, 64);
+                for (int w = 0; w < 4; w++) dRi[w] += other_half(dRi[w]);
                 // force of this slot: f_a = sum_w dE/dRi_w * Ri_d[w][a], Ri_d = -dfeat/dstd
                 if (live && h == 0) {
                     const float ir = __builtin_amdgcn_rcpf(r);
@@ -596,8 +621,10 @@ dp_fused_bwd_kernel(FusedParams p) {
                     for (int a = 0; a < 3; a++) f[a] = -(x[a] * cf + gw[a + 1] * s * ir);
                     const int64_t slot = (i * p.T + t) * (int64_t)p.mnl + __float_as_int(e.w);
                     if (p.fslot) {
-                        // gather mode: the neighbor term is summed later through the reverse slot map
-                        p.fslot[slot] = make_float4(f[0], f[1], f[2], 0.f);
+                        // gather mode: the force on j goes to the partner slot (j's entry naming i),
+                        // so that j's gather reads its own entries contiguously
+                        const int rs = p.rev[slot];
+                        if (rs >= 0) p.fslot[rs] = make_float4(f[0], f[1], f[2], 0.f);
                     } else {
                         const int64_t j = g.fbase + p.list[slot] - 1;
                         atomic_add(p.force + 3 * j, f[0]);
@@ -615,7 +642,7 @@ dp_fused_bwd_kernel(FusedParams p) {
         }
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            const float x = wave_sum(fc[a]);
+            const float x = wave_sum_dpp(fc[a]);
             if (lane == 0) {
                 if (p.fcenter) p.fcenter[3 * i + a] = x;
                 else atomic_add(p.force + 3 * i + a, x);
@@ -624,7 +651,7 @@ dp_fused_bwd_kernel(FusedParams p) {
         // the atom's virial (FP32 over its ~100 slots) joins the wave's FP64 frame sum
 #pragma unroll
         for (int v = 0; v < 9; v++) {
-            const float x = wave_sum(vf[v]);
+            const float x = wave_sum_dpp(vf[v]);
             if (lane == 0) vacc[v] += (double)x;
         }
         __builtin_amdgcn_wave_barrier();
@@ -651,23 +678,22 @@ dp_fused_bwd_kernel(FusedParams p) {
     }
 }
 
-// force[i] = fcenter[i] + sum over used slots s of i of fslot[rev[s]]; one
-// wavefront per atom, fixed reduction order (deterministic).
+// force[i] = fcenter[i] + sum over the used entries s of i's list of fslot[s]
+// (each written by the partner atom through the reverse slot map); one
+// wavefront per atom reading its own entries contiguously, fixed reduction
+// order (deterministic).
 __global__ void __launch_bounds__(256)
-gather_kernel(const int32_t *__restrict__ rev, const int32_t *__restrict__ counts, const float4 *__restrict__ fslot,
-              const float *__restrict__ fcenter, float *__restrict__ force, int T, int mnl, int64_t n) {
+gather_kernel(const int32_t *__restrict__ counts, const float4 *__restrict__ fslot, const float *__restrict__ fcenter,
+              float *__restrict__ force, int T, int mnl, int64_t n) {
     const int lane = threadIdx.x & (WAVE - 1);
     for (int64_t i = (int64_t)blockIdx.x * 4 + threadIdx.x / WAVE; i < n; i += (int64_t)gridDim.x * 4) {
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
         for (int t = 0; t < T; t++) {
             const int cnt = counts[i * T + t];
-            const int32_t *rv = rev + (i * T + t) * (int64_t)mnl;
+            const float4 *fr = fslot + (i * T + t) * (int64_t)mnl;
             for (int k = lane; k < cnt; k += WAVE) {
-                const int r = rv[k];
-                if (r >= 0) {
-                    const float4 f = fslot[r];
-                    a0 += f.x; a1 += f.y; a2 += f.z;
-                }
+                const float4 f = fr[k];
+                a0 += f.x; a1 += f.y; a2 += f.z;
             }
         }
         a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
@@ -761,18 +787,18 @@ MATPL_EXPORT int matpl_dp_fused_backward(const double *pos, const double *lattic
     if (rev) {
         MATPL_CHECK_ARG(fslot && fcenter && aligned16(fslot), "dp_fused_backward: gather mode needs fslot/fcenter");
         p.fslot = reinterpret_cast<float4 *>(fslot);
+        p.rev = rev;
         p.fcenter = fcenter;
     }
     return dispatch(p, E, bf16, false, (hipStream_t)stream);
 }
 
-MATPL_EXPORT int matpl_dp_fused_gather(const int32_t *rev, const int32_t *counts, const float *fslot,
-                                       const float *fcenter, float *force, int ntypes, int mnl, int64_t natoms,
-                                       void *stream) {
+MATPL_EXPORT int matpl_dp_fused_gather(const int32_t *counts, const float *fslot, const float *fcenter, float *force,
+                                       int ntypes, int mnl, int64_t natoms, void *stream) {
     if (natoms == 0) return OK;
-    MATPL_CHECK_ARG(rev && counts && fslot && fcenter && force && aligned16(fslot), "dp_fused_gather: bad pointer");
+    MATPL_CHECK_ARG(counts && fslot && fcenter && force && aligned16(fslot), "dp_fused_gather: bad pointer");
     MATPL_CHECK_ARG(ntypes >= 1 && mnl >= 1, "dp_fused_gather: bad sizes");
-    hipLaunchKernelGGL(gather_kernel, dim3(grid_cap(natoms, 4)), dim3(256), 0, (hipStream_t)stream, rev, counts,
+    hipLaunchKernelGGL(gather_kernel, dim3(grid_cap(natoms, 4)), dim3(256), 0, (hipStream_t)stream, counts,
                        reinterpret_cast<const float4 *>(fslot), fcenter, force, ntypes, mnl, natoms);
     return check_launch("gather_kernel");
 }

@@ -223,6 +223,6 @@ class FusedDP:
             e1.record()
             self.events.append((e0, e1))
         if rev is not None:
-            _lib.call("matpl_dp_fused_gather", _ptr(rev), _ptr(nl_counts), _ptr(fslot), _ptr(fcenter), _ptr(F), T, mnl,
-                      N, _stream())
+            _lib.call("matpl_dp_fused_gather", _ptr(nl_counts), _ptr(fslot), _ptr(fcenter), _ptr(F), T, mnl, N,
+                      _stream())
         return Ei, F, V, status
