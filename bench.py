@@ -230,7 +230,7 @@ def main():
         ref = DPMDFP64(frame, w, vel, dt=1.0, device=dev)
         chk = DPMD(frame, w, vel, precision=args.precision, skin=0.5, mnl=160, device=dev)
         F64 = ref.F64
-        force_rmse = float(torch.sqrt(((chk.forces().double() - F64) ** 2).mean()).item())
+        force_rmse = float(torch.sqrt(((chk.F.double() - F64) ** 2).mean()).item())
         e_rel = abs(chk.energy() - float(ref.E.sum().item())) / abs(float(ref.E.sum().item()))
         log(f"force RMSE vs FP64 = {force_rmse:.3e} eV/A, |dE|/E = {e_rel:.2e}")
         del chk

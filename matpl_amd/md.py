@@ -31,31 +31,9 @@ def _affine(x: torch.Tensor, M: np.ndarray, out=None) -> torch.Tensor:
     return out
 
 
-def spatial_order(frame: Frame, bin_width: float = 4.0) -> np.ndarray:
-    """Permutation grouping atoms by type, then by the Morton (Z-order) index of a grid of
-    ~bin_width cells, so that atoms close in space are close in memory (the MD-engine practice
-    of periodic atom sorting): neighbor lists, positions and per-entry force buffers of nearby
-    atoms then share cache lines."""
-    f = frame.frac() % 1.0
-    lengths = np.linalg.norm(np.asarray(frame.lattice, np.float64), axis=1)
-    nb = np.maximum(1, np.floor(lengths / bin_width)).astype(np.int64)
-    cell = np.minimum((f * nb).astype(np.int64), nb - 1)
-    code = np.zeros(len(f), np.int64)
-    for bit in range(21):
-        for d in range(3):
-            code |= ((cell[:, d] >> bit) & 1) << (3 * bit + d)
-    return np.lexsort((code, frame.types))
-
-
 class DPMD:
     def __init__(self, frame: Frame, w: DPWeights, velocities: np.ndarray, precision="bf16", skin=0.5,
-                 mnl=None, dt=1.0, rebuild_every=10, device="cuda", sort=True):
-        """sort: keep atoms in spatial order internally (``order`` maps internal -> input index;
-        ``forces()`` / ``positions()`` return input order)."""
-        self.order = spatial_order(frame) if sort else np.arange(frame.natoms)
-        frame = Frame(frame.lattice, frame.pos[self.order], frame.types[self.order], frame.elements)
-        velocities = np.asarray(velocities)[self.order]
-        self.inv_order = torch.as_tensor(np.argsort(self.order), device=device)
+                 mnl=None, dt=1.0, rebuild_every=10, device="cuda"):
         self.dev = torch.device(device)
         self.cfg = w.cfg
         self.N = frame.natoms
@@ -153,13 +131,6 @@ class DPMD:
 
     def energy(self) -> float:
         return float(self.Ei.double().sum().item())
-
-    def forces(self) -> torch.Tensor:
-        """Forces [N, 3] (float32) in the input atom order."""
-        return self.F[self.inv_order]
-
-    def positions(self) -> torch.Tensor:
-        return self.pos[self.inv_order]
 
 
 class DPMDFP64:
