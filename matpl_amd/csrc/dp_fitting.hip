@@ -160,15 +160,44 @@ dp_fitting_kernel(FitParams p) {
             for (int w = 0; w < 4; w++)
 #pragma unroll
                 for (int q = 0; q < 8; q++) aq[w][q] = AS(n, w, 8 * h + q);
-            for (int s = 0; s < E; s++) {
-                float drv[8];
-                const float ap0 = AS(n, 0, s), ap1 = AS(n, 1, s), ap2 = AS(n, 2, s), ap3 = AS(n, 3, s);
+            // K-steps go in pairs (s, s + 1) with two register sets of weight chunks; each set is
+            // reloaded right after its MFMA has read it, one pair ahead of its next use, so the
+            // global-memory latency overlaps a pair of steps (no moves of in-flight registers).
+            // A step beyond E (odd E) multiplies zeros.
+            const bf16x8 *wr0 = reinterpret_cast<const bf16x8 *>(reinterpret_cast<const char *>(w0f) +
+                                                                 (size_t)lane * p.Dh * 2);
+            const bf16x8 *wr1 = reinterpret_cast<const bf16x8 *>(reinterpret_cast<const char *>(w0f) +
+                                                                 ((size_t)64 + lane) * p.Dh * 2);
+            const int Em = E - 1;
+            bf16x8 a0 = wr0[0], a1 = wr1[0];
+            bf16x8 b0 = wr0[1 < Em ? 1 : Em], b1 = wr1[1 < Em ? 1 : Em];
+            for (int s = 0; s < E; s += 2) {
 #pragma unroll
-                for (int q = 0; q < 8; q++) drv[q] = ap0 * aq[0][q] + ap1 * aq[1][q] + ap2 * aq[2][q] + ap3 * aq[3][q];
+                for (int half = 0; half < 2; half++) {
+                    const int st = s + half;
+                    const int sc = st < Em ? st : Em;
+                    const float live = st < E ? 1.0f : 0.0f;
+                    const float ap0 = AS(n, 0, sc) * live, ap1 = AS(n, 1, sc) * live, ap2 = AS(n, 2, sc) * live,
+                                ap3 = AS(n, 3, sc) * live;
+                    bf16x8 b;
 #pragma unroll
-                for (int ob = 0; ob < 2; ob++)
-                    mma<true>(acc[ob], reinterpret_cast<const char *>(w0f) + ((size_t)ob * 64 + lane) * p.Dh * 2, s,
-                              drv);
+                    for (int q = 0; q < 8; q++)
+                        b[q] = (__bf16)(ap0 * aq[0][q] + ap1 * aq[1][q] + ap2 * aq[2][q] + ap3 * aq[3][q]);
+                    const int sn = st + 2 < Em ? st + 2 : Em;
+                    if (half == 0) {
+                        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0], 0, 0, 0);
+                        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1], 0, 0, 0);
+                        a0 = wr0[sn];
+                        a1 = wr1[sn];
+                        __builtin_amdgcn_sched_barrier(0);   // keep the prefetch here
+                    } else {
+                        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b0, b, acc[0], 0, 0, 0);
+                        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b1, b, acc[1], 0, 0, 0);
+                        b0 = wr0[sn];
+                        b1 = wr1[sn];
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                }
             }
         } else {
             // K-step s covers DR index k = 2 s + h
@@ -259,16 +288,40 @@ dp_fitting_kernel(FitParams p) {
         for (int w = 0; w < 4; w++)
 #pragma unroll
             for (int r = 0; r < 8; r++) aq2[w][r] = AS(n, w, feat(r, h));
+        // BF16: the 4 weight chunks of the next block are loaded while this block computes
+        bf16x8 wnext[4];
+        if constexpr (BF16) {
+            const bf16x8 *wb0 = reinterpret_cast<const bf16x8 *>(reinterpret_cast<const char *>(w0b) +
+                                                                 (size_t)lane * 32 * 2);
+#pragma unroll
+            for (int c = 0; c < 4; c++) wnext[c] = wb0[c];
+        }
         for (int blk = 0; blk < p.nb0; blk++) {
             floatx16 g;
 #pragma unroll
             for (int r = 0; r < 16; r++) g[r] = 0.0f;
-            const char *wb = reinterpret_cast<const char *>(w0b) + ((size_t)blk * 64 + lane) * 32 * (BF16 ? 2 : 4);
-            constexpr int CH = BF16 ? 4 : 32, PER = BF16 ? 8 : 1;
+            if constexpr (BF16) {
+                bf16x8 wcur[4];
 #pragma unroll
-            for (int c = 0; c < CH; c++) {
-                const int v = c * PER;
-                mma<BF16>(g, wb, c, &dz0[v / 16][v % 16]);
+                for (int c = 0; c < 4; c++) wcur[c] = wnext[c];
+                {
+                    const int bn = blk + 1 < p.nb0 ? blk + 1 : blk;
+                    const bf16x8 *wbn = reinterpret_cast<const bf16x8 *>(reinterpret_cast<const char *>(w0b) +
+                                                                         ((size_t)bn * 64 + lane) * 32 * 2);
+#pragma unroll
+                    for (int c = 0; c < 4; c++) wnext[c] = wbn[c];
+                }
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    bf16x8 b;
+#pragma unroll
+                    for (int q = 0; q < 8; q++) b[q] = (__bf16)dz0[(c * 8) / 16][(c * 8) % 16 + q];
+                    g = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wcur[c], b, g, 0, 0, 0);
+                }
+            } else {
+                const char *wb = reinterpret_cast<const char *>(w0b) + ((size_t)blk * 64 + lane) * 32 * 4;
+#pragma unroll
+                for (int c = 0; c < 32; c++) mma<false>(g, wb, c, &dz0[c / 16][c % 16]);
             }
             const int p0 = 2 * blk, p1 = 2 * blk + 1;
             float s0[4] = {0.f, 0.f, 0.f, 0.f}, s1[4] = {0.f, 0.f, 0.f, 0.f};
