@@ -31,6 +31,10 @@ METRIC = "atom-steps/s (energy+force+virial) vs plain FP64 path, at force RMSE â
 REPS = (19, 19, 23)
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+# HBM bytes per launch of dp_fused_bwd_kernel on the 19x19x23 workload: rocprofv3 FETCH_SIZE + WRITE_SIZE,
+# each scaled by a copy kernel of known size (profiles/dp_md_hfo2_99636_hbm_traffic.md; re-measure after
+# kernel changes). Counters cannot be read from inside the timed run.
+BWD_TRAFFIC_BYTES = 587.1e6
 
 
 def log(*a):
@@ -219,7 +223,9 @@ def main():
         flops = flops_per_neighbor_bwd(w.cfg.emb_out) * nvalid
         achieved = flops / kt / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / BF16_PEAK_TFLOPS, 5), "traffic": None,
+                    "frac": round(achieved / BF16_PEAK_TFLOPS, 5),
+                    "traffic": BWD_TRAFFIC_BYTES if reps == REPS and args.precision == "bf16" else None,
+                    "traffic_GBps": round(BWD_TRAFFIC_BYTES / kt / 1e9, 1) if reps == REPS else None,
                     "kernel": "dp_fused_bwd_kernel", "avg_ms": round(kt * 1e3, 4),
                     "in_cutoff_neighbors": nvalid}
 
