@@ -136,6 +136,49 @@ def nep_leg(steps=20):
     return out
 
 
+def nep_train_leg(steps=3):
+    """NEP training gradient on 128 frames of a 4-element Ag/Au/Si/C alloy (N_f uniform in [32, 1024],
+    0.06 atoms/A^3, 62,713 atoms in all), descriptor as the SiC leg with 4 types: one step = neighbor
+    search + features + fitting nets + forces/virials (create_graph) + the backward of an
+    energy + force + virial loss to the fitting-net and c2/c3 coefficient gradients, in float32 and
+    float64."""
+    import torch
+    from matpl_amd import nep as mn
+    from matpl_amd import structures as st
+    from matpl_amd.nepnet import NEPConfig, NEPWeights
+    dev = torch.device("cuda:0")
+    frames = st.agausic_frames(nframes=128, nmin=32, nmax=1024, density=0.06, dmin=1.9, seed=1004)
+    natoms = sum(f.natoms for f in frames)
+    cfg = NEPConfig(ntypes=4, l_max=[4, 2, 0])
+    w = NEPWeights.random(cfg, seed=2024)
+    gp = mn.pack_frames(frames, cfg.rcut_radial, dev)
+    out = {"workload": f"NEP training gradient, 128 Ag-Au-Si-C frames ({natoms} atoms), rc 6/4 A, n_max 4/4, "
+                       "basis 12, l_max [4,2,0], 30 hidden, E+F+V loss backward to all parameters incl. c2/c3",
+           "natoms": natoms}
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        model = mn.NEPTorch(w, dt, dev)
+        params = [q for q in model.parameters() if q.requires_grad]
+
+        def step():
+            nb = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes)
+            E, Ei, F, V = model(nb["NN_r"], nb["NL_r"], nb["Ri_r"], nb["NN_a"], nb["NL_a"], nb["Ri_a"],
+                                gp["num_atoms"], gp["atom_type_map"], create_graph=True)
+            loss = (E ** 2).sum() + (F ** 2).sum() + (V ** 2).sum()
+            return torch.autograd.grad(loss, params)
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        dt_s = time.perf_counter() - t0
+        out[name] = {"value": round(natoms * steps / dt_s, 1), "unit": "atom-steps/s",
+                     "ms_per_step": round(dt_s * 1e3 / steps, 2)}
+    out["speedup_fp32_vs_fp64"] = round(out["fp32"]["value"] / out["fp64"]["value"], 2)
+    log(f"NEP train: fp32 {out['fp32']['ms_per_step']} ms/step, fp64 {out['fp64']['ms_per_step']} ms/step")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,7 +190,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", default=None, help="override cells, e.g. 8,8,8")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a captured HIP graph")
-    ap.add_argument("--no-nep", action="store_true", help="skip the NEP (SiC 1,728 atoms) leg")
+    ap.add_argument("--no-nep", action="store_true", help="skip the NEP legs (SiC 1,728 atoms; 128-frame training)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MiB",
                     help="after the run, copy a buffer of this size 3 times (a known-traffic kernel for "
                          "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -263,6 +306,7 @@ def main():
         del a, b
 
     nep = None if args.no_nep else nep_leg()
+    nep_train = None if args.no_nep else nep_train_leg()
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -279,6 +323,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "nep": nep,
+        "nep_train": nep_train,
     }
     print(json.dumps(out), flush=True)
 
