@@ -134,10 +134,12 @@ dp_fitting_kernel(FitParams p) {
     for (int tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
         const int t = p.tiles[3 * tile], a0 = p.tiles[3 * tile + 1], cnt = p.tiles[3 * tile + 2];
         // stage A and clear dA
-        for (int x = lane; x < 32 * 4 * E; x += WAVE) {
-            const int an = x / (4 * E), wc = x - an * 4 * E;
-            As[an * RS + wc] = an < cnt ? p.A[(int64_t)(a0 + an) * 4 * E + wc] : 0.0f;
+        // (atom, offset) advance by 64 elements per step without a division (4E >= 64)
+        for (int x = lane, an = 0, wc = lane; x < 32 * 4 * E; x += WAVE) {
+            As[an * RS + wc] = an < cnt ? p.A[(int64_t)a0 * 4 * E + x] : 0.0f;
             dAs[an * RS + wc] = 0.0f;
+            wc += WAVE;
+            if (wc >= 4 * E) { wc -= 4 * E; an++; }
         }
         __syncthreads();
         const void *w0f = reinterpret_cast<const char *>(p.w0f) + (size_t)t * p.type_stride_w0f * (BF16 ? 2 : 4);
@@ -288,6 +290,11 @@ dp_fitting_kernel(FitParams p) {
         for (int w = 0; w < 4; w++)
 #pragma unroll
             for (int r = 0; r < 8; r++) aq2[w][r] = AS(n, w, feat(r, h));
+        float d2[4][8];             // dA[w][q] of the q-side (q = f(r,h)), over all blocks
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+#pragma unroll
+            for (int r = 0; r < 8; r++) d2[w][r] = 0.0f;
         // BF16: the 4 weight chunks of the next block are loaded while this block computes
         bf16x8 wnext[4];
         if constexpr (BF16) {
@@ -331,13 +338,11 @@ dp_fitting_kernel(FitParams p) {
                                   p1 < E ? AS(n, 2, p1) : 0.f, p1 < E ? AS(n, 3, p1) : 0.f};
 #pragma unroll
             for (int r = 0; r < 8; r++) {
-                // dA[w][q] of the q-side (q = f(r,h)) goes straight to this lane's LDS row
-                const int q = feat(r, h);
 #pragma unroll
                 for (int w = 0; w < 4; w++) {
                     s0[w] += g[r] * aq2[w][r];
                     s1[w] += g[r + 8] * aq2[w][r];
-                    DAS(n, w, q) += g[r] * ap0[w] + g[r + 8] * ap1[w];
+                    d2[w][r] += g[r] * ap0[w] + g[r + 8] * ap1[w];
                 }
             }
 #pragma unroll
@@ -355,10 +360,17 @@ dp_fitting_kernel(FitParams p) {
             }
         }
         __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int q = feat(r, h);
+#pragma unroll
+            for (int w = 0; w < 4; w++) DAS(n, w, q) += d2[w][r];
+        }
         __syncthreads();
-        for (int x2 = lane; x2 < 32 * 4 * E; x2 += WAVE) {
-            const int an = x2 / (4 * E), wc = x2 - an * 4 * E;
-            if (an < cnt) p.dA[(int64_t)(a0 + an) * 4 * E + wc] = dAs[an * RS + wc];
+        for (int x2 = lane, an = 0, wc = lane; x2 < 32 * 4 * E; x2 += WAVE) {
+            if (an < cnt) p.dA[(int64_t)a0 * 4 * E + x2] = dAs[an * RS + wc];
+            wc += WAVE;
+            if (wc >= 4 * E) { wc -= 4 * E; an++; }
         }
         __syncthreads();
     }
