@@ -83,6 +83,13 @@ struct Gates16<false> {
     __device__ __forceinline__ float get(int r) const { return v[r]; }
 };
 
+// Value of lane l ^ 32 (same lane of the other half-wave), via v_permlane32_swap.
+__device__ __forceinline__ float other_half(float v) {
+    const unsigned u = __float_as_uint(v);
+    const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    return __uint_as_float((threadIdx.x & 32) ? r[0] : r[1]);
+}
+
 template <bool BF16>
 __device__ __forceinline__ void mma(floatx16 &acc, const void *w, int idx, const float *x) {
     // idx: chunk index; BF16: 8 values per chunk, FP32: 1 value per chunk
@@ -173,6 +180,12 @@ dp_fitting_kernel(FitParams p) {
             const int Em = E - 1;
             bf16x8 a0 = wr0[0], a1 = wr1[0];
             bf16x8 b0 = wr0[1 < Em ? 1 : Em], b1 = wr1[1 < Em ? 1 : Em];
+            // odd K-steps accumulate separately (two independent MFMA chains per output block)
+            floatx16 accb[2];
+#pragma unroll
+            for (int ob = 0; ob < 2; ob++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) accb[ob][r] = 0.0f;
             for (int s = 0; s < E; s += 2) {
 #pragma unroll
                 for (int half = 0; half < 2; half++) {
@@ -193,14 +206,18 @@ dp_fitting_kernel(FitParams p) {
                         a1 = wr1[sn];
                         __builtin_amdgcn_sched_barrier(0);   // keep the prefetch here
                     } else {
-                        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b0, b, acc[0], 0, 0, 0);
-                        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b1, b, acc[1], 0, 0, 0);
+                        accb[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b0, b, accb[0], 0, 0, 0);
+                        accb[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b1, b, accb[1], 0, 0, 0);
                         b0 = wr0[sn];
                         b1 = wr1[sn];
                         __builtin_amdgcn_sched_barrier(0);
                     }
                 }
             }
+#pragma unroll
+            for (int ob = 0; ob < 2; ob++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[ob][r] += accb[ob][r];
         } else {
             // K-step s covers DR index k = 2 s + h
             for (int s = 0; s < p.D / 2; s++) {
@@ -295,41 +312,9 @@ dp_fitting_kernel(FitParams p) {
         for (int w = 0; w < 4; w++)
 #pragma unroll
             for (int r = 0; r < 8; r++) d2[w][r] = 0.0f;
-        // BF16: the 4 weight chunks of the next block are loaded while this block computes
-        bf16x8 wnext[4];
-        if constexpr (BF16) {
-            const bf16x8 *wb0 = reinterpret_cast<const bf16x8 *>(reinterpret_cast<const char *>(w0b) +
-                                                                 (size_t)lane * 32 * 2);
-#pragma unroll
-            for (int c = 0; c < 4; c++) wnext[c] = wb0[c];
-        }
-        for (int blk = 0; blk < p.nb0; blk++) {
-            floatx16 g;
-#pragma unroll
-            for (int r = 0; r < 16; r++) g[r] = 0.0f;
-            if constexpr (BF16) {
-                bf16x8 wcur[4];
-#pragma unroll
-                for (int c = 0; c < 4; c++) wcur[c] = wnext[c];
-                {
-                    const int bn = blk + 1 < p.nb0 ? blk + 1 : blk;
-                    const bf16x8 *wbn = reinterpret_cast<const bf16x8 *>(reinterpret_cast<const char *>(w0b) +
-                                                                         ((size_t)bn * 64 + lane) * 32 * 2);
-#pragma unroll
-                    for (int c = 0; c < 4; c++) wnext[c] = wbn[c];
-                }
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    bf16x8 b;
-#pragma unroll
-                    for (int q = 0; q < 8; q++) b[q] = (__bf16)dz0[(c * 8) / 16][(c * 8) % 16 + q];
-                    g = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wcur[c], b, g, 0, 0, 0);
-                }
-            } else {
-                const char *wb = reinterpret_cast<const char *>(w0b) + ((size_t)blk * 64 + lane) * 32 * 4;
-#pragma unroll
-                for (int c = 0; c < 32; c++) mma<false>(g, wb, c, &dz0[c / 16][c % 16]);
-            }
+        // VALU part of one DR block: its rows p0 = 2 blk, p1 = 2 blk + 1 (dA of the p side, summed over
+        // both halves of the wave) and the q-side terms (kept in d2)
+        auto block_dA = [&](int blk, const floatx16 &g) {
             const int p0 = 2 * blk, p1 = 2 * blk + 1;
             float s0[4] = {0.f, 0.f, 0.f, 0.f}, s1[4] = {0.f, 0.f, 0.f, 0.f};
             const float ap0[4] = {p0 < E ? AS(n, 0, p0) : 0.f, p0 < E ? AS(n, 1, p0) : 0.f,
@@ -347,8 +332,8 @@ dp_fitting_kernel(FitParams p) {
             }
 #pragma unroll
             for (int w = 0; w < 4; w++) {
-                s0[w] += __shfl_xor(s0[w], 32, 64);
-                s1[w] += __shfl_xor(s1[w], 32, 64);
+                s0[w] += other_half(s0[w]);
+                s1[w] += other_half(s1[w]);
             }
             if (h == 0 && p0 < E) {
 #pragma unroll
@@ -357,6 +342,57 @@ dp_fitting_kernel(FitParams p) {
             if (h == 1 && p1 < E) {
 #pragma unroll
                 for (int w = 0; w < 4; w++) DAS(n, w, p1) += s1[w];
+            }
+        };
+        if constexpr (BF16) {
+            // dz0 as bf16 B operands once; blocks in pairs (two independent MFMA chains), the weight
+            // chunks of the next pair loaded while this pair computes
+            bf16x8 dzb[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+#pragma unroll
+                for (int q = 0; q < 8; q++) dzb[c][q] = (__bf16)dz0[(c * 8) / 16][(c * 8) % 16 + q];
+            const int nbm = p.nb0 - 1;
+            auto wrow = [&](int b) {
+                return reinterpret_cast<const bf16x8 *>(reinterpret_cast<const char *>(w0b) +
+                                                        ((size_t)b * 64 + lane) * 32 * 2);
+            };
+            bf16x8 nA[4], nB[4];
+            {
+                const bf16x8 *ra = wrow(0), *rb = wrow(1 < nbm ? 1 : nbm);
+#pragma unroll
+                for (int c = 0; c < 4; c++) { nA[c] = ra[c]; nB[c] = rb[c]; }
+            }
+            for (int blk = 0; blk < p.nb0; blk += 2) {
+                bf16x8 wA[4], wB[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) { wA[c] = nA[c]; wB[c] = nB[c]; }
+                {
+                    const int ba = blk + 2 < nbm ? blk + 2 : nbm, bb = blk + 3 < nbm ? blk + 3 : nbm;
+                    const bf16x8 *ra = wrow(ba), *rb = wrow(bb);
+#pragma unroll
+                    for (int c = 0; c < 4; c++) { nA[c] = ra[c]; nB[c] = rb[c]; }
+                }
+                floatx16 gA, gB;
+#pragma unroll
+                for (int r = 0; r < 16; r++) { gA[r] = 0.0f; gB[r] = 0.0f; }
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    gA = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wA[c], dzb[c], gA, 0, 0, 0);
+                    gB = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wB[c], dzb[c], gB, 0, 0, 0);
+                }
+                block_dA(blk, gA);
+                if (blk + 1 < p.nb0) block_dA(blk + 1, gB);
+            }
+        } else {
+            for (int blk = 0; blk < p.nb0; blk++) {
+                floatx16 g;
+#pragma unroll
+                for (int r = 0; r < 16; r++) g[r] = 0.0f;
+                const char *wb = reinterpret_cast<const char *>(w0b) + ((size_t)blk * 64 + lane) * 32 * 4;
+#pragma unroll
+                for (int c = 0; c < 32; c++) mma<false>(g, wb, c, &dz0[c / 16][c % 16]);
+                block_dA(blk, g);
             }
         }
         __syncthreads();
