@@ -16,10 +16,11 @@
 // reference's convention for the [N, S, 4] gradient of the descriptor ops.
 // Neighbor loops stop at the first -1 entry as the reference's do.
 //
-// Layout: one wavefront per atom, lanes over neighbor slots; per-atom sums
-// over slots (s_nlm, their tangents) go through LDS atomics; per-type
-// coefficient gradients are accumulated per block in LDS and added to global
-// memory once per block (persistent grid).
+// Layout: one wavefront per atom, lanes over neighbor slots for the per-pair
+// work; sums over slots (s_nlm, their tangents, coefficient gradients) run
+// transposed with lanes over outputs, so lanes never add to one address; the
+// per-type coefficient gradients are accumulated per block in LDS and added
+// to global memory once per block (persistent grid).
 #include "common.h"
 
 #pragma clang fp contract(off)
@@ -34,20 +35,28 @@ constexpr int NM_MAX = 8;       // radial functions per descriptor part
 constexpr int NLM = 24;
 constexpr double PI_D = 3.14159265358979323846;
 
+// f_k(r) and f_k'(r) for k = 0, 1, 2, ... in order, one per next() call
+// (Chebyshev recursion, nothing stored per k).
 template <typename T>
-__device__ __forceinline__ void fk_fkp(T r, T rc, int nb, T *f, T *fp) {
-    const T rcinv = T(1) / rc;
-    T fc = T(0), fcp = T(0);
-    if (r < rc) {
-        const T a = T(PI_D) * r * rcinv;
-        fc = T(0.5) * cos(a) + T(0.5);
-        fcp = -T(0.5 * PI_D) * sin(a) * rcinv;
+struct Basis {
+    T x, xp, fc, fcp, t0, t1, tp0, tp1;
+    int k;
+    __device__ __forceinline__ Basis(T r, T rc) {
+        const T rcinv = T(1) / rc;
+        fc = T(0);
+        fcp = T(0);
+        if (r < rc) {
+            const T a = T(PI_D) * r * rcinv;
+            fc = T(0.5) * cos(a) + T(0.5);
+            fcp = -T(0.5 * PI_D) * sin(a) * rcinv;
+        }
+        const T d = r * rcinv - T(1);
+        x = T(2) * d * d - T(1);
+        xp = T(4) * d * rcinv;
+        t0 = T(1); t1 = x; tp0 = T(0); tp1 = T(1);
+        k = 0;
     }
-    const T d = r * rcinv - T(1);
-    const T x = T(2) * d * d - T(1);
-    const T xp = T(4) * d * rcinv;
-    T t0 = T(1), t1 = x, tp0 = T(0), tp1 = T(1);
-    for (int k = 0; k < nb; k++) {
+    __device__ __forceinline__ void next(T &f, T &fp) {
         T t, tp;
         if (k == 0) { t = t0; tp = tp0; }
         else if (k == 1) { t = t1; tp = tp1; }
@@ -56,55 +65,45 @@ __device__ __forceinline__ void fk_fkp(T r, T rc, int nb, T *f, T *fp) {
             tp = T(2) * t1 + T(2) * x * tp1 - tp0;
             t0 = t1; t1 = t; tp0 = tp1; tp1 = tp;
         }
-        f[k] = (t + T(1)) * T(0.5) * fc;
-        fp[k] = T(0.5) * (tp * xp * fc + (t + T(1)) * fcp);
+        k++;
+        f = (t + T(1)) * T(0.5) * fc;
+        fp = T(0.5) * (tp * xp * fc + (t + T(1)) * fcp);
     }
-}
+};
 
-// 24 real harmonics of u = (x,y,z)/r (accumulate_s order) and their partial
-// derivatives with respect to (r, x, y, z) taken independently.
-template <typename T>
-__device__ __forceinline__ void blm_and_grad(T r, T X, T Y, T Z, T b[NLM], T db[NLM][4]) {
-    const T ir = T(1) / r;
-    const T x = X * ir, y = Y * ir, z = Z * ir;
+// The 24 real harmonics b_k(u) of a unit vector u (accumulate_s order): calls
+// fn(k, b_k, db_k/dux, db_k/duy, db_k/duz) for k = 0..23. With r and (x, y, z)
+// as independent inputs and u = (x, y, z)/r:
+//   db_k/dx_w = (db_k/du_w) / r,   db_k/dr = -(u . db_k/du) / r.
+template <typename T, typename Fn>
+__device__ __forceinline__ void blm_each(T x, T y, T z, Fn &&fn) {
     const T x2 = x * x, y2 = y * y, z2 = z * z, xy2 = x2 - y2;
-    // value and d/du_(x,y,z)
-    T du[NLM][3];
-#define SET(k, v, a, bb, c) { b[k] = (v); du[k][0] = (a); du[k][1] = (bb); du[k][2] = (c); }
-    SET(0, z, T(0), T(0), T(1));
-    SET(1, x, T(1), T(0), T(0));
-    SET(2, y, T(0), T(1), T(0));
-    SET(3, T(3) * z2 - T(1), T(0), T(0), T(6) * z);
-    SET(4, x * z, z, T(0), x);
-    SET(5, y * z, T(0), z, y);
-    SET(6, xy2, T(2) * x, -T(2) * y, T(0));
-    SET(7, T(2) * x * y, T(2) * y, T(2) * x, T(0));
-    SET(8, (T(5) * z2 - T(3)) * z, T(0), T(0), T(15) * z2 - T(3));
-    SET(9, (T(5) * z2 - T(1)) * x, T(5) * z2 - T(1), T(0), T(10) * x * z);
-    SET(10, (T(5) * z2 - T(1)) * y, T(0), T(5) * z2 - T(1), T(10) * y * z);
-    SET(11, xy2 * z, T(2) * x * z, -T(2) * y * z, xy2);
-    SET(12, T(2) * x * y * z, T(2) * y * z, T(2) * x * z, T(2) * x * y);
-    SET(13, (x2 - T(3) * y2) * x, T(3) * x2 - T(3) * y2, -T(6) * x * y, T(0));
-    SET(14, (T(3) * x2 - y2) * y, T(6) * x * y, T(3) * x2 - T(3) * y2, T(0));
-    SET(15, (T(35) * z2 - T(30)) * z2 + T(3), T(0), T(0), T(140) * z2 * z - T(60) * z);
-    SET(16, (T(7) * z2 - T(3)) * x * z, (T(7) * z2 - T(3)) * z, T(0), T(21) * x * z2 - T(3) * x);
-    SET(17, (T(7) * z2 - T(3)) * y * z, T(0), (T(7) * z2 - T(3)) * z, T(21) * y * z2 - T(3) * y);
-    SET(18, (T(7) * z2 - T(1)) * xy2, T(2) * x * (T(7) * z2 - T(1)), -T(2) * y * (T(7) * z2 - T(1)),
-        T(14) * z * xy2);
-    SET(19, (T(7) * z2 - T(1)) * x * y * T(2), T(2) * y * (T(7) * z2 - T(1)), T(2) * x * (T(7) * z2 - T(1)),
-        T(28) * x * y * z);
-    SET(20, (x2 - T(3) * y2) * x * z, (T(3) * x2 - T(3) * y2) * z, -T(6) * x * y * z, (x2 - T(3) * y2) * x);
-    SET(21, (T(3) * x2 - y2) * y * z, T(6) * x * y * z, (T(3) * x2 - T(3) * y2) * z, (T(3) * x2 - y2) * y);
-    SET(22, xy2 * xy2 - T(4) * x2 * y2, T(4) * x2 * x - T(12) * x * y2, T(4) * y2 * y - T(12) * x2 * y, T(0));
-    SET(23, T(4) * x * y * xy2, T(12) * x2 * y - T(4) * y2 * y, T(4) * x2 * x - T(12) * x * y2, T(0));
-#undef SET
-#pragma unroll
-    for (int k = 0; k < NLM; k++) {
-        db[k][1] = du[k][0] * ir;
-        db[k][2] = du[k][1] * ir;
-        db[k][3] = du[k][2] * ir;
-        db[k][0] = -(x * du[k][0] + y * du[k][1] + z * du[k][2]) * ir;
-    }
+    fn(0, z, T(0), T(0), T(1));
+    fn(1, x, T(1), T(0), T(0));
+    fn(2, y, T(0), T(1), T(0));
+    fn(3, T(3) * z2 - T(1), T(0), T(0), T(6) * z);
+    fn(4, x * z, z, T(0), x);
+    fn(5, y * z, T(0), z, y);
+    fn(6, xy2, T(2) * x, -T(2) * y, T(0));
+    fn(7, T(2) * x * y, T(2) * y, T(2) * x, T(0));
+    fn(8, (T(5) * z2 - T(3)) * z, T(0), T(0), T(15) * z2 - T(3));
+    fn(9, (T(5) * z2 - T(1)) * x, T(5) * z2 - T(1), T(0), T(10) * x * z);
+    fn(10, (T(5) * z2 - T(1)) * y, T(0), T(5) * z2 - T(1), T(10) * y * z);
+    fn(11, xy2 * z, T(2) * x * z, -T(2) * y * z, xy2);
+    fn(12, T(2) * x * y * z, T(2) * y * z, T(2) * x * z, T(2) * x * y);
+    fn(13, (x2 - T(3) * y2) * x, T(3) * x2 - T(3) * y2, -T(6) * x * y, T(0));
+    fn(14, (T(3) * x2 - y2) * y, T(6) * x * y, T(3) * x2 - T(3) * y2, T(0));
+    fn(15, (T(35) * z2 - T(30)) * z2 + T(3), T(0), T(0), T(140) * z2 * z - T(60) * z);
+    fn(16, (T(7) * z2 - T(3)) * x * z, (T(7) * z2 - T(3)) * z, T(0), T(21) * x * z2 - T(3) * x);
+    fn(17, (T(7) * z2 - T(3)) * y * z, T(0), (T(7) * z2 - T(3)) * z, T(21) * y * z2 - T(3) * y);
+    fn(18, (T(7) * z2 - T(1)) * xy2, T(2) * x * (T(7) * z2 - T(1)), -T(2) * y * (T(7) * z2 - T(1)),
+       T(14) * z * xy2);
+    fn(19, (T(7) * z2 - T(1)) * x * y * T(2), T(2) * y * (T(7) * z2 - T(1)), T(2) * x * (T(7) * z2 - T(1)),
+       T(28) * x * y * z);
+    fn(20, (x2 - T(3) * y2) * x * z, (T(3) * x2 - T(3) * y2) * z, -T(6) * x * y * z, (x2 - T(3) * y2) * x);
+    fn(21, (T(3) * x2 - y2) * y * z, T(6) * x * y * z, (T(3) * x2 - T(3) * y2) * z, (T(3) * x2 - y2) * y);
+    fn(22, xy2 * xy2 - T(4) * x2 * y2, T(4) * x2 * x - T(12) * x * y2, T(4) * y2 * y - T(12) * x2 * y, T(0));
+    fn(23, T(4) * x * y * xy2, T(12) * x2 * y - T(4) * y2 * y, T(4) * x2 * x - T(12) * x * y2, T(0));
 }
 
 // ------------------------------------------------------------ q(s) and its gradient
@@ -118,7 +117,6 @@ __constant__ double kC3B[NLM] = {
 __constant__ double kC4B[5] = {-0.007499480826664, -0.134990654879954, 0.067495327439977, 0.404971964639861,
                                -0.809943929279723};
 __constant__ double kC5B[3] = {0.026596810706114, 0.053193621412227, 0.026596810706114};
-__constant__ int kLstart[5] = {0, 3, 8, 15, 24};
 
 // Forward-mode dual number (value, tangent) for Hessian-vector products.
 template <typename T>
@@ -134,35 +132,42 @@ template <typename T> __device__ __forceinline__ Dual<T> operator*(Dual<T> a, Du
 template <typename T> __device__ __forceinline__ Dual<T> operator*(T a, Dual<T> b) { return {a * b.v, a * b.e}; }
 
 // q for one radial index: q3[l] (l < l3), then 4-body, then 5-body (nq outputs).
+// The 4/5-body terms need l3 = 4, so their slots are q[4] and q[5]. Every index
+// below is a compile-time constant once unrolled, keeping s, q and D in registers.
+constexpr int LSTART[5] = {0, 3, 8, 15, 24};
 template <typename T, typename S>
 __device__ __forceinline__ int q_of_s(const S s[NLM], int l3, int l4, int l5, S q[6]) {
-    int nq = 0;
-    for (int l = 0; l < l3; l++) {
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
         S acc(T(0));
-        for (int m = kLstart[l]; m < kLstart[l + 1]; m++) acc = acc + T(kC3B[m]) * (s[m] * s[m]);
-        q[nq++] = acc;
+#pragma unroll
+        for (int m = LSTART[l]; m < LSTART[l + 1]; m++) acc = acc + T(kC3B[m]) * (s[m] * s[m]);
+        q[l] = acc;
     }
     if (l4 > 0) {
-        q[nq++] = T(kC4B[0]) * (s[3] * s[3] * s[3]) + T(kC4B[1]) * (s[3] * (s[4] * s[4] + s[5] * s[5])) +
+        q[4] = T(kC4B[0]) * (s[3] * s[3] * s[3]) + T(kC4B[1]) * (s[3] * (s[4] * s[4] + s[5] * s[5])) +
                   T(kC4B[2]) * (s[3] * (s[6] * s[6] + s[7] * s[7])) + T(kC4B[3]) * (s[6] * (s[5] * s[5] - s[4] * s[4])) +
                   T(kC4B[4]) * (s[4] * s[5] * s[7]);
     }
     if (l5 > 0) {
         const S s0 = s[0] * s[0], s12 = s[1] * s[1] + s[2] * s[2];
-        q[nq++] = T(kC5B[0]) * (s0 * s0) + T(kC5B[1]) * (s0 * s12) + T(kC5B[2]) * (s12 * s12);
+        q[5] = T(kC5B[0]) * (s0 * s0) + T(kC5B[1]) * (s0 * s12) + T(kC5B[2]) * (s12 * s12);
     }
-    return nq;
+    return l3 + (l4 > 0) + (l5 > 0);
 }
 
 // D[m] = sum_F gq[F] dq_F/ds_m (gq in q_of_s order).
 template <typename T, typename S>
 __device__ __forceinline__ void dq_ds(const S s[NLM], const T *gq, int l3, int l4, int l5, S D[NLM]) {
+#pragma unroll
     for (int m = 0; m < NLM; m++) D[m] = S(T(0));
-    int f = 0;
-    for (int l = 0; l < l3; l++, f++)
-        for (int m = kLstart[l]; m < kLstart[l + 1]; m++) D[m] = D[m] + (T(2) * T(kC3B[m]) * gq[f]) * s[m];
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+        if (l < l3)
+#pragma unroll
+            for (int m = LSTART[l]; m < LSTART[l + 1]; m++) D[m] = D[m] + (T(2) * T(kC3B[m]) * gq[l]) * s[m];
     if (l4 > 0) {
-        const T g = gq[f++];
+        const T g = gq[4];
         const T c0 = T(kC4B[0]), c1 = T(kC4B[1]), c2 = T(kC4B[2]), c3 = T(kC4B[3]), c4 = T(kC4B[4]);
         D[3] = D[3] + g * (T(3) * c0 * (s[3] * s[3]) + c1 * (s[4] * s[4] + s[5] * s[5]) + c2 * (s[6] * s[6] + s[7] * s[7]));
         D[4] = D[4] + g * (T(2) * c1 * (s[3] * s[4]) - T(2) * c3 * (s[6] * s[4]) + c4 * (s[5] * s[7]));
@@ -171,7 +176,7 @@ __device__ __forceinline__ void dq_ds(const S s[NLM], const T *gq, int l3, int l
         D[7] = D[7] + g * (T(2) * c2 * (s[3] * s[7]) + c4 * (s[4] * s[5]));
     }
     if (l5 > 0) {
-        const T g = gq[f++];
+        const T g = gq[5];
         const T c0 = T(kC5B[0]), c1 = T(kC5B[1]), c2 = T(kC5B[2]);
         const S s12 = s[1] * s[1] + s[2] * s[2];
         D[0] = D[0] + g * (T(4) * c0 * (s[0] * s[0] * s[0]) + T(2) * c1 * (s[0] * s12));
@@ -357,14 +362,35 @@ __device__ __forceinline__ void block_flush_add(T *lds, T *glob, int count) {
     }
 }
 
+// Feature kernels. One wavefront per centre atom. Per-neighbor work runs with
+// lanes over the atom's list entries (slots) and writes per-slot factors to a
+// per-wave LDS chunk. Every sum over slots then runs transposed, with lanes
+// over its outputs reading the chunk as broadcasts, so no two lanes of a
+// wavefront ever add to the same address:
+//   radial  q_n                  lanes over slots, wave sums (registers)
+//   radial  dL/dc[ti][tj][n][q]  = g_n sum_{k: tj} v_k[q]: lanes over (tj, q)
+//   angular s_nm, P_nm           = sum_k u_k[n] w_k[m]: lanes over (n, m)
+//   angular dL/dc                = sum_k u_k[n] v_k[q]: lanes over (n, q), one
+//                                  LDS add per lane per slot into the block's table
+// The block's coefficient-gradient table goes to global memory with one atomic
+// per entry at the end.
 // mode: 0 = forward features, 1 = backward (gRi, gc), 2 = second order (ggfeat, gc)
+constexpr int RCH = WAVE;        // radial slots per chunk
+constexpr int ACH = 32;          // angular slots per chunk (at most)
+template <int MODE>
+__host__ __device__ constexpr int ang_ch() { return MODE == 2 ? 16 : ACH; }
+constexpr int ANG_WAVES = 2;     // wavefronts per angular block (LDS per wave is large)
+
 template <typename T, int MODE>
 __global__ void __launch_bounds__(NEP_WAVES * WAVE)
 nep_radial_kernel(FeatArgs a) {
     extern __shared__ unsigned char smem_raw[];
-    T *gcl = reinterpret_cast<T *>(smem_raw);                // [T*T*nm*nb] block accumulator (MODE > 0)
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     const int ncoef = a.T * a.T * a.nm * a.nb;
+    T *gcl = reinterpret_cast<T *>(smem_raw);                    // [T*T*nm*nb] block table (MODE > 0)
+    // per-wave chunk: v[RCH][nb] and the neighbor type of each slot (MODE > 0)
+    T *vch = gcl + ncoef + (size_t)wid * RCH * a.nb;
+    int *tch = reinterpret_cast<int *>(gcl + ncoef + (size_t)NEP_WAVES * RCH * a.nb) + wid * RCH;
     if (MODE > 0) {
         for (int x = threadIdx.x; x < ncoef; x += blockDim.x) gcl[x] = T(0);
         __syncthreads();
@@ -374,74 +400,120 @@ nep_radial_kernel(FeatArgs a) {
     for (int64_t i = (int64_t)blockIdx.x * NEP_WAVES + wid; i < a.N; i += (int64_t)gridDim.x * NEP_WAVES) {
         const int ti = (int)a.atom_map[i];
         const int cnt = slot_count(a.NL, i, a.S);
-        T acc[NM_MAX];
-        for (int n = 0; n < a.nm; n++) acc[n] = T(0);
-        T g[NM_MAX];
-        if (MODE > 0)
-            for (int n = 0; n < a.nm; n++) g[n] = reinterpret_cast<const T *>(a.gfeat)[i * a.ld + a.col0 + n];
-        for (int k = lane; k < a.S; k += WAVE) {
-            T *gRi = MODE == 1 ? reinterpret_cast<T *>(a.gRi) + (i * a.S + k) * 4 : nullptr;
-            if (k >= cnt) {
-                if (MODE == 1) { gRi[0] = gRi[1] = gRi[2] = gRi[3] = T(0); }
-                continue;
+        T acc[NM_MAX], g[NM_MAX];
+#pragma unroll
+        for (int n = 0; n < NM_MAX; n++) {
+            acc[n] = T(0);
+            g[n] = (MODE > 0 && n < a.nm) ? reinterpret_cast<const T *>(a.gfeat)[i * a.ld + a.col0 + n] : T(0);
+        }
+        if (MODE == 1)
+            for (int k = cnt + lane; k < a.S; k += WAVE) {
+                T *gRi = reinterpret_cast<T *>(a.gRi) + (i * a.S + k) * 4;
+                gRi[0] = gRi[1] = gRi[2] = gRi[3] = T(0);
             }
-            const int tj = (int)a.atom_map[a.NL[i * a.S + k]];
-            const T r = Ri[(i * a.S + k) * 4];
-            T f[NB_MAX], fp[NB_MAX];
-            fk_fkp<T>(r, (T)a.rc, a.nb, f, fp);
-            const T *cij = c + ((int64_t)(ti * a.T + tj) * a.nm) * a.nb;
-            T *gc = gcl + ((ti * a.T + tj) * a.nm) * a.nb;
-            if (MODE == 0) {
-                for (int n = 0; n < a.nm; n++) {
-                    T s = T(0);
-                    for (int q = 0; q < a.nb; q++) s += cij[n * a.nb + q] * f[q];
-                    acc[n] += s;
+        for (int k0 = 0; k0 < cnt; k0 += RCH) {
+            const int k = k0 + lane;
+            if (k < cnt) {
+                const int tj = (int)a.atom_map[a.NL[i * a.S + k]];
+                const T r = Ri[(i * a.S + k) * 4];
+                const T gg = MODE == 2 ? reinterpret_cast<const T *>(a.gg)[(i * a.S + k) * 4] : T(0);
+                const T *cij = c + ((int64_t)(ti * a.T + tj) * a.nm) * a.nb;
+                Basis<T> bs(r, (T)a.rc);
+                T sn[NM_MAX];
+#pragma unroll
+                for (int n = 0; n < NM_MAX; n++) sn[n] = T(0);
+                for (int q = 0; q < a.nb; q++) {
+                    T f, fp;
+                    bs.next(f, fp);
+                    const T fq = MODE == 0 ? f : fp;
+#pragma unroll
+                    for (int n = 0; n < NM_MAX; n++)
+                        if (n < a.nm) sn[n] += cij[n * a.nb + q] * fq;
+                    if (MODE == 1) vch[lane * a.nb + q] = f;            // dc = g_n f_q
+                    if (MODE == 2) vch[lane * a.nb + q] = gg * fp;      // dc = g_n gg fp_q
                 }
-            } else if (MODE == 1) {
-                T d = T(0);
-                for (int n = 0; n < a.nm; n++) {
-                    T s = T(0);
-                    for (int q = 0; q < a.nb; q++) {
-                        s += cij[n * a.nb + q] * fp[q];
-                        atomicAdd(gc + n * a.nb + q, g[n] * f[q]);
+                if (MODE == 1) {
+                    T d = T(0);
+#pragma unroll
+                    for (int n = 0; n < NM_MAX; n++)
+                        if (n < a.nm) d += g[n] * sn[n];
+                    T *gRi = reinterpret_cast<T *>(a.gRi) + (i * a.S + k) * 4;
+                    gRi[0] = d; gRi[1] = gRi[2] = gRi[3] = T(0);
+                } else {
+#pragma unroll
+                    for (int n = 0; n < NM_MAX; n++) acc[n] += MODE == 0 ? sn[n] : gg * sn[n];
+                }
+                if (MODE > 0) tch[lane] = tj;
+            }
+            if (MODE > 0) {
+                // dc[ti][tj][n][q] += g_n sum_{slots of type tj} v[q]: lanes over (tj, q)
+                __builtin_amdgcn_wave_barrier();
+                const int nk = cnt - k0 < RCH ? cnt - k0 : RCH;
+                for (int o = lane; o < a.T * a.nb; o += WAVE) {
+                    const int tj = o / a.nb, q = o - tj * a.nb;
+                    T sv = T(0);
+                    for (int kk = 0; kk < nk; kk++)
+                        if (tch[kk] == tj) sv += vch[kk * a.nb + q];
+                    if (sv != T(0)) {
+                        T *gcp = gcl + ((ti * a.T + tj) * a.nm) * a.nb + q;
+#pragma unroll
+                        for (int n = 0; n < NM_MAX; n++)
+                            if (n < a.nm) atomicAdd(gcp + n * a.nb, g[n] * sv);
                     }
-                    d += g[n] * s;
                 }
-                gRi[0] = d; gRi[1] = gRi[2] = gRi[3] = T(0);
-            } else {
-                const T gg = reinterpret_cast<const T *>(a.gg)[(i * a.S + k) * 4];
-                for (int n = 0; n < a.nm; n++) {
-                    T s = T(0);
-                    for (int q = 0; q < a.nb; q++) {
-                        s += cij[n * a.nb + q] * fp[q];
-                        atomicAdd(gc + n * a.nb + q, gg * g[n] * fp[q]);
-                    }
-                    acc[n] += gg * s;
-                }
+                __builtin_amdgcn_wave_barrier();
             }
         }
         if (MODE == 0 || MODE == 2) {
             T *out = reinterpret_cast<T *>(MODE == 0 ? a.feats : a.ggfeat);
-            for (int n = 0; n < a.nm; n++) {
-                const T v = wave_sum(acc[n]);
-                if (lane == 0) out[i * a.ld + a.col0 + n] = v;
+#pragma unroll
+            for (int n = 0; n < NM_MAX; n++) {
+                if (n < a.nm) {
+                    const T v = wave_sum(acc[n]);
+                    if (lane == 0) out[i * a.ld + a.col0 + n] = v;
+                }
             }
         }
     }
     if (MODE > 0) block_flush_add(gcl, reinterpret_cast<T *>(a.gc), ncoef);
 }
 
-// angular: per-atom LDS arrays of nm*24 values (s, D, P, E)
+// LDS elements of one angular wave: per-atom arrays A, B, C of nm*24, and the
+// slot chunk (CH = ang_ch slots): MODE 0 u[CH][nm] + w[CH][24]; MODE 1
+// u[CH][nm] + v[CH][nb]; MODE 2 two (u, w) pairs for P, then two (u, v) pairs
+// for the explicit c-gradient.
+template <int MODE>
+__host__ __device__ inline int ang_chunk_elems(int nm, int nb) {
+    constexpr int CH = ang_ch<MODE>();
+    if (MODE == 0) return CH * (nm + NLM);
+    if (MODE == 1) return CH * (nm + nb);
+    return CH * (2 * nm + 2 * NLM) + CH * (2 * nm + 2 * nb);
+}
+template <int MODE>
+__host__ __device__ inline int ang_wave_elems(int nm, int nb) { return 3 * nm * NLM + ang_chunk_elems<MODE>(nm, nb); }
+
 template <typename T, int MODE>
-__global__ void __launch_bounds__(NEP_WAVES * WAVE)
+__global__ void __launch_bounds__(ANG_WAVES * WAVE)
 nep_angular_kernel(FeatArgs a) {
     extern __shared__ unsigned char smem_raw[];
+    constexpr int CH = ang_ch<MODE>();
+    constexpr int NRED = (NM_MAX * NLM + WAVE - 1) / WAVE;
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     const int ncoef = a.T * a.T * a.nm * a.nb;
     const int per = a.nm * NLM;
-    T *gcl = reinterpret_cast<T *>(smem_raw);                   // [ncoef]
-    T *wv = gcl + ncoef + (size_t)wid * 3 * per;                // per wave: arrays A, B, C of nm*24
-    T *A = wv, *B = wv + per, *C = wv + 2 * per;
+    const int nm = a.nm, nb = a.nb;
+    T *gcl = reinterpret_cast<T *>(smem_raw);                                  // [ncoef]
+    T *wv = gcl + ncoef + (size_t)wid * ang_wave_elems<MODE>(nm, nb);
+    T *A = wv, *B = wv + per, *C = wv + 2 * per;                               // s, D, P / E
+    T *X = wv + 3 * per;                                                       // slot chunk
+    int *tch = reinterpret_cast<int *>(gcl + ncoef + (size_t)ANG_WAVES * ang_wave_elems<MODE>(nm, nb)) +
+               wid * ACH;
+    // chunk views
+    T *u1 = X, *u2 = X + CH * nm;                          // MODE 0/1: u = u1; MODE 2: gnp, gn
+    T *w1 = X + (MODE == 2 ? 2 : 1) * CH * nm;             // MODE 0: b; MODE 2: gg0 b
+    T *w2 = w1 + CH * NLM;                                 // MODE 2: gg . db
+    T *z1 = X + 2 * CH * nm + 2 * CH * NLM, *z2 = z1 + CH * nm;   // MODE 2: B beta, B delta
+    T *v1 = MODE == 1 ? X + CH * nm : z2 + CH * nm, *v2 = v1 + CH * nb;
     if (MODE > 0) {
         for (int x = threadIdx.x; x < ncoef; x += blockDim.x) gcl[x] = T(0);
         __syncthreads();
@@ -450,120 +522,233 @@ nep_angular_kernel(FeatArgs a) {
     const T *Ri = reinterpret_cast<const T *>(a.Ri);
     const T *gfe = reinterpret_cast<const T *>(a.gfeat);
     const int nf = a.l3 + (a.l4 > 0) + (a.l5 > 0);
-    for (int64_t i = (int64_t)blockIdx.x * NEP_WAVES + wid; i < a.N; i += (int64_t)gridDim.x * NEP_WAVES) {
+    const int nmb = nm * nb;
+    // dc[ti][tch[kk]][n][q] += sum of the u*v pairs of each slot kk of a chunk: lanes over (n, q)
+    auto add_dc = [&](int ti, int nk, const T *p1, const T *q1, const T *p2, const T *q2) {
+        __builtin_amdgcn_wave_barrier();
+        for (int o = lane; o < nmb; o += WAVE) {
+            const int n = o / nb, q = o - n * nb;
+            for (int kk = 0; kk < nk; kk++) {
+                T d = p1[kk * nm + n] * q1[kk * nb + q];
+                if (p2) d += p2[kk * nm + n] * q2[kk * nb + q];
+                atomicAdd(gcl + (ti * a.T + tch[kk]) * nmb + o, d);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    };
+    // g_n = sum_q c[n][q] f_q and g'_n over the streamed basis; optionally stores f_q / f'_q to LDS rows
+    auto radial_g = [&](T r, const T *cij, T gn[NM_MAX], T gnp[NM_MAX], T *vf, T *vfp) {
+        Basis<T> bs(r, (T)a.rc);
+#pragma unroll
+        for (int n = 0; n < NM_MAX; n++) { gn[n] = T(0); gnp[n] = T(0); }
+        for (int q = 0; q < nb; q++) {
+            T f, fp;
+            bs.next(f, fp);
+#pragma unroll
+            for (int n = 0; n < NM_MAX; n++)
+                if (n < nm) { gn[n] += cij[n * nb + q] * f; gnp[n] += cij[n * nb + q] * fp; }
+            if (vf) vf[q] = f;
+            if (vfp) vfp[q] = fp;
+        }
+    };
+    for (int64_t i = (int64_t)blockIdx.x * ANG_WAVES + wid; i < a.N; i += (int64_t)gridDim.x * ANG_WAVES) {
         const int ti = (int)a.atom_map[i];
         const int cnt = slot_count(a.NL, i, a.S);
         const T *s_at = reinterpret_cast<const T *>(a.s_save) + i * per;
-        // ---- stage per-atom arrays
-        if (MODE == 0) {
-            for (int x = lane; x < per; x += WAVE) A[x] = T(0);           // A = s
-        } else {
-            for (int x = lane; x < per; x += WAVE) { A[x] = s_at[x]; C[x] = T(0); }
+        if (MODE > 0) {
+            for (int x = lane; x < per; x += WAVE) A[x] = s_at[x];                // A = s
             __builtin_amdgcn_wave_barrier();
             // B = D = dE/ds (per radial index n, lanes over n)
-            for (int n = lane; n < a.nm; n += WAVE) {
+            for (int n = lane; n < nm; n += WAVE) {
                 T gq[6];
-                for (int f = 0; f < nf; f++) gq[f] = gfe[i * a.ld + a.col0 + aidx(f, n, a.nm)];
+#pragma unroll
+                for (int f = 0; f < 6; f++) gq[f] = f < nf ? gfe[i * a.ld + a.col0 + aidx(f, n, nm)] : T(0);
                 T D[NLM];
                 dq_ds<T, T>(A + n * NLM, gq, a.l3, a.l4, a.l5, D);
                 for (int m = 0; m < NLM; m++) B[n * NLM + m] = D[m];
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        // ---- slot pass 1
-        for (int k = lane; k < a.S; k += WAVE) {
-            T *gRi = MODE == 1 ? reinterpret_cast<T *>(a.gRi) + (i * a.S + k) * 4 : nullptr;
-            if (k >= cnt) {
-                if (MODE == 1) { gRi[0] = gRi[1] = gRi[2] = gRi[3] = T(0); }
-                continue;
+        if (MODE == 1)
+            for (int k = cnt + lane; k < a.S; k += WAVE) {
+                T *gRi = reinterpret_cast<T *>(a.gRi) + (i * a.S + k) * 4;
+                gRi[0] = gRi[1] = gRi[2] = gRi[3] = T(0);
             }
-            const int tj = (int)a.atom_map[a.NL[i * a.S + k]];
-            const T *e = Ri + (i * a.S + k) * 4;
-            const T r = e[0];
-            T f[NB_MAX], fp[NB_MAX];
-            fk_fkp<T>(r, (T)a.rc, a.nb, f, fp);
-            T b[NLM], db[NLM][4];
-            blm_and_grad<T>(r, e[1], e[2], e[3], b, db);
-            const T *cij = c + ((int64_t)(ti * a.T + tj) * a.nm) * a.nb;
-            T *gc = gcl + ((ti * a.T + tj) * a.nm) * a.nb;
-            T gr[4] = {T(0), T(0), T(0), T(0)};
-            const T *gg = MODE == 2 ? reinterpret_cast<const T *>(a.gg) + (i * a.S + k) * 4 : nullptr;
-            for (int n = 0; n < a.nm; n++) {
-                T gn = T(0), gnp = T(0);
-                for (int q = 0; q < a.nb; q++) { gn += cij[n * a.nb + q] * f[q]; gnp += cij[n * a.nb + q] * fp[q]; }
+        __builtin_amdgcn_wave_barrier();
+        // accumulators of the transposed sums (MODE 0: s, MODE 2: P); lane owns outputs lane + 64 j
+        T red[NRED];
+#pragma unroll
+        for (int j = 0; j < NRED; j++) red[j] = T(0);
+        for (int k0 = 0; k0 < cnt; k0 += CH) {
+            const int nk = cnt - k0 < CH ? cnt - k0 : CH;
+            const int k = k0 + lane;
+            if (lane < nk) {
+                const int tj = (int)a.atom_map[a.NL[i * a.S + k]];
+                const T *e = Ri + (i * a.S + k) * 4;
+                const T r = e[0], ir = T(1) / r;
+                const T ux = e[1] * ir, uy = e[2] * ir, uz = e[3] * ir;
+                const T *cij = c + ((int64_t)(ti * a.T + tj) * nm) * nb;
+                tch[lane] = tj;
+                T gn[NM_MAX], gnp[NM_MAX];
                 if (MODE == 0) {
-                    for (int m = 0; m < NLM; m++) atomicAdd(A + n * NLM + m, gn * b[m]);
+                    radial_g(r, cij, gn, gnp, nullptr, nullptr);
+#pragma unroll
+                    for (int n = 0; n < NM_MAX; n++)
+                        if (n < nm) u1[lane * nm + n] = gn[n];
+                    blm_each<T>(ux, uy, uz, [&](int m, T bv, T, T, T) { w1[lane * NLM + m] = bv; });
                 } else if (MODE == 1) {
-                    T Db = T(0);
+                    // gRi = sum_nm D_nm d(g_n b_m)/dRi; dc[n][q] = f_q (D b)_n
+                    radial_g(r, cij, gn, gnp, v1 + lane * nb, nullptr);
+                    T b[NLM];
+                    blm_each<T>(ux, uy, uz, [&](int m, T bv, T, T, T) { b[m] = bv; });
+                    T Dg[NLM], Db[NM_MAX];
+#pragma unroll
+                    for (int n = 0; n < NM_MAX; n++) Db[n] = T(0);
+                    T gr0 = T(0);
+#pragma unroll
                     for (int m = 0; m < NLM; m++) {
-                        const T D = B[n * NLM + m];
-                        Db += D * b[m];
-                        gr[0] += D * (gnp * b[m] + gn * db[m][0]);
-                        gr[1] += D * gn * db[m][1];
-                        gr[2] += D * gn * db[m][2];
-                        gr[3] += D * gn * db[m][3];
+                        T dg = T(0), dgp = T(0);
+#pragma unroll
+                        for (int n = 0; n < NM_MAX; n++)
+                            if (n < nm) {
+                                const T Bv = B[n * NLM + m];
+                                dg += Bv * gn[n];
+                                dgp += Bv * gnp[n];
+                                Db[n] += Bv * b[m];
+                            }
+                        Dg[m] = dg;
+                        gr0 += dgp * b[m];
                     }
-                    for (int q = 0; q < a.nb; q++) atomicAdd(gc + n * a.nb + q, Db * f[q]);
+#pragma unroll
+                    for (int n = 0; n < NM_MAX; n++)
+                        if (n < nm) u1[lane * nm + n] = Db[n];
+                    // V = sum_m Dg_m db_m/du
+                    T V0 = T(0), V1 = T(0), V2 = T(0);
+                    blm_each<T>(ux, uy, uz, [&](int m, T, T d0, T d1, T d2) {
+                        V0 += Dg[m] * d0; V1 += Dg[m] * d1; V2 += Dg[m] * d2;
+                    });
+                    T *gRi = reinterpret_cast<T *>(a.gRi) + (i * a.S + k) * 4;
+                    gRi[0] = gr0 - (ux * V0 + uy * V1 + uz * V2) * ir;
+                    gRi[1] = V0 * ir;
+                    gRi[2] = V1 * ir;
+                    gRi[3] = V2 * ir;
                 } else {
-                    // C = P = sum_slots gg . d(g_n b_m)/dRi ; c-gradient of the explicit term
+                    // P_nm = sum_slots gg . d(g_n b_m)/dRi = g'_n beta_m + g_n delta_m with
+                    // beta_m = gg0 b_m, delta_m = (db_m/du . (gg_xyz - gg0 u)) / r; explicit c-gradient
+                    // sum_m B_nm dP_nm/dc[n][q] = f'_q (B beta)_n + f_q (B delta)_n
+                    const T *gg = reinterpret_cast<const T *>(a.gg) + (i * a.S + k) * 4;
+                    const T g0 = gg[0];
+                    const T gx = gg[1] - g0 * ux, gy = gg[2] - g0 * uy, gz = gg[3] - g0 * uz;
+                    radial_g(r, cij, gn, gnp, v2 + lane * nb, v1 + lane * nb);
+                    T be[NLM], de[NLM];
+                    blm_each<T>(ux, uy, uz, [&](int m, T bv, T d0, T d1, T d2) {
+                        be[m] = g0 * bv;
+                        de[m] = (d0 * gx + d1 * gy + d2 * gz) * ir;
+                    });
+#pragma unroll
                     for (int m = 0; m < NLM; m++) {
-                        const T t = gg[0] * (gnp * b[m] + gn * db[m][0]) +
-                                    gn * (gg[1] * db[m][1] + gg[2] * db[m][2] + gg[3] * db[m][3]);
-                        atomicAdd(C + n * NLM + m, t);
+                        w1[lane * NLM + m] = be[m];
+                        w2[lane * NLM + m] = de[m];
                     }
-                    for (int q = 0; q < a.nb; q++) {
-                        T acc = T(0);
-                        for (int m = 0; m < NLM; m++) {
-                            const T dfb = gg[0] * (fp[q] * b[m] + f[q] * db[m][0]) +
-                                          f[q] * (gg[1] * db[m][1] + gg[2] * db[m][2] + gg[3] * db[m][3]);
-                            acc += B[n * NLM + m] * dfb;
+#pragma unroll
+                    for (int n = 0; n < NM_MAX; n++)
+                        if (n < nm) {
+                            u1[lane * nm + n] = gnp[n];
+                            u2[lane * nm + n] = gn[n];
+                            T sb = T(0), sd = T(0);
+#pragma unroll
+                            for (int m = 0; m < NLM; m++) {
+                                sb += B[n * NLM + m] * be[m];
+                                sd += B[n * NLM + m] * de[m];
+                            }
+                            z1[lane * nm + n] = sb;
+                            z2[lane * nm + n] = sd;
                         }
-                        atomicAdd(gc + n * a.nb + q, acc);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (MODE == 0 || MODE == 2) {
+                // transposed sum over the chunk's slots: lanes own (n, m) outputs
+#pragma unroll
+                for (int j = 0; j < NRED; j++) {
+                    const int o = lane + j * WAVE;
+                    if (o < per) {
+                        const int n = o / NLM, m = o - n * NLM;
+                        T sacc = T(0);
+                        for (int kk = 0; kk < nk; kk++) {
+                            sacc += u1[kk * nm + n] * w1[kk * NLM + m];
+                            if (MODE == 2) sacc += u2[kk * nm + n] * w2[kk * NLM + m];
+                        }
+                        red[j] += sacc;
                     }
                 }
             }
-            if (MODE == 1) { gRi[0] = gr[0]; gRi[1] = gr[1]; gRi[2] = gr[2]; gRi[3] = gr[3]; }
+            if (MODE == 1) add_dc(ti, nk, u1, v1, nullptr, nullptr);
+            if (MODE == 2) add_dc(ti, nk, z1, v1, z2, v2);
+            if (MODE == 0) __builtin_amdgcn_wave_barrier();
+        }
+        if (MODE == 0 || MODE == 2) {
+            T *dst = MODE == 0 ? A : C;
+#pragma unroll
+            for (int j = 0; j < NRED; j++)
+                if (lane + j * WAVE < per) dst[lane + j * WAVE] = red[j];
         }
         __builtin_amdgcn_wave_barrier();
         if (MODE == 0) {
             T *fo = reinterpret_cast<T *>(a.feats);
             T *so = reinterpret_cast<T *>(a.s_save) + i * per;
             for (int x = lane; x < per; x += WAVE) so[x] = A[x];
-            for (int n = lane; n < a.nm; n += WAVE) {
+            for (int n = lane; n < nm; n += WAVE) {
                 T q[6];
                 const int nq = q_of_s<T, T>(A + n * NLM, a.l3, a.l4, a.l5, q);
-                for (int f = 0; f < nq; f++) fo[i * a.ld + a.col0 + aidx(f, n, a.nm)] = q[f];
+#pragma unroll
+                for (int f = 0; f < 6; f++)
+                    if (f < nq) fo[i * a.ld + a.col0 + aidx(f, n, nm)] = q[f];
             }
         } else if (MODE == 2) {
-            // ggfeat[F] = J P (directional derivative of q along P); E = H(gq) P  -> stored in A
+            // ggfeat = J P (directional derivative of q along P); E = H(gq) P -> stored in C
             T *go = reinterpret_cast<T *>(a.ggfeat);
-            for (int n = lane; n < a.nm; n += WAVE) {
+            for (int n = lane; n < nm; n += WAVE) {
                 Dual<T> sd[NLM];
                 for (int m = 0; m < NLM; m++) sd[m] = Dual<T>(A[n * NLM + m], C[n * NLM + m]);
                 Dual<T> q[6];
                 const int nq = q_of_s<T, Dual<T>>(sd, a.l3, a.l4, a.l5, q);
-                for (int f = 0; f < nq; f++) go[i * a.ld + a.col0 + aidx(f, n, a.nm)] = q[f].e;
+#pragma unroll
+                for (int f = 0; f < 6; f++)
+                    if (f < nq) go[i * a.ld + a.col0 + aidx(f, n, nm)] = q[f].e;
                 T gq[6];
-                for (int f = 0; f < nf; f++) gq[f] = gfe[i * a.ld + a.col0 + aidx(f, n, a.nm)];
+#pragma unroll
+                for (int f = 0; f < 6; f++) gq[f] = f < nf ? gfe[i * a.ld + a.col0 + aidx(f, n, nm)] : T(0);
                 Dual<T> D[NLM];
                 dq_ds<T, Dual<T>>(sd, gq, a.l3, a.l4, a.l5, D);
                 for (int m = 0; m < NLM; m++) C[n * NLM + m] = D[m].e;     // E = (H P)_m
             }
             __builtin_amdgcn_wave_barrier();
-            // c-gradient through s: sum_slots sum_m E_nm f_k b_m
-            for (int k = lane; k < cnt; k += WAVE) {
-                const int tj = (int)a.atom_map[a.NL[i * a.S + k]];
-                const T *e = Ri + (i * a.S + k) * 4;
-                T f[NB_MAX], fp[NB_MAX];
-                fk_fkp<T>(e[0], (T)a.rc, a.nb, f, fp);
-                T b[NLM], db[NLM][4];
-                blm_and_grad<T>(e[0], e[1], e[2], e[3], b, db);
-                T *gc = gcl + ((ti * a.T + tj) * a.nm) * a.nb;
-                for (int n = 0; n < a.nm; n++) {
-                    T Eb = T(0);
-                    for (int m = 0; m < NLM; m++) Eb += C[n * NLM + m] * b[m];
-                    for (int q = 0; q < a.nb; q++) atomicAdd(gc + n * a.nb + q, Eb * f[q]);
+            // c-gradient through s: dc[ti][tj][n][q] += (E b)_n f_q over the slots
+            for (int k0 = 0; k0 < cnt; k0 += CH) {
+                const int nk = cnt - k0 < CH ? cnt - k0 : CH;
+                if (lane < nk) {
+                    const int k = k0 + lane;
+                    const T *e = Ri + (i * a.S + k) * 4;
+                    const T ir = T(1) / e[0];
+                    Basis<T> bs(e[0], (T)a.rc);
+                    for (int q = 0; q < nb; q++) {
+                        T f, fp;
+                        bs.next(f, fp);
+                        v1[lane * nb + q] = f;
+                    }
+                    T b[NLM];
+                    blm_each<T>(e[1] * ir, e[2] * ir, e[3] * ir, [&](int m, T bv, T, T, T) { b[m] = bv; });
+                    tch[lane] = (int)a.atom_map[a.NL[i * a.S + k]];
+#pragma unroll
+                    for (int n = 0; n < NM_MAX; n++)
+                        if (n < nm) {
+                            T Eb = T(0);
+#pragma unroll
+                            for (int m = 0; m < NLM; m++) Eb += C[n * NLM + m] * b[m];
+                            u1[lane * nm + n] = Eb;
+                        }
                 }
+                add_dc(ti, nk, u1, v1, nullptr, nullptr);
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -675,13 +860,21 @@ int feat_launch(FeatArgs &a, int angular, int mode, hipStream_t st) {
     MATPL_CHECK_ARG(!angular || a.l5 == 0 || a.l4 > 0, "nep_feat: the 5-body term needs the 4-body term");
     if (a.N == 0) return OK;
     const size_t ncoef = (size_t)a.T * a.T * a.nm * a.nb;
-    size_t lds = (mode > 0 ? ncoef : 0) * sizeof(T);
-    if (angular) lds = ncoef * sizeof(T) + (size_t)NEP_WAVES * 3 * a.nm * NLM * sizeof(T);
+    size_t lds = 0;
+    if (!angular && mode > 0)
+        lds = (ncoef + (size_t)NEP_WAVES * RCH * a.nb) * sizeof(T) + (size_t)NEP_WAVES * RCH * sizeof(int);
+    if (angular) {
+        const int we = mode == 0 ? ang_wave_elems<0>(a.nm, a.nb)
+                     : mode == 1 ? ang_wave_elems<1>(a.nm, a.nb) : ang_wave_elems<2>(a.nm, a.nb);
+        lds = (ncoef + (size_t)ANG_WAVES * we) * sizeof(T) + (size_t)ANG_WAVES * ACH * sizeof(int);
+    }
     MATPL_CHECK_ARG(lds <= 64 * 1024, "nep_feat: LDS %zu too large", lds);
-    int64_t blocks = (a.N + NEP_WAVES - 1) / NEP_WAVES;
-    if (mode > 0 && blocks > 512) blocks = 512;      // few blocks -> few global coefficient atomics
+    const int waves = angular ? ANG_WAVES : NEP_WAVES;
+    int64_t blocks = (a.N + waves - 1) / waves;
+    // few blocks -> few global coefficient atomics, while keeping >= 4 wavefronts per CU
+    if (mode > 0 && blocks > 2048 / waves) blocks = 2048 / waves;
     if (blocks > 8192) blocks = 8192;
-    dim3 grid((unsigned)blocks), block(NEP_WAVES * WAVE);
+    dim3 grid((unsigned)blocks), block(waves * WAVE);
     if (!angular) {
         if (mode == 0) hipLaunchKernelGGL((nep_radial_kernel<T, 0>), grid, block, lds, st, a);
         else if (mode == 1) hipLaunchKernelGGL((nep_radial_kernel<T, 1>), grid, block, lds, st, a);
