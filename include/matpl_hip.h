@@ -286,6 +286,26 @@ int matpl_nep_virial(int f64, const int64_t *NL, const void *dE, const void *Rij
                      void *stream);
 
 /*
+ * Centre term of the NEP force, F[i] = -sum_{k,w} dE[i][k][w] Ri_d[i][k][w][:]
+ * over all S slots, F [N][3]. No counterpart in the reference's operator
+ * library: it replaces the torch.matmul that the reference's NEP model uses to
+ * form this term before calculateNepForce (src/model/nep_net.py:608).
+ */
+int matpl_nep_centre_force(int f64, const void *dE, const void *Ri_d, int64_t N, int S, void *F, void *stream);
+
+/*
+ * Per-frame virial [nframes][9] alone (zeroed, then accumulated): the first
+ * output of matpl_nep_virial without the per-atom rows, for callers that do not
+ * use atom_virial (the reference's NEP model returns only the frame virial,
+ * src/model/nep_net.py:438, 613-622). Atom i belongs to frame atom_frame[i] (int32 [N],
+ * frames contiguous); each list entry counts toward its centre atom's frame (lists
+ * never leave a frame, where this equals the sum of the per-atom rows). Same pair terms as launch_calculate_nepvirial
+ * (src/op/include/calculate_nepfeat.h:21).
+ */
+int matpl_nep_frame_virial(int f64, const int64_t *NL, const void *dE, const void *Rij, const void *Ri_d,
+                           const int32_t *atom_frame, int64_t nframes, int64_t N, int S, void *virial, void *stream);
+
+/*
  * Backward of the whole force (centre + neighbor terms) with respect to dE:
  * g[i][k] = Ri_d[i][k] . (gF[NL[i][k]] - gF[i]), g [N][S][4] from gF [N][3].
  * Replaces launch_calculate_nepforce_grad (src/op/include/calculate_nepfeat.h:11).

@@ -5,7 +5,11 @@ CalcOps_cuda.calculateNepFeat / calculateNepMbFeat, per-type fitting nets,
 dE = dEtot/dRi by autograd (create_graph, so force losses train the
 parameters), Ri_d from [r, x, y, z] (nep_net.py:440-468), forces and virial
 from calculateNepForce / calculateNepVirial, returned negated as the
-reference does.
+reference does. Where the reference forms the centre force with a batched
+torch.matmul (nep_net.py:608) this uses a HIP reduction; the virial comes
+from the frame-only variant of the virial op (the per-atom rows are unused).
+Fitting-layer products run tall-by-small or split over atoms (_TallMM), at
+every derivative order.
 
 Host-side input packing (``pack_frames``) restates expand_box
 (src/pre_data/nep_data_loader.py:220-254): replicate each cell num_cell times
@@ -104,6 +108,44 @@ def ri_d(Ri):
     return out
 
 
+_SPLIT_ROWS = 512
+
+
+def _tall_tn(x, g):
+    """x^T g for tall x [K, a] and g [K, b] with K >> a, b: the reduction over K is split into
+    row blocks of a batched product and summed (one GEMM tile looping over all K is serial).
+    Built from differentiable ops, so its own backward is two tall-by-small products."""
+    K = x.shape[0]
+    if K <= 2 * _SPLIT_ROWS:
+        return x.t() @ g
+    nblk = -(-K // _SPLIT_ROWS)
+    pad = nblk * _SPLIT_ROWS - K
+    if pad:
+        x = torch.nn.functional.pad(x, (0, 0, 0, pad))
+        g = torch.nn.functional.pad(g, (0, 0, 0, pad))
+    xs = x.reshape(nblk, _SPLIT_ROWS, x.shape[1]).transpose(1, 2)
+    gs = g.reshape(nblk, _SPLIT_ROWS, g.shape[1])
+    return torch.bmm(xs, gs).sum(0)
+
+
+class _TallMM(torch.autograd.Function):
+    """x @ W for tall x [K, a], small W [a, b]; the weight gradient goes through _tall_tn and the
+    input gradient through _TallMM again, so every product of any derivative order is either
+    tall-by-small or split over K."""
+
+    @staticmethod
+    def forward(ctx, x, W):
+        ctx.save_for_backward(x, W)
+        return x @ W
+
+    @staticmethod
+    def backward(ctx, g):
+        x, W = ctx.saved_tensors
+        gx = _TallMM.apply(g, W.t()) if ctx.needs_input_grad[0] else None
+        gW = _tall_tn(x, g) if ctx.needs_input_grad[1] else None
+        return gx, gW
+
+
 class _FitNet(torch.nn.Module):
     """NEP fitting net (src/model/nep_fitting.py:40-163): tanh first layer, residual hidden layers
     (with resnet_dt), linear output."""
@@ -124,7 +166,7 @@ class _FitNet(torch.nn.Module):
     def forward(self, x):
         L = len(self.W)
         for l in range(L):
-            z = x @ self.W[l] + self.b[l]
+            z = _TallMM.apply(x, self.W[l]) + self.b[l]
             if l == L - 1:
                 return z
             h = torch.tanh(z)
@@ -174,8 +216,9 @@ class NEPTorch(torch.nn.Module):
         l3 = self.cfg.l_max[0]
         feats = self.features(Ri_r, NL_r, Ri_a, NL_a, atom_type_map)
         Ei = self.energies(feats, atom_type_map)
-        sizes = num_atom.reshape(-1).tolist()
-        Etot = torch.stack([e.sum() for e in Ei.split(sizes)]).unsqueeze(-1)
+        nf = num_atom.numel()
+        frame = torch.repeat_interleave(torch.arange(nf, device=Ei.device), num_atom.reshape(-1).to(Ei.device))
+        Etot = torch.zeros(nf, dtype=Ei.dtype, device=Ei.device).index_add(0, frame, Ei).unsqueeze(-1)
         if not is_calc_f:
             return Etot, Ei, None, None
         if create_graph is None:
@@ -189,9 +232,8 @@ class NEPTorch(torch.nn.Module):
                 continue
             N, S = NL.shape
             d = ri_d(Ri.detach())
-            centre = -torch.matmul(dE.reshape(N, 1, S * 4), d.reshape(N, S * 4, 3)).squeeze(-2)
-            F = ops.calculateNepForce(NL, dE, d, centre)[0]
-            V = ops.calculateNepVirial(NL, dE, Ri.detach()[..., 1:].contiguous(), d, num_atom)[0]
+            F = ops.calculateNepForce(NL, dE, d, nep_ops.nep_centre_force(dE, d))[0]
+            V = nep_ops.nep_frame_virial(NL, dE, Ri.detach()[..., 1:].contiguous(), d, num_atom)
             Force = F if Force is None else Force + F
             Virial = V if Virial is None else Virial + V
         return Etot, Ei, -Force, -Virial

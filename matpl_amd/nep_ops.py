@@ -236,6 +236,60 @@ class NepVirial(torch.autograd.Function):
         return None, g, None, None, None
 
 
+class NepFrameVirial(torch.autograd.Function):
+    """The per-frame virial of calculateNepVirial without the per-atom rows (what the NEP model
+    keeps, src/model/nep_net.py:613-622); same backward as NepVirial."""
+
+    @staticmethod
+    def forward(ctx, NL, dE, Rij, Ri_d, num_atom):
+        NL = NL.to(torch.int64).contiguous()
+        dE = dE.contiguous()
+        Rij = Rij.to(dE.dtype).contiguous()
+        Ri_d = Ri_d.to(dE.dtype).contiguous()
+        N, S = NL.shape
+        _expect(dE, N * S * 4, "dE [N, S, 4]")
+        _expect(Ri_d, N * S * 12, "Ri_d [N, S, 4, 3]")
+        _expect(Rij, N * S * 3, "Rij [N, S, 3]")
+        start, af = _frame_index(num_atom, dE.device)
+        if af.numel() != N:
+            raise ValueError(f"nep frame virial: num_atom sums to {af.numel()}, list has {N} atoms")
+        nf = start.numel() - 1
+        virial = torch.empty((nf, 9), dtype=dE.dtype, device=dE.device)
+        _lib.call("matpl_nep_frame_virial", _f64(dE), _ptr(NL), _ptr(dE), _ptr(Rij), _ptr(Ri_d), _ptr(af), nf, N, S,
+                  _ptr(virial), _stream())
+        ctx.save_for_backward(NL, Rij, Ri_d, af)
+        ctx.dE_shape = dE.shape
+        return virial
+
+    @staticmethod
+    def backward(ctx, gV):
+        NL, Rij, Ri_d, af = ctx.saved_tensors
+        N, S = NL.shape
+        g = torch.empty(ctx.dE_shape, dtype=Ri_d.dtype, device=Ri_d.device)
+        gV = gV.to(Ri_d.dtype).contiguous()
+        _lib.call("matpl_nep_virial_grad", _f64(Ri_d), _ptr(NL), _ptr(Rij), _ptr(Ri_d), _ptr(gV), _ptr(af), _ptr(g),
+                  N, S, _stream())
+        return None, g, None, None, None
+
+
+def nep_frame_virial(NL, dE, Rij, Ri_d, num_atom):
+    return NepFrameVirial.apply(NL, dE, Rij, Ri_d, num_atom)
+
+
+def nep_centre_force(dE, Ri_d):
+    """-sum_{k,w} dE[i,k,w] Ri_d[i,k,w,:] -> [N, 3] (the centre term src/model/nep_net.py:608 forms with
+    torch.matmul). Not differentiable: calculateNepForce's backward carries the centre term itself and
+    returns no gradient for F."""
+    N = dE.shape[0]
+    S = dE.numel() // max(4 * N, 1)
+    _expect(Ri_d, N * S * 12, "Ri_d [N, S, 4, 3]")
+    dE = dE.detach().contiguous()
+    Ri_d = Ri_d.detach().to(dE.dtype).contiguous()
+    F = torch.empty((N, 3), dtype=dE.dtype, device=dE.device)
+    _lib.call("matpl_nep_centre_force", _f64(dE), _ptr(dE), _ptr(Ri_d), N, S, _ptr(F), _stream())
+    return F
+
+
 # ------------------------------------------------------------------ neighbor search
 def _nep_geo(num_atoms, box, box_orig, num_cell, position):
     dev = position.device

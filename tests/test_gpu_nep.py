@@ -190,6 +190,40 @@ def test_force_virial_ops_and_grads(dev):
                 continue
             wantv[i, s] = Ridc[i, s] @ gV.cpu()[fr].reshape(3, 3) @ Ric[i, s, 1:]
     assert torch.allclose(gotv.cpu(), wantv, rtol=1e-12, atol=1e-12)
+    # model-path variants: centre term by HIP reduction, frame virial without per-atom rows
+    assert torch.allclose(nep_ops.nep_centre_force(dE, Rid).cpu(), -pair.sum(1), rtol=1e-12, atol=1e-12)
+    # (sums by the centre atom's frame; the random lists here cross frames, real ones never do)
+    Vc = torch.zeros(2, 9, dtype=torch.float64)
+    for i in range(N):
+        for s in range(S):
+            if int(NLc[i, s]) >= 0:
+                Vc[0 if i < 4 else 1] += torch.outer(pair[i, s], Ric[i, s, 1:]).reshape(9)
+    Vf = nep_ops.nep_frame_virial(NL, dE, Ri[..., 1:].contiguous(), Rid, num_atom)
+    assert torch.allclose(Vf.cpu(), Vc, rtol=1e-12, atol=1e-12)
+    (gotf,) = torch.autograd.grad(Vf, dE, gV)
+    assert torch.allclose(gotf.cpu(), wantv, rtol=1e-12, atol=1e-12)
+
+
+def test_frame_virial_many_frames(dev):
+    """Frames shorter and longer than one wavefront's run of atoms, and an empty frame."""
+    _, Ri, NL, _ = _rand_inputs(dev, N=40, S=70, seed=7)
+    N, S = NL.shape
+    g = torch.Generator().manual_seed(8)
+    sizes = [3, 0, 1, 17, 9, 10]
+    num_atom = torch.tensor(sizes, device=dev)
+    # neighbor indices inside each atom's own frame, -1 padding kept
+    start = np.cumsum([0] + sizes)
+    fr = np.repeat(np.arange(len(sizes)), sizes)
+    lo = torch.as_tensor(start[fr]).unsqueeze(1)
+    width = torch.as_tensor(np.array(sizes)[fr]).unsqueeze(1)
+    NL = torch.where(NL.cpu() >= 0, lo + torch.randint(0, 1 << 20, (N, S), generator=g) % width, NL.cpu()).to(dev)
+    dE = torch.randn(N, S, 4, generator=g, dtype=torch.float64).to(dev)
+    Rid = mn.ri_d(Ri)
+    AV = nep_ops.calculateNepVirial(NL, dE, Ri[..., 1:].contiguous(), Rid, num_atom)[1].cpu()
+    want = torch.stack([r.sum(0) for r in AV.split(num_atom.tolist())])
+    for dt, tol in ((torch.float64, 1e-12), (torch.float32, 1e-4)):
+        got = nep_ops.nep_frame_virial(NL, dE.to(dt), Ri[..., 1:].to(dt).contiguous(), Rid.to(dt), num_atom)
+        assert torch.allclose(got.double().cpu(), want, rtol=tol, atol=tol * want.abs().max().item())
 
 
 @pytest.mark.parametrize("lmax", [(4, 2, 0), (4, 2, 1)])
