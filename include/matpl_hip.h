@@ -113,6 +113,32 @@ int matpl_dp_neighbor_cell(const double *lattice_host, const double *frac, const
                            void *workspace, int64_t workspace_bytes, void *stream);
 
 /*
+ * DP neighbor data from a LAMMPS full neighbor list, on the device. Replaces
+ * PairMATPL::generate_neighdata (src/lmps/lammps-libtorch/MATPL/pair_matpl.cpp:522-616).
+ * Inputs: x [nall][3] float64 (local then ghost atoms), type [nall] int32
+ * (LAMMPS types, 1-based), type_to_model [ntypes_lmp] int32 (model type
+ * index per LAMMPS type, -1 = not in the model), ilist [inum], numneigh
+ * [inum] and firstneigh [inum] int64 (offsets into neigh [nneigh], int32
+ * atom indices < nall), all indexed by atom as in LAMMPS. For entry ii, centre
+ * i = ilist[ii]: every neighbor j with |x_j - x_i|^2 <= rcut^2, in list
+ * order, goes to bucket type_to_model[type[j]-1] of row i:
+ * list [inum][ntypes][m] int32 = j + 1 (0 = empty), dR [inum][ntypes][m][4]
+ * float64 = [r, dx, dy, dz], imagetype [inum] = model type of i, counts
+ * [inum][ntypes] (may be NULL) = neighbors found. min_rsq (may be NULL;
+ * device uint64 holding +inf double bits before the call) receives the
+ * smallest accepted |dR|^2 as double bits. status (device int32, zero
+ * before the call): 1 + i for a bucket over m, -1 centre index outside
+ * [0, inum), -2 neighbor index outside [0, nall) or list range outside
+ * neigh, -3 unmapped type.
+ */
+int matpl_dp_lmp_neighdata(const double *x, const int32_t *type, const int32_t *type_to_model, int ntypes_lmp,
+                           const int32_t *ilist, const int32_t *numneigh, const int64_t *firstneigh,
+                           const int32_t *neigh, int64_t nneigh, int64_t inum, int64_t nall, int ntypes, int m,
+                           double rcut,
+                           int32_t *imagetype, int32_t *list, double *dR, int32_t *counts, void *min_rsq,
+                           int32_t *status, void *stream);
+
+/*
  * Reverse slot map of a symmetric neighbor list: rev [N][T][m] int32, for a
  * used entry s of atom i naming j with image sh, the index of the entry of
  * j (bucket of i's type) naming i with image -sh; -1 on unused entries.

@@ -378,6 +378,124 @@ __global__ void reverse_kernel(const int32_t *__restrict__ list, const int32_t *
     }
 }
 
+// ------------------------------------------------------------------ LAMMPS packing
+// PairMATPL::generate_neighdata (src/lmps/lammps-libtorch/MATPL/pair_matpl.cpp:522-616)
+// on the device. Input is LAMMPS's full neighbor list as CSR (neighbors of atom i
+// are neigh[firstneigh[i] .. firstneigh[i] + numneigh[i]), indices over local +
+// ghost atoms) and positions x [nall][3] of local and ghost atoms. For list entry
+// ii (centre i = ilist[ii]) every neighbor j with |x_j - x_i|^2 <= rc^2 goes,
+// in LAMMPS list order, to bucket t_j = type_to_model[type[j] - 1] of row i:
+//   list [inum][T][m] = j + 1 (0 = empty), dR [inum][T][m][4] = [r, dx, dy, dz],
+//   imagetype[ii] = t_i, counts [inum][T] = neighbors found (may exceed m).
+// Unused slots are zeroed (the reference keeps whatever the buffer held).
+// status (zero before the call) receives the first problem: 1 + i for a bucket
+// that overflowed m (the reference writes past it), -1 for a centre i outside
+// [0, inum), -2 for a neighbor outside [0, nall) or a list range outside
+// neigh[0 .. nneigh), -3 for a type with no model
+// type. min_rsq (uint64, +inf bits before the call) receives the smallest
+// accepted |dR|^2 as double bits (non-negative doubles order like their bits),
+// for the reference's "two atoms too close" check (pair_matpl.cpp:605-611).
+// One wavefront per list entry, lanes over its neighbors, integer wave scans
+// to keep list order inside each bucket.
+__global__ void __launch_bounds__(256)
+lmp_pack_kernel(const double *__restrict__ x, const int32_t *__restrict__ type,
+                const int32_t *__restrict__ type_to_model, int ntypes_lmp, const int32_t *__restrict__ ilist,
+                const int32_t *__restrict__ numneigh, const int64_t *__restrict__ firstneigh,
+                const int32_t *__restrict__ neigh, int64_t nneigh, int64_t inum, int64_t nall, int T, int m,
+                double rc2,
+                int32_t *__restrict__ imagetype, int32_t *__restrict__ list, double *__restrict__ dR,
+                int32_t *__restrict__ counts, unsigned long long *__restrict__ min_rsq,
+                int32_t *__restrict__ status) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t ii = blockIdx.x * (int64_t)(blockDim.x / WAVE) + (threadIdx.x / WAVE);
+    if (ii >= inum) return;
+    const int64_t i = ilist[ii];
+    if (i < 0 || i >= inum) {
+        if (lane == 0) atomicCAS(status, 0, -1);
+        return;
+    }
+    auto model_type = [&](int64_t a) -> int {
+        const int lt = type[a];
+        return (lt >= 1 && lt <= ntypes_lmp) ? type_to_model[lt - 1] : -1;
+    };
+    const int ti = model_type(i);
+    if (ti < 0 || ti >= T) {
+        if (lane == 0) atomicCAS(status, 0, -3);
+        return;
+    }
+    if (lane == 0) imagetype[ii] = ti;
+    const int64_t S = (int64_t)T * m;
+    int32_t *li = list + i * S;
+    double *di = dR + i * S * 4;
+    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+    const int jnum = numneigh[i];
+    const int64_t j0 = firstneigh[i];
+    if (jnum < 0 || j0 < 0 || j0 + jnum > nneigh) {
+        if (lane == 0) atomicCAS(status, 0, -2);
+        return;
+    }
+    const int32_t *jl = neigh + j0;
+    int base[16];
+    for (int t = 0; t < T; t++) base[t] = 0;
+    double rmin = __longlong_as_double(0x7ff0000000000000LL);
+    bool bad_j = false, bad_t = false;
+    for (int jc = 0; jc < jnum; jc += WAVE) {
+        const int jj = jc + lane;
+        int64_t j = jj < jnum ? (int64_t)jl[jj] : -1;
+        if (jj < jnum && (j < 0 || j >= nall)) { bad_j = true; j = -1; }
+        int tj = -1;
+        double dx = 0.0, dy = 0.0, dz = 0.0, rsq = 0.0;
+        if (j >= 0) {
+            dx = x[3 * j] - xi;
+            dy = x[3 * j + 1] - yi;
+            dz = x[3 * j + 2] - zi;
+            rsq = dx * dx + dy * dy + dz * dz;
+            if (rsq <= rc2) {
+                tj = model_type(j);
+                if (tj < 0 || tj >= T) { bad_t = true; tj = -1; }
+            }
+        }
+        int pos = 0;
+        for (int t = 0; t < T; t++) {
+            const int v = tj == t ? 1 : 0;
+            const int incl = wave_incl_scan(v);
+            if (tj == t) pos = base[t] + incl - 1;
+            base[t] += __shfl(incl, WAVE - 1, WAVE);
+        }
+        if (tj >= 0) {
+            rmin = fmin(rmin, rsq);
+            if (pos < m) {
+                const int64_t slot = (int64_t)tj * m + pos;
+                li[slot] = (int32_t)(j + 1);
+                di[4 * slot] = sqrt(rsq);
+                di[4 * slot + 1] = dx;
+                di[4 * slot + 2] = dy;
+                di[4 * slot + 3] = dz;
+            }
+        }
+    }
+    bool over = false;
+    for (int t = 0; t < T; t++) {
+        over = over || base[t] > m;
+        const int used = base[t] < m ? base[t] : m;
+        for (int k = used + lane; k < m; k += WAVE) {
+            const int64_t slot = (int64_t)t * m + k;
+            li[slot] = 0;
+            di[4 * slot] = di[4 * slot + 1] = di[4 * slot + 2] = di[4 * slot + 3] = 0.0;
+        }
+        if (counts && lane == 0) counts[i * T + t] = base[t];
+    }
+    // wave minimum of the accepted |dR|^2, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) rmin = fmin(rmin, __shfl_xor(rmin, off, WAVE));
+    if (lane == 0 && min_rsq) atomicMin(min_rsq, (unsigned long long)__double_as_longlong(rmin));
+    const bool any_bad_j = __any(bad_j), any_bad_t = __any(bad_t);
+    if (lane == 0) {
+        if (any_bad_j) atomicCAS(status, 0, -2);
+        else if (any_bad_t) atomicCAS(status, 0, -3);
+        else if (over) atomicCAS(status, 0, (int)(i + 1));
+    }
+}
+
 bool small_box_host(const double *L, double rc) {
     for (int d = 0; d < 3; d++) {
         const double len = sqrt(L[3 * d] * L[3 * d] + L[3 * d + 1] * L[3 * d + 1] + L[3 * d + 2] * L[3 * d + 2]);
@@ -506,4 +624,27 @@ MATPL_EXPORT int matpl_dp_neighbor_reverse(const int32_t *list, const int32_t *s
     hipLaunchKernelGGL(reverse_kernel, dim3(grid_cap(n * ntypes * m)), dim3(256), 0, (hipStream_t)stream, list, shift,
                        counts, type_map, frame_start, atom_frame, ntypes, m, n, rev, status);
     return check_launch("reverse_kernel");
+}
+
+MATPL_EXPORT int matpl_dp_lmp_neighdata(const double *x, const int32_t *type, const int32_t *type_to_model,
+                                        int ntypes_lmp, const int32_t *ilist, const int32_t *numneigh,
+                                        const int64_t *firstneigh, const int32_t *neigh, int64_t nneigh, int64_t inum,
+                                        int64_t nall,
+                                        int ntypes, int m, double rcut, int32_t *imagetype, int32_t *list,
+                                        double *dR, int32_t *counts, void *min_rsq, int32_t *status,
+                                        void *stream) {
+    MATPL_CHECK_ARG(ntypes >= 1 && ntypes <= 16 && m >= 1 && inum >= 0 && nall >= inum && ntypes_lmp >= 1,
+                    "dp_lmp_neighdata: bad sizes");
+    MATPL_CHECK_ARG(inum * ntypes * (int64_t)m < (int64_t)0x7fffffff, "dp_lmp_neighdata: too many slots for int32");
+    MATPL_CHECK_ARG(nall < (int64_t)0x7fffffff, "dp_lmp_neighdata: too many atoms for int32 indices");
+    MATPL_CHECK_ARG(rcut > 0.0, "dp_lmp_neighdata: rcut must be positive");
+    if (inum == 0) return OK;
+    MATPL_CHECK_ARG(nneigh >= 0, "dp_lmp_neighdata: bad neighbor array size");
+    MATPL_CHECK_ARG(x && type && type_to_model && ilist && numneigh && firstneigh && (neigh || nneigh == 0) &&
+                        imagetype && list && dR && status,
+                    "dp_lmp_neighdata: null pointer");
+    hipLaunchKernelGGL(lmp_pack_kernel, dim3((unsigned)((inum + 3) / 4)), dim3(256), 0, (hipStream_t)stream, x, type,
+                       type_to_model, ntypes_lmp, ilist, numneigh, firstneigh, neigh, nneigh, inum, nall, ntypes, m,
+                       rcut * rcut, imagetype, list, dR, counts, (unsigned long long *)min_rsq, status);
+    return check_launch("lmp_pack_kernel");
 }

@@ -9,6 +9,9 @@ image-enumerating brute-force kernel (the reference's own small-box rule).
 
 An overflowing bucket (more than m neighbors of one type) raises
 OverflowError, as the reference stops with "maxNeighborNum too small".
+
+``generate_neighdata`` is the LAMMPS-side packing (PairMATPL::generate_neighdata)
+from a full neighbor list over local + ghost atoms, on the GPU.
 """
 from __future__ import annotations
 
@@ -145,6 +148,69 @@ def reverse_map(nl: NeighborList, type_map: torch.Tensor, m: int, frame_start=No
     _lib.call("matpl_dp_neighbor_reverse", _ptr(nl.list), _ptr(nl.shift), _ptr(nl.counts), _ptr(tm),
               _ptr(frame_start), _ptr(atom_frame), T, m, n, _ptr(rev), _ptr(status), _stream())
     return rev, status
+
+
+@dataclass
+class LmpNeighData:
+    imagetype_map: torch.Tensor     # [inum] int32, model type of each centre
+    neighbor_list: torch.Tensor     # [1, inum, T*m] int32, j + 1 over local + ghost atoms, 0 = empty
+    dR_neigh: torch.Tensor          # [1, inum, T*m, 4] float64, [r, dx, dy, dz]
+    counts: torch.Tensor            # [inum, T] int32 neighbors found per bucket
+    min_rsq: float                  # smallest accepted |dR|^2 (inf if none)
+
+
+TOO_CLOSE_RSQ = 0.61               # pair_matpl.cpp:605-611 (compared with |dR|^2)
+
+
+def generate_neighdata(x, atom_type, type_to_model, ilist, numneigh, firstneigh, neigh, ntypes: int, m: int,
+                       rcut: float, check=True) -> LmpNeighData:
+    """PairMATPL::generate_neighdata (src/lmps/lammps-libtorch/MATPL/pair_matpl.cpp:522-616) on the GPU.
+
+    x [nall, 3] float64 local-then-ghost positions, atom_type [nall] LAMMPS types (1-based),
+    type_to_model [ntypes_lmp] model type per LAMMPS type (-1 = absent), and LAMMPS's full
+    neighbor list as CSR: ilist [inum], numneigh [inum], firstneigh [inum] (int64 offsets into
+    neigh), neigh (atom indices over local + ghost). Neighbors within rcut (<=) fill the
+    buckets of their model type in list order. With ``check`` (one device sync) an overflowing
+    bucket raises OverflowError (the reference overruns its buffer), bad indices or types raise
+    ValueError, and a pair closer than sqrt(0.61) A raises RuntimeError as the reference does.
+    """
+    dev = x.device
+    inum = int(ilist.numel())
+    nall = int(x.shape[0])
+    S = ntypes * m
+    imagetype = torch.empty(inum, dtype=torch.int32, device=dev)
+    lst = torch.empty((1, inum, S), dtype=torch.int32, device=dev)
+    dR = torch.empty((1, inum, S, 4), dtype=torch.float64, device=dev)
+    counts = torch.empty((inum, ntypes), dtype=torch.int32, device=dev)
+    min_rsq = torch.full((1,), 0x7FF0000000000000, dtype=torch.int64, device=dev)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    xs = x.to(torch.float64).contiguous()
+    ty = atom_type.to(torch.int32).contiguous()
+    t2m = torch.as_tensor(type_to_model, dtype=torch.int32).to(dev).contiguous()
+    il = ilist.to(torch.int32).contiguous()
+    nn = numneigh.to(torch.int32).contiguous()
+    fn = firstneigh.to(torch.int64).contiguous()
+    ng = neigh.to(torch.int32).contiguous()
+    if nall != ty.numel() or nn.numel() < inum or fn.numel() < inum:
+        raise ValueError("generate_neighdata: x/type/numneigh/firstneigh sizes do not match")
+    if inum:
+        _lib.call("matpl_dp_lmp_neighdata", _ptr(xs), _ptr(ty), _ptr(t2m), int(t2m.numel()), _ptr(il), _ptr(nn),
+                  _ptr(fn), _ptr(ng), int(ng.numel()), inum, nall, ntypes, m, float(rcut), _ptr(imagetype), _ptr(lst), _ptr(dR),
+                  _ptr(counts), _ptr(min_rsq), _ptr(status), _stream())
+    rmin = float("inf")
+    if check and inum:
+        st, mr = int(status.item()), int(min_rsq.item())
+        if st > 0:
+            raise OverflowError(f"generate_neighdata: atom {st - 1} has more than m={m} neighbors of one type")
+        if st < 0:
+            raise ValueError({-1: "generate_neighdata: ilist names an atom outside [0, inum)",
+                              -2: "generate_neighdata: a neighbor index is outside [0, nall) or a list range "
+                                  "outside neigh",
+                              -3: "generate_neighdata: an atom type has no model type"}[st])
+        rmin = float(np.array([mr], dtype=np.int64).view(np.float64)[0])
+        if rmin < TOO_CLOSE_RSQ:
+            raise RuntimeError(f"there are two atoms too close, min_dR_all = {rmin}")
+    return LmpNeighData(imagetype, lst, dR, counts, rmin)
 
 
 def image_dr(lst: torch.Tensor, dR: torch.Tensor) -> torch.Tensor:
