@@ -179,6 +179,80 @@ def nep_train_leg(steps=3):
     return out
 
 
+def dp_train_leg(steps=5):
+    """DP training step on 256 Li10GeP2S12-stoichiometry frames (N_f uniform in [64, 512], 2.0 g/cm^3,
+    73.7k atoms in all), rc 6 A, m = 150 per type (S = 600 slots), nets as the MD workload: one step =
+    env mat + embedding and fitting nets + dE/dRi (create_graph) + forces + the backward of an energy +
+    force loss to every net parameter + an Adam update. Batched path (matpl_amd.dp_train.DPTrain: all
+    frames packed, compact buckets, FP32 descriptors, BF16 or FP32 products) against the plain FP64
+    path (DPTorch over all 600 slots, frames in chunks of 32 with gradients accumulated)."""
+    import torch
+    from matpl_amd import structures as st
+    from matpl_amd.dp import DPTorch
+    from matpl_amd.dp_train import DPTrain, pack_batch, plain_grads, train_grads
+    from matpl_amd.dpnet import DPConfig, DPWeights
+    from matpl_amd.neighbor import calc_stat
+    dev = torch.device("cuda:0")
+    frames = st.lgps_frames(nframes=256, nmin=64, nmax=512, seed=1002)
+    cfg = DPConfig(ntypes=4, m=150)
+    batch = pack_batch(frames, cfg, dev, seed=7, keep_full=True)
+    davg, dstd = calc_stat(batch.list_full[0], batch.image_dr_full[0, ..., 1:], batch.type_map, 4, cfg.rcut_smooth,
+                           cfg.rcut)
+    w = DPWeights.random(cfg, seed=1002, davg=davg.cpu().numpy(), dstd=dstd.cpu().numpy())
+    N = batch.natoms
+    out = {"workload": f"DP training step, 256 Li-Ge-P-S frames ({N} atoms), rc 6 A, m=150 per type (S=600), "
+                       "E+F loss backward to all net parameters + Adam", "natoms": N, "mc": batch.mc}
+    grads = {}
+
+    def timed(name, fn, nsteps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            r = fn()
+        torch.cuda.synchronize()
+        dt_s = time.perf_counter() - t0
+        out[name] = {"value": round(N * nsteps / dt_s, 1), "unit": "atom-steps/s",
+                     "ms_per_step": round(dt_s * 1e3 / nsteps, 2)}
+        return r
+
+    for prec in ("bf16", "fp32"):
+        model = DPTrain(w, prec, dev)
+        params = [q for q in model.parameters() if q.requires_grad]
+        opt = torch.optim.Adam(params, lr=1e-4)
+
+        def step(model=model, params=params, opt=opt):
+            loss, g = train_grads(model, batch)
+            for q, gq in zip(params, g):
+                q.grad = gq
+            opt.step()
+            return g
+        # gradients at the initial weights, for the comparison below
+        grads[prec] = [None if x is None else x.detach().clone() for x in train_grads(model, batch)[1]]
+        timed(prec, step, steps)
+    ref = DPTorch(w, torch.float64, dev)
+    rparams = [q for q in ref.parameters() if q.requires_grad]
+    ropt = torch.optim.Adam(rparams, lr=1e-4)
+    grads["fp64"] = [None if x is None else x.detach().clone() for x in plain_grads(ref, batch, chunk_frames=32)[1]]
+
+    def rstep():
+        loss, g = plain_grads(ref, batch, chunk_frames=32)
+        for q, gq in zip(rparams, g):
+            q.grad = gq
+        ropt.step()
+    timed("fp64_plain", rstep, 1)
+    for prec in ("bf16", "fp32"):
+        worst = 0.0
+        for a, b in zip(grads[prec], grads["fp64"]):
+            if a is not None and b is not None and b.norm() > 0:
+                worst = max(worst, float((a.double() - b).norm() / b.norm()))
+        out[f"grad_rel_err_{prec}_vs_fp64"] = float(f"{worst:.3g}")
+        out[f"speedup_{prec}_vs_fp64_plain"] = round(out[prec]["value"] / out["fp64_plain"]["value"], 1)
+    log(f"DP train: bf16 {out['bf16']['ms_per_step']} ms, fp32 {out['fp32']['ms_per_step']} ms, "
+        f"fp64 plain {out['fp64_plain']['ms_per_step']} ms per step (mc={batch.mc})")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +265,7 @@ def main():
     ap.add_argument("--reps", default=None, help="override cells, e.g. 8,8,8")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a captured HIP graph")
     ap.add_argument("--no-nep", action="store_true", help="skip the NEP legs (SiC 1,728 atoms; 128-frame training)")
+    ap.add_argument("--no-train", action="store_true", help="skip the DP training leg (256 LGPS frames)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MiB",
                     help="after the run, copy a buffer of this size 3 times (a known-traffic kernel for "
                          "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -307,6 +382,7 @@ def main():
 
     nep = None if args.no_nep else nep_leg()
     nep_train = None if args.no_nep else nep_train_leg()
+    dp_train = None if args.no_train else dp_train_leg()
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -324,6 +400,7 @@ def main():
         "cpu_baseline": cpu,
         "nep": nep,
         "nep_train": nep_train,
+        "dp_train": dp_train,
     }
     print(json.dumps(out), flush=True)
 

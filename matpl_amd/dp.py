@@ -36,10 +36,12 @@ class _Net(torch.nn.Module):
         self.r = torch.nn.ParameterList([torch.nn.Parameter(_t(r if r is not None else [0.0], dtype, device))
                                          for _, _, r in layers])
 
-    def forward(self, x):
+    def forward(self, x, mm=None, add=None):
+        """mm(x, W): the product to use (default: the plain matmul); add(y, b): the bias add."""
         nl = len(self.W)
         for l in range(nl):
-            z = x @ self.W[l] + self.b[l]
+            y = x @ self.W[l] if mm is None else mm(x, self.W[l])
+            z = y + self.b[l] if add is None else add(y, self.b[l])
             if self.kind == 1 and l == nl - 1:
                 return z
             h = torch.tanh(z)

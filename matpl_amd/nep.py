@@ -25,6 +25,7 @@ import numpy as np
 import torch
 
 from . import calcops, nep_ops
+from .tallmm import _SPLIT_ROWS, _TallMM, _tall_tn  # noqa: F401
 from .nepnet import NEPWeights
 
 
@@ -106,44 +107,6 @@ def ri_d(Ri):
     eye = torch.eye(3, dtype=Ri.dtype, device=Ri.device)
     out[:, :, 1:4, :] = eye * m.unsqueeze(-1).to(Ri.dtype)
     return out
-
-
-_SPLIT_ROWS = 512
-
-
-def _tall_tn(x, g):
-    """x^T g for tall x [K, a] and g [K, b] with K >> a, b: the reduction over K is split into
-    row blocks of a batched product and summed (one GEMM tile looping over all K is serial).
-    Built from differentiable ops, so its own backward is two tall-by-small products."""
-    K = x.shape[0]
-    if K <= 2 * _SPLIT_ROWS:
-        return x.t() @ g
-    nblk = -(-K // _SPLIT_ROWS)
-    pad = nblk * _SPLIT_ROWS - K
-    if pad:
-        x = torch.nn.functional.pad(x, (0, 0, 0, pad))
-        g = torch.nn.functional.pad(g, (0, 0, 0, pad))
-    xs = x.reshape(nblk, _SPLIT_ROWS, x.shape[1]).transpose(1, 2)
-    gs = g.reshape(nblk, _SPLIT_ROWS, g.shape[1])
-    return torch.bmm(xs, gs).sum(0)
-
-
-class _TallMM(torch.autograd.Function):
-    """x @ W for tall x [K, a], small W [a, b]; the weight gradient goes through _tall_tn and the
-    input gradient through _TallMM again, so every product of any derivative order is either
-    tall-by-small or split over K."""
-
-    @staticmethod
-    def forward(ctx, x, W):
-        ctx.save_for_backward(x, W)
-        return x @ W
-
-    @staticmethod
-    def backward(ctx, g):
-        x, W = ctx.saved_tensors
-        gx = _TallMM.apply(g, W.t()) if ctx.needs_input_grad[0] else None
-        gW = _tall_tn(x, g) if ctx.needs_input_grad[1] else None
-        return gx, gW
 
 
 class _FitNet(torch.nn.Module):

@@ -1,0 +1,116 @@
+"""Products of a tall matrix (rows = atoms or neighbor slots) with a small weight matrix, for
+nets trained through autograd.
+
+hipBLAS picks a single-tile kernel for x^T g when x and g are tall and thin
+(the weight gradient of a layer over all atoms / slots): one tile loops over
+all K rows serially. ``_TallMM`` replaces x @ W so that the weight gradient,
+at every derivative order, goes through ``_tall_tn``, which splits K into row
+blocks of a batched product and sums them.
+"""
+from __future__ import annotations
+
+import torch
+
+_SPLIT_ROWS = 512
+
+
+def _tall_tn(x, g):
+    """x^T g for tall x [K, a] and g [K, b] with K >> a, b: the reduction over K is split into
+    row blocks of a batched product and summed (one GEMM tile looping over all K is serial).
+    Built from differentiable ops, so its own backward is two tall-by-small products."""
+    K = x.shape[0]
+    if K <= 2 * _SPLIT_ROWS:
+        return x.t() @ g
+    nblk = K // _SPLIT_ROWS
+    Kf = nblk * _SPLIT_ROWS
+    xs = x[:Kf].reshape(nblk, _SPLIT_ROWS, x.shape[1]).transpose(1, 2)
+    gs = g[:Kf].reshape(nblk, _SPLIT_ROWS, g.shape[1])
+    out = torch.bmm(xs, gs).sum(0)
+    if Kf < K:
+        out = out + x[Kf:].t() @ g[Kf:]
+    return out
+
+
+class _TallMM(torch.autograd.Function):
+    """x @ W for tall x [K, a], small W [a, b]; the weight gradient goes through _tall_tn and the
+    input gradient through _TallMM again, so every product of any derivative order is either
+    tall-by-small or split over K."""
+
+    @staticmethod
+    def forward(ctx, x, W):
+        ctx.save_for_backward(x, W)
+        return x @ W
+
+    @staticmethod
+    def backward(ctx, g):
+        x, W = ctx.saved_tensors
+        gx = _TallMM.apply(g, W.t()) if ctx.needs_input_grad[0] else None
+        gW = _tall_tn(x, g) if ctx.needs_input_grad[1] else None
+        return gx, gW
+
+
+def tall_mm(x, W):
+    """x @ W for x [..., a] (any leading shape) and W [a, b]."""
+    lead = x.shape[:-1]
+    return _TallMM.apply(x.reshape(-1, x.shape[-1]), W).reshape(*lead, W.shape[-1])
+
+
+class _TallMMLow(torch.autograd.Function):
+    """x @ W with both operands rounded to ``dt`` (bfloat16) for the product on the matrix cores
+    (FP32 accumulation, result rounded to ``dt``, returned as FP32); the input gradient is the same
+    low-precision product, the weight gradient an FP32 split product."""
+
+    @staticmethod
+    def forward(ctx, x, W, dt):
+        ctx.save_for_backward(x, W)
+        ctx.dt = dt
+        return (x.to(dt) @ W.to(dt)).float()
+
+    @staticmethod
+    def backward(ctx, g):
+        x, W = ctx.saved_tensors
+        gx = _TallMMLow.apply(g, W.t(), ctx.dt) if ctx.needs_input_grad[0] else None
+        gW = _tall_tn(x.float(), g.float()) if ctx.needs_input_grad[1] else None
+        return gx, gW, None
+
+
+def tall_mm_bf16(x, W):
+    """x @ W, BF16 operands and product, FP32 accumulation; x [..., a], W [a, b]."""
+    lead = x.shape[:-1]
+    return _TallMMLow.apply(x.reshape(-1, x.shape[-1]), W, torch.bfloat16).reshape(*lead, W.shape[-1])
+
+
+
+_SUM_ROWS = 256
+
+
+def colsum(g):
+    """Sum over all leading dimensions of g [..., b] -> [b], in two stages over row blocks (torch's
+    single reduction over a tall dimension into a few outputs runs on a handful of threads)."""
+    g2 = g.reshape(-1, g.shape[-1])
+    K = g2.shape[0]
+    if K <= 4 * _SUM_ROWS:
+        return g2.sum(0)
+    nblk = K // _SUM_ROWS
+    Kf = nblk * _SUM_ROWS
+    out = g2[:Kf].reshape(nblk, _SUM_ROWS, g2.shape[1]).sum(1).sum(0)
+    if Kf < K:
+        out = out + g2[Kf:].sum(0)
+    return out
+
+
+class _BiasAdd(torch.autograd.Function):
+    """y + b (b broadcast over the rows of y); the bias gradient is colsum(g), itself built from
+    differentiable ops."""
+
+    @staticmethod
+    def forward(ctx, y, b):
+        return y + b
+
+    @staticmethod
+    def backward(ctx, g):
+        return g, colsum(g)
+
+
+def add_bias(y, b):
+    return _BiasAdd.apply(y, b)

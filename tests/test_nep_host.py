@@ -1,7 +1,6 @@
 """Host-side NEP packing (matpl_amd.nep.pack_frames / expand_box) against the oracle's C restatement of
 expand_box, bit for bit, including skewed and small cells that need several replicas."""
 import numpy as np
-import torch
 
 import oracle.nep as on
 from matpl_amd import nep as mn
@@ -21,24 +20,3 @@ def test_pack_frames_matches_oracle():
         for k in want:
             assert np.array_equal(got[k].numpy().reshape(-1), np.asarray(want[k]).reshape(-1)), (k, rc)
 
-
-def test_tall_products_match_plain_autograd():
-    """The fitting layers' split-K product: values, gradients and second derivatives equal the plain
-    x @ W ones (row counts below, at and above the split threshold, with padding)."""
-    from matpl_amd.nep import _SPLIT_ROWS, _TallMM
-    torch.manual_seed(0)
-    for K in (100, 2 * _SPLIT_ROWS + 1, 3 * _SPLIT_ROWS + 7):
-        x = torch.randn(K, 7, dtype=torch.float64, requires_grad=True)
-        W = torch.randn(7, 5, dtype=torch.float64, requires_grad=True)
-        outs = []
-        for f in (lambda a, b: _TallMM.apply(a, b), lambda a, b: a @ b):
-            xx, WW = x.detach().clone().requires_grad_(), W.detach().clone().requires_grad_()
-            y = torch.tanh(f(xx, WW))
-            (gx,) = torch.autograd.grad(y.pow(2).sum(), xx, create_graph=True)
-            gW = torch.autograd.grad((gx ** 2).sum(), WW)[0]
-            outs.append((y.detach(), gx.detach(), gW))
-        for a, b in zip(*outs):
-            assert torch.allclose(a, b, rtol=1e-12, atol=1e-10)
-    xs = torch.randn(2 * _SPLIT_ROWS + 9, 3, dtype=torch.float64, requires_grad=True)
-    Ws = torch.randn(3, 2, dtype=torch.float64, requires_grad=True)
-    assert torch.autograd.gradgradcheck(lambda a, b: _TallMM.apply(a, b), (xs, Ws))
