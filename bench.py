@@ -264,6 +264,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", default=None, help="override cells, e.g. 8,8,8")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a captured HIP graph")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
     ap.add_argument("--no-nep", action="store_true", help="skip the NEP legs (SiC 1,728 atoms; 128-frame training)")
     ap.add_argument("--no-train", action="store_true", help="skip the DP training leg (256 LGPS frames)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MiB",
@@ -287,6 +288,11 @@ def main():
 
     md = DPMD(frame, w, vel, precision=args.precision, skin=0.5, mnl=160, dt=1.0, rebuild_every=10, device=dev)
     md.check()
+    launch = None
+    if not args.no_tune:
+        from matpl_amd import autotune
+        key = autotune.workload_key(w.cfg.emb_sizes[0], w.cfg.ntypes, md.mnl, w.cfg.m, N, args.precision)
+        launch = autotune.tune_fused(md.compute, key, log=log)
     for _ in range(args.warmup):
         md.step()
     while md.nstep % md.rebuild_every:
@@ -397,6 +403,7 @@ def main():
         "speedup_vs_fp64_dropin": round(value / fp64["value"], 2) if fp64 else None,
         "force_rmse_vs_fp64_eV_per_A": force_rmse,
         "roofline": roofline,
+        "launch": launch,
         "cpu_baseline": cpu,
         "nep": nep,
         "nep_train": nep_train,

@@ -210,6 +210,17 @@ int matpl_dp_fused_backward(const double *pos, const double *lattice, const int3
                             double rcut, double rcut_smooth, double scale, const float *dA, const int32_t *rev,
                             float *fslot, float *fcenter, float *force, double *virial, int32_t *status,
                             void *stream);
+/*
+ * Launch shape of the fused forward (kernel 0) or backward (kernel 1) kernel
+ * for later calls in this process: the persistent grid's block cap
+ * (default 4096), the weights staged in LDS (1), read from global memory (0)
+ * or LDS when they fit (-1, default), and the XCD-aware block order (1) or
+ * plain order (0, default). Results do not depend on the shape except through
+ * the virial's FP64 summation order. No counterpart in the reference: it is
+ * the knob the launch autotuner (matpl_amd/autotune.py) turns.
+ */
+int matpl_dp_fused_set_launch(int kernel, int max_blocks, int lds_weights, int xcd);
+
 /* force[i] = fcenter[i] + sum of fslot over the used entries of atom i's list
  * (counts [N][T]), in a fixed order (deterministic, no atomics). No reference
  * counterpart (the reference scatters with atomics, calculateForce.cu:92). */

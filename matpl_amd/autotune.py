@@ -1,0 +1,123 @@
+"""Launch-shape autotuner for the fused DP kernels.
+
+The fused forward and backward kernels (csrc/dp_fused.hip) run a persistent
+grid, one atom per wavefront, four wavefronts per block. Three launch choices
+change their speed but not their results:
+  * the grid's block cap (how many blocks stride over the atoms);
+  * whether the weights of every type pair are staged in LDS per block or read
+    through the caches from global memory (LDS costs occupancy as T grows);
+  * the XCD-aware block order, which gives each of the eight XCDs (and its L2)
+    one contiguous range of atoms instead of every eighth block.
+``tune_fused`` times each candidate on the caller's own workload (coordinate
+descent: the forward kernel's shape, then the backward's), keeps the fastest,
+sets it in the library (matpl_dp_fused_set_launch, process-wide) and caches it
+in a JSON file keyed by device, net width, types, list width, atom count
+(rounded to a power of two) and precision. A later call with the same key
+reuses the cached shape without timing.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from typing import Callable, Dict, Optional
+
+import torch
+
+from . import _lib
+
+BLOCK_CAPS = (512, 768, 1024, 1536, 2048, 3072, 4096, 8192, 16384)
+DEFAULT = {"max_blocks": 4096, "lds_weights": -1, "xcd": 0}
+
+
+def set_launch(kernel: int, max_blocks: int, lds_weights: int, xcd: int) -> None:
+    _lib.call("matpl_dp_fused_set_launch", kernel, max_blocks, lds_weights, xcd)
+
+
+def reset() -> None:
+    for k in (0, 1):
+        set_launch(k, **DEFAULT)
+
+
+def cache_path() -> str:
+    return os.environ.get("MATPL_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "matpl_amd", "fused_launch.json"))
+
+
+def workload_key(E: int, T: int, mnl: int, m: int, natoms: int, precision: str) -> str:
+    dev = torch.cuda.get_device_name(torch.cuda.current_device()) if torch.cuda.is_available() else "cpu"
+    nb = 1 << max(0, int(round(math.log2(max(natoms, 1)))))
+    return f"{dev}|E{E}|T{T}|mnl{mnl}|m{m}|n~{nb}|{precision}"
+
+
+def _load() -> Dict:
+    try:
+        with open(cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(key: str, val: Dict) -> None:
+    data = _load()
+    data[key] = val
+    path = cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + f".{os.getpid()}"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass          # an unwritable cache only costs a re-tune next time
+
+
+def _time(run: Callable[[], None], reps: int) -> float:
+    run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune_fused(run: Callable[[], None], key: str, reps: int = 10, use_cache: bool = True,
+               log: Optional[Callable[[str], None]] = None) -> Dict:
+    """Chooses and sets the launch shape of both fused kernels for the workload ``run`` enqueues
+    (one whole evaluation on the current stream). Returns {"forward": cfg, "backward": cfg,
+    "ms": time per evaluation with the chosen shapes, "ms_default": with the defaults, "cached": bool}."""
+    if use_cache:
+        hit = _load().get(key)
+        if hit:
+            for k, name in ((0, "forward"), (1, "backward")):
+                set_launch(k, **hit[name])
+            return dict(hit, cached=True)
+    reset()
+    ms_default = _time(run, reps)
+    best = {0: dict(DEFAULT), 1: dict(DEFAULT)}
+    best_ms = ms_default
+    for k in (0, 1):
+        for lw in (-1, 0):
+            for xcd in (0, 1):
+                for cap in BLOCK_CAPS:
+                    cand = {"max_blocks": cap, "lds_weights": lw, "xcd": xcd}
+                    if cand == best[k]:
+                        continue
+                    set_launch(k, **cand)
+                    try:
+                        ms = _time(run, reps)
+                    except _lib.MatplError:
+                        continue
+                    if ms < best_ms:
+                        best_ms, best[k] = ms, cand
+        set_launch(k, **best[k])
+    # one more measurement of the chosen pair, so the reported time is not a lucky sample
+    best_ms = min(best_ms, _time(run, reps))
+    res = {"forward": best[0], "backward": best[1], "ms": round(best_ms, 4), "ms_default": round(ms_default, 4)}
+    if log:
+        log(f"autotune {key}: {res}")
+    _store(key, res)
+    return dict(res, cached=False)

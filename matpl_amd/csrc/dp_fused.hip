@@ -99,7 +99,28 @@ struct FusedParams {
     double rc2;                 // model cutoff^2 (FP64 test)
     float rmin, rmax, scale;    // scale = 1 / (m * type_nums)
     float iw;                   // 1 / (rmax - rmin)
+    int xcd;                    // 1: XCD-aware block order (grid a multiple of NXCD)
 };
+
+// Launch shape of each fused kernel (0 = forward, 1 = backward), process-wide,
+// set by matpl_dp_fused_set_launch (the autotuner): the persistent grid's block
+// cap, where the weights live (-1 auto: LDS when they fit, 0 global, 1 LDS) and
+// the XCD-aware block order.
+struct LaunchCfg {
+    int max_blocks;
+    int lds_weights;
+    int xcd;
+};
+LaunchCfg g_launch[2] = {{4096, -1, 0}, {4096, -1, 0}};
+constexpr int NXCD = 8;
+
+// Blocks are dispatched to the XCDs round-robin (block b on XCD b % 8). With
+// xcd set, block b takes logical position (b % 8) * (G / 8) + b / 8, so that
+// each XCD, and its L2, works through one contiguous range of atoms.
+__device__ __forceinline__ int64_t logical_block(int xcd) {
+    const unsigned b = blockIdx.x, G = gridDim.x;
+    return xcd ? (int64_t)(b % NXCD) * (G / NXCD) + b / NXCD : (int64_t)b;
+}
 
 // In BF16 mode the forward weights and biases of the embedding net come
 // pre-multiplied by TANH_K = 2 log2(e) (matpl_amd/dp_fused.py), so a layer's
@@ -441,7 +462,7 @@ dp_fused_fwd_kernel(FusedParams p) {
     int *nvs = L.nvs;
     const int h = lane >> 5, n = lane & 31;
 
-    for (int64_t i = (int64_t)blockIdx.x * FUSED_WAVES + wid; i < p.natoms; i += (int64_t)gridDim.x * FUSED_WAVES) {
+    for (int64_t i = logical_block(p.xcd) * FUSED_WAVES + wid; i < p.natoms; i += (int64_t)gridDim.x * FUSED_WAVES) {
         const int ti = p.type_map[i];
         const float *tinfo = p.tinfo + ti * NTI;
         const AtomGeo g = atom_geo(p, i);
@@ -519,7 +540,7 @@ dp_fused_bwd_kernel(FusedParams p) {
     if (lane < 9) vacc[lane] = 0.0;
     int vframe = -1;
 
-    for (int64_t i = (int64_t)blockIdx.x * FUSED_WAVES + wid; i < p.natoms; i += (int64_t)gridDim.x * FUSED_WAVES) {
+    for (int64_t i = logical_block(p.xcd) * FUSED_WAVES + wid; i < p.natoms; i += (int64_t)gridDim.x * FUSED_WAVES) {
         const int ti = p.type_map[i];
         const float *tinfo = p.tinfo + ti * NTI;
         const AtomGeo g = atom_geo(p, i);
@@ -710,8 +731,14 @@ int launch_lw(FusedParams &p, bool fwd, hipStream_t st) {
     const size_t lds = fwd ? fused_lds_bytes<BF16, LW, false>(p.T, p.cap) : fused_lds_bytes<BF16, LW, true>(p.T, p.cap);
     MATPL_CHECK_ARG(lds <= 64 * 1024, "dp_fused: T*mnl too large for LDS (%zu bytes)", lds);
     // persistent-style grid: a few blocks per CU, each strides over atoms
+    const LaunchCfg &cfg = g_launch[fwd ? 0 : 1];
     int64_t blocks = (p.natoms + FUSED_WAVES - 1) / FUSED_WAVES;
-    if (blocks > 4096) blocks = 4096;
+    if (blocks > cfg.max_blocks) blocks = cfg.max_blocks;
+    p.xcd = 0;
+    if (cfg.xcd && blocks >= NXCD) {
+        blocks -= blocks % NXCD;
+        p.xcd = 1;
+    }
     if (blocks < 1) blocks = 1;
     if (fwd) {
         hipLaunchKernelGGL((dp_fused_fwd_kernel<E, BF16, LW>), dim3((unsigned)blocks), dim3(FUSED_WAVES * WAVE), lds,
@@ -727,7 +754,10 @@ int launch_lw(FusedParams &p, bool fwd, hipStream_t st) {
 // (2 blocks = 8 waves per CU); otherwise they are read from global memory.
 template <int E, bool BF16>
 int launch(FusedParams &p, bool fwd, hipStream_t st) {
-    if (fused_lds_bytes<BF16, true, true>(p.T, p.cap) <= 64 * 1024) return launch_lw<E, BF16, true>(p, fwd, st);
+    const int mode = g_launch[fwd ? 0 : 1].lds_weights;
+    const bool fits = fused_lds_bytes<BF16, true, true>(p.T, p.cap) <= 64 * 1024;
+    MATPL_CHECK_ARG(mode != 1 || fits, "dp_fused: weights do not fit in LDS for this T and list width");
+    if (mode == 1 || (mode < 0 && fits)) return launch_lw<E, BF16, true>(p, fwd, st);
     return launch_lw<E, BF16, false>(p, fwd, st);
 }
 
@@ -802,3 +832,13 @@ MATPL_EXPORT int matpl_dp_fused_gather(const int32_t *counts, const float *fslot
                        reinterpret_cast<const float4 *>(fslot), fcenter, force, ntypes, mnl, natoms);
     return check_launch("gather_kernel");
 }
+
+MATPL_EXPORT int matpl_dp_fused_set_launch(int kernel, int max_blocks, int lds_weights, int xcd) {
+    MATPL_CHECK_ARG(kernel == 0 || kernel == 1, "dp_fused_set_launch: kernel is 0 (forward) or 1 (backward)");
+    MATPL_CHECK_ARG(max_blocks >= 1 && max_blocks <= (1 << 20), "dp_fused_set_launch: max_blocks out of range");
+    MATPL_CHECK_ARG(lds_weights >= -1 && lds_weights <= 1 && (xcd == 0 || xcd == 1),
+                    "dp_fused_set_launch: lds_weights in {-1, 0, 1}, xcd in {0, 1}");
+    g_launch[kernel] = LaunchCfg{max_blocks, lds_weights, xcd};
+    return OK;
+}
+

@@ -1,0 +1,70 @@
+"""Launch shapes of the fused DP kernels (matpl_dp_fused_set_launch) change speed, not results: every
+shape the autotuner may pick gives the same energies and forces bit for bit (gather mode sums in a
+fixed order) and the virial to FP64 summation order; the autotuner picks one, sets it and caches it."""
+import numpy as np
+import pytest
+import torch
+
+from matpl_amd import autotune
+from matpl_amd import structures as st
+from matpl_amd.dp_fused import FusedDP
+from matpl_amd.dpnet import DPConfig, DPWeights
+from matpl_amd.neighbor import find_neighbore_gpu, reverse_map
+
+pytestmark = pytest.mark.gpu
+
+
+def _workload(dev, precision="bf16"):
+    f = st.fluorite_hfo2((3, 3, 3), seed=21)
+    cfg = DPConfig(ntypes=2, m=120)
+    w = DPWeights.random(cfg, seed=5, davg=np.tile([0.1, 0, 0, 0], (2, 1)), dstd=np.tile([0.2, 0.1, 0.1, 0.1], (2, 1)))
+    nl = find_neighbore_gpu(f.lattice, f.frac(), f.types, 2, cfg.m, cfg.rcut, device=dev, want_dR=False,
+                            want_shift=True)
+    L = torch.as_tensor(f.lattice, device=dev)
+    pos = (nl.frac_wrapped @ L).contiguous()
+    tm = torch.as_tensor(f.types, device=dev)
+    rev, rst = reverse_map(nl, tm, cfg.m)
+    assert int(rst.item()) == 0
+    model = FusedDP(w, precision, dev)
+    ranges = [(int(np.searchsorted(f.types, t, "left")), int(np.searchsorted(f.types, t, "right"))) for t in range(2)]
+
+    def run():
+        return model(pos, L.reshape(1, 9).contiguous(), tm, nl.list, nl.shift, nl.counts, cfg.m,
+                     type_ranges=ranges, rev=rev)
+    return f, cfg, run
+
+
+def test_results_independent_of_launch_shape(dev):
+    f, cfg, run = _workload(dev)
+    print(f"case launch shapes: N={f.natoms}", flush=True)
+    try:
+        autotune.reset()
+        Ei0, F0, V0, st0 = [x.clone() for x in run()]
+        for k in (0, 1):
+            for cand in ({"max_blocks": 8, "lds_weights": 0, "xcd": 1}, {"max_blocks": 3, "lds_weights": -1, "xcd": 1},
+                         {"max_blocks": 64, "lds_weights": 1, "xcd": 0}, {"max_blocks": 16384, "lds_weights": 0,
+                                                                          "xcd": 1}):
+                autotune.set_launch(k, **cand)
+                Ei, F, V, status = run()
+                torch.cuda.synchronize()
+                assert int(status.item()) == 0
+                assert torch.equal(Ei, Ei0) and torch.equal(F, F0), (k, cand)
+                assert torch.allclose(V, V0, rtol=1e-12, atol=1e-12 * float(V0.abs().max())), (k, cand)
+            autotune.set_launch(k, **autotune.DEFAULT)
+    finally:
+        autotune.reset()
+
+
+def test_autotuner_sets_and_caches(dev, tmp_path, monkeypatch):
+    monkeypatch.setenv("MATPL_TUNE_CACHE", str(tmp_path / "tune.json"))
+    f, cfg, run = _workload(dev)
+    print("case autotune", flush=True)
+    try:
+        key = autotune.workload_key(25, 2, cfg.m, cfg.m, f.natoms, "bf16")
+        r1 = autotune.tune_fused(run, key, reps=2)
+        assert not r1["cached"] and r1["ms"] <= r1["ms_default"] * 1.5
+        assert (tmp_path / "tune.json").exists()
+        r2 = autotune.tune_fused(run, key, reps=2)
+        assert r2["cached"] and r2["forward"] == r1["forward"] and r2["backward"] == r1["backward"]
+    finally:
+        autotune.reset()
