@@ -114,8 +114,8 @@ def tune_fused(run: Callable[[], None], key: str, reps: int = 10, use_cache: boo
                     if ms < best_ms:
                         best_ms, best[k] = ms, cand
         set_launch(k, **best[k])
-    # one more measurement of the chosen pair, so the reported time is not a lucky sample
-    best_ms = min(best_ms, _time(run, reps))
+    # a fresh measurement of the chosen pair (the search's own minimum is biased low)
+    best_ms = _time(run, reps)
     res = {"forward": best[0], "backward": best[1], "ms": round(best_ms, 4), "ms_default": round(ms_default, 4)}
     if log:
         log(f"autotune {key}: {res}")
