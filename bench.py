@@ -33,8 +33,9 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 # HBM bytes per launch of dp_fused_bwd_kernel on the 19x19x23 workload: rocprofv3 FETCH_SIZE + WRITE_SIZE,
 # each scaled by a copy kernel of known size (profiles/dp_md_hfo2_99636_hbm_traffic.md; re-measure after
-# kernel changes). Counters cannot be read from inside the timed run.
-BWD_TRAFFIC_BYTES = 587.1e6
+# kernel changes). Counters cannot be read from inside the timed run. The XCD-aware block order (the
+# autotuner's choice here) keeps partner-slot writes within one XCD's L2; the plain order writes more.
+BWD_TRAFFIC_BYTES = {"xcd": 458.6e6, "plain": 587.1e6}
 
 
 def log(*a):
@@ -230,7 +231,7 @@ def dp_train_leg(steps=5):
         # gradients at the initial weights, for the comparison below
         grads[prec] = [None if x is None else x.detach().clone() for x in train_grads(model, batch)[1]]
         timed(prec, step, steps)
-    ref = DPTorch(w, torch.float64, dev)
+    ref = DPTorch(w, torch.float64, dev, tall=True)
     rparams = [q for q in ref.parameters() if q.requires_grad]
     ropt = torch.optim.Adam(rparams, lr=1e-4)
     grads["fp64"] = [None if x is None else x.detach().clone() for x in plain_grads(ref, batch, chunk_frames=32)[1]]
@@ -265,6 +266,9 @@ def main():
     ap.add_argument("--reps", default=None, help="override cells, e.g. 8,8,8")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a captured HIP graph")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuned ones")
+    ap.add_argument("--launch", default=None, metavar="FB,FL,FX,BB,BL,BX",
+                    help="fixed launch shapes of the fused forward/backward kernels (blocks, LDS weights -1/0/1, "
+                         "XCD order 0/1), no tuning")
     ap.add_argument("--no-nep", action="store_true", help="skip the NEP legs (SiC 1,728 atoms; 128-frame training)")
     ap.add_argument("--no-train", action="store_true", help="skip the DP training leg (256 LGPS frames)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MiB",
@@ -289,8 +293,10 @@ def main():
     md = DPMD(frame, w, vel, precision=args.precision, skin=0.5, mnl=160, dt=1.0, rebuild_every=10, device=dev)
     md.check()
     launch = None
-    if not args.no_tune:
-        from matpl_amd import autotune
+    from matpl_amd import autotune
+    if args.launch:
+        launch = autotune.parse_launch(args.launch)
+    elif not args.no_tune:
         key = autotune.workload_key(w.cfg.emb_sizes[0], w.cfg.ntypes, md.mnl, w.cfg.m, N, args.precision)
         launch = autotune.tune_fused(md.compute, key, log=log)
     for _ in range(args.warmup):
@@ -342,14 +348,17 @@ def main():
     nvalid = int(nl_rc.counts.sum().item())      # in-cutoff neighbors at the initial configuration
     del nl_rc
     roofline = None
+    traffic = None
+    if reps == REPS and args.precision == "bf16":
+        traffic = BWD_TRAFFIC_BYTES["xcd" if launch and launch["backward"]["xcd"] else "plain"]
     if bwd_ms and nvalid:
         kt = float(np.mean(bwd_ms)) * 1e-3
         flops = flops_per_neighbor_bwd(w.cfg.emb_out) * nvalid
         achieved = flops / kt / 1e12
         roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(achieved / BF16_PEAK_TFLOPS, 5),
-                    "traffic": BWD_TRAFFIC_BYTES if reps == REPS and args.precision == "bf16" else None,
-                    "traffic_GBps": round(BWD_TRAFFIC_BYTES / kt / 1e9, 1) if reps == REPS else None,
+                    "traffic": traffic,
+                    "traffic_GBps": round(traffic / kt / 1e9, 1) if traffic else None,
                     "kernel": "dp_fused_bwd_kernel", "avg_ms": round(kt * 1e3, 4),
                     "in_cutoff_neighbors": nvalid}
 

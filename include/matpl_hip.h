@@ -173,6 +173,21 @@ int matpl_dp_calc_stat(const int32_t *list, const double *dR, const int32_t *typ
                        int ntypes, double rmin, double rmax, double *davg, double *dstd, double *workspace,
                        int64_t workspace_elems, void *stream);
 
+/* ------------------------------------------- batched small products */
+/*
+ * Batched products of small matrices (float32 or float64, contiguous
+ * operands), one per batch element n:
+ *   mode 0: out[n][a][b] = sum_k P1[n][k][a] P2[n][k][b]   (P1^T P2)
+ *   mode 1: out[n][K][a] = sum_j P1[n][k][j] P2[n][a][j]   (P1 P2^T, P1 [K][b], P2 [a][b])
+ *   mode 2: out[n][K][b] = sum_i P1[n][k][i] P2[n][i][b]   (P1 P2,   P1 [K][a], P2 [a][b])
+ * The two operands of one element must fit in 64 KB. No counterpart in the
+ * reference: it replaces the batched torch.matmul calls of the per-atom
+ * descriptor contraction and of DR (src/model/dp_dp.py:332, 369, 412) and
+ * of their derivatives in the batched training step.
+ */
+int matpl_bsmm(int f64, int mode, const void *P1, const void *P2, void *out, int64_t n, int K, int a, int b,
+               void *stream);
+
 /* ------------------------------------------- fused DP descriptor + nets */
 /*
  * Forward: for every atom, the in-cutoff neighbors of its neighbor list

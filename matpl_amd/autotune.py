@@ -44,8 +44,29 @@ def cache_path() -> str:
                           os.path.join(os.path.expanduser("~"), ".cache", "matpl_amd", "fused_launch.json"))
 
 
+def device_id() -> str:
+    """GPU architecture and CU count (the marketing name can read empty under a profiler)."""
+    if not torch.cuda.is_available():
+        return "cpu"
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    arch = getattr(p, "gcnArchName", "") or p.name or "gpu"
+    return f"{arch.split(':')[0]}x{p.multi_processor_count}"
+
+
+def parse_launch(spec: str) -> Dict:
+    """'fb,fl,fx,bb,bl,bx' (blocks, LDS weights, XCD order of the forward, then the backward kernel)."""
+    v = [int(x) for x in spec.split(",")]
+    if len(v) != 6:
+        raise ValueError("launch spec needs 6 integers: fwd blocks,lds,xcd,bwd blocks,lds,xcd")
+    f = {"max_blocks": v[0], "lds_weights": v[1], "xcd": v[2]}
+    b = {"max_blocks": v[3], "lds_weights": v[4], "xcd": v[5]}
+    set_launch(0, **f)
+    set_launch(1, **b)
+    return {"forward": f, "backward": b, "fixed": True}
+
+
 def workload_key(E: int, T: int, mnl: int, m: int, natoms: int, precision: str) -> str:
-    dev = torch.cuda.get_device_name(torch.cuda.current_device()) if torch.cuda.is_available() else "cpu"
+    dev = device_id()
     nb = 1 << max(0, int(round(math.log2(max(natoms, 1)))))
     return f"{dev}|E{E}|T{T}|mnl{mnl}|m{m}|n~{nb}|{precision}"
 

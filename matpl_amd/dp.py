@@ -18,6 +18,7 @@ from . import calcops
 from .compress import calculateCompress
 from .dpnet import DPWeights
 from .neighbor import env_mat
+from .tallmm import add_bias, tall_mm
 
 
 def _t(a, dtype, device):
@@ -60,14 +61,17 @@ class _Net(torch.nn.Module):
 
 
 class DPTorch(torch.nn.Module):
-    def __init__(self, w: DPWeights, dtype=torch.float64, device="cuda", compress=None):
+    def __init__(self, w: DPWeights, dtype=torch.float64, device="cuda", compress=None, tall: bool = False):
         """compress: a matpl_amd.compress.CompressTable; the embedding nets are then evaluated from
-        the table with CalcOps_cuda.calculateCompress (dp_dp.py:350-355, 572-601)."""
+        the table with CalcOps_cuda.calculateCompress (dp_dp.py:350-355, 572-601).
+        tall: layer products and bias adds through matpl_amd.tallmm, whose weight and bias
+        gradients are split over rows (for training: hipBLAS runs x^T g over all slots in one tile)."""
         super().__init__()
         cfg = w.cfg
         self.cfg = cfg
         self.dtype = dtype
         self.compress = compress
+        self.mm, self.add = (tall_mm, add_bias) if tall else (None, None)
         T = cfg.ntypes
         self.emb = torch.nn.ModuleList([_Net(w.emb[p], 0, dtype, device) for p in range(T * T)])
         self.fit = torch.nn.ModuleList([_Net(w.fit[t], 1, dtype, device) for t in range(T)])
@@ -97,7 +101,7 @@ class DPTorch(torch.nn.Module):
             for tj in present:
                 Rt = Ri[:, idx, tj * m:(tj + 1) * m]              # [B, n, m, 4]
                 if self.compress is None:
-                    G = self.emb[ti * T + tj](Rt[..., 0:1])        # [B, n, m, E]
+                    G = self.emb[ti * T + tj](Rt[..., 0:1], self.mm, self.add)   # [B, n, m, E]
                 else:
                     f2, coef = self.compress.lookup(Rt[..., 0], ti * T + tj)
                     G = calculateCompress(f2, coef)[0].reshape(Rt.shape[:-1] + (-1,))
@@ -105,7 +109,7 @@ class DPTorch(torch.nn.Module):
                 A = part if A is None else A + part
             A = A / (m * len(present))
             DR = A.transpose(-2, -1) @ A[..., :cfg.M2]              # [B, n, E, M2]
-            Ei_t = self.fit[ti](DR.reshape(B, idx.numel(), -1))     # [B, n, 1]
+            Ei_t = self.fit[ti](DR.reshape(B, idx.numel(), -1), self.mm, self.add)   # [B, n, 1]
             Ei = Ei.index_copy(1, idx, Ei_t)
         Etot = Ei.sum(1)
         dE = torch.autograd.grad([Etot], [Ri], grad_outputs=[torch.ones_like(Etot)], retain_graph=True,

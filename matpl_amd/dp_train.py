@@ -17,6 +17,8 @@ differentiates through all of it. ``DPTorch`` in FP64 is that path here (the pla
   * descriptors Ri, Ri_d in FP32 from the HIP env-mat kernel;
   * embedding and fitting products in BF16 on the matrix cores with FP32 accumulation and FP32
     activations ("bf16"), or all in FP32 ("fp32");
+  * the per-atom contractions A = Ri^T G and DR = A^T A' (and all their derivatives) on the
+    batched small-product kernels (matpl_amd.bsmm);
   * the centre force by a HIP reduction; neighbor forces by CalcOps_cuda.calculateForce, whose
     backward with respect to dE is the hand-written force-gradient kernel, the edge through
     which the force loss reaches the nets;
@@ -33,6 +35,7 @@ import numpy as np
 import torch
 
 from . import calcops, nep_ops
+from .bsmm import bmm_tn
 from .dp import DPTorch, _Net
 from .dpnet import DPWeights
 from .neighbor import env_mat, find_neighbore_batch, image_dr
@@ -127,13 +130,13 @@ class DPTrain(torch.nn.Module):
                 net = self.emb[ti * T + tj]
                 Rt = Ri[0, idx, tj * mc:(tj + 1) * mc]                      # [n, mc, 4]
                 G = net(Rt[..., 0:1], self.mm, add_bias)                             # [n, mc, E]
-                part = Rt.transpose(-2, -1) @ G                              # [n, 4, E]
+                part = bmm_tn(Rt, G)                                         # [n, 4, E]
                 if m > mc:
                     gpad = net(self.ri_pad[ti, 0].view(1, 1), self.mm)        # [1, E]
                     part = part + (m - mc) * (self.ri_pad[ti].view(4, 1) * gpad)
                 A = part if A is None else A + part
             A = A / (m * len(present))
-            DR = A.transpose(-2, -1) @ A[..., :cfg.M2]
+            DR = bmm_tn(A, A[..., :cfg.M2])
             Ei = Ei.index_copy(0, idx, self.fit[ti](DR.reshape(idx.numel(), -1), self.mm, add_bias).reshape(-1))
         Etot = torch.zeros(batch.nframes, dtype=Ei.dtype, device=Ei.device).index_add(0, batch.frame, Ei)
         dE = torch.autograd.grad([Etot.sum()], [Ri], create_graph=create_graph)[0]

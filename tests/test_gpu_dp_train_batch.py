@@ -35,6 +35,11 @@ def test_packed_plain_path_equals_per_frame(dev):
     print(f"case packed vs per-frame: N={batch.natoms} mc={batch.mc}", flush=True)
     model = DPTorch(w, torch.float64, dev)
     l_all, g_all = plain_grads(model, batch)
+    l_tall, g_tall = plain_grads(DPTorch(w, torch.float64, dev, tall=True), batch)
+    assert abs(l_all.item() - l_tall.item()) < 1e-12 * abs(l_all.item())
+    for a, b in zip(g_all, g_tall):
+        if a is not None:
+            assert _rel(a, b) < 1e-10
     l_one, g_one = plain_grads(model, batch, chunk_frames=1)
     assert abs(l_all.item() - l_one.item()) < 1e-12 * abs(l_all.item())
     for a, b in zip(g_all, g_one):
@@ -47,7 +52,7 @@ def test_batched_step_matches_plain_fp64(dev, precision, tol):
     frames, cfg, batch, w = _setup(dev)
     print(f"case {precision}: N={batch.natoms} mc={batch.mc} (m={cfg.m})", flush=True)
     assert batch.mc < cfg.m          # the closed-form empty-slot term is exercised
-    ref_loss, ref_g = plain_grads(DPTorch(w, torch.float64, dev), batch)
+    ref_loss, ref_g = plain_grads(DPTorch(w, torch.float64, dev, tall=True), batch)
     loss, g = train_grads(DPTrain(w, precision, dev), batch)
     assert abs(loss.item() - ref_loss.item()) < tol * abs(ref_loss.item())
     worst = 0.0
