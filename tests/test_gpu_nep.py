@@ -243,6 +243,36 @@ def test_model_energy_force_virial(dev, lmax):
     assert _rel(V.detach().cpu(), want["V"]) < 1e-10
 
 
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-10), (torch.float32, 1e-5)])
+def test_trained_hfo2_model(dev, dtype, tol):
+    """The trained HfO2 NEP model the reference ships (tests/golden/hfo2_nep; nep4, O/Hf, l_max [4,2,1],
+    100 hidden units, rc 6/6 A) on four of its DFT-labelled training frames: the HIP path equals the
+    FP64 oracle, and reproduces the DFT forces."""
+    import os
+    from matpl_amd.nep_io import read_nep_txt
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hfo2_nep")
+    w, el, mx = read_nep_txt(os.path.join(here, "nep.txt"))
+    d = np.load(os.path.join(here, "frames.npz"))
+    types = np.array([{8: el.index("O"), 72: el.index("Hf")}[int(z)] for z in d["image_type"]], np.int32)
+    frames = []
+    for k in range(d["lattice"].shape[0]):
+        L = d["lattice"][k].reshape(3, 3)
+        frames.append(st.Frame(L, d["position"][k].reshape(-1, 3) @ L, types, el))
+    print(f"case trained HfO2 NEP {dtype}: {len(frames)} frames x {frames[0].natoms} atoms", flush=True)
+    want = on.nep_evaluate(frames, w)
+    cfg = w.cfg
+    gp = mn.pack_frames(frames, cfg.rcut_radial, dev)
+    nb = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes)
+    model = mn.NEPTorch(w, dtype, dev)
+    Etot, Ei, F, V = model(nb["NN_r"], nb["NL_r"], nb["Ri_r"], nb["NN_a"], nb["NL_a"], nb["Ri_a"], gp["num_atoms"],
+                           gp["atom_type_map"], create_graph=False)
+    assert _rel(Etot.detach().cpu().double().reshape(-1), want["E"]) < tol
+    assert _rel(F.detach().cpu().double(), want["F"]) < 100 * tol
+    assert _rel(V.detach().cpu().double(), want["V"]) < 100 * tol
+    Fp = F.detach().cpu().double().numpy().reshape(len(frames), -1)
+    assert float(np.sqrt(((Fp - d["forces"]) ** 2).mean())) < 0.1
+
+
 def test_model_float32(dev):
     frames = _frames()
     cfg = NEPConfig(ntypes=4, l_max=[4, 2, 0])
