@@ -78,6 +78,13 @@ class DPTorch(torch.nn.Module):
         self.register_buffer("davg_full", torch.as_tensor(w.davg_full(), dtype=dtype, device=device))
         self.register_buffer("dstd_full", torch.as_tensor(w.dstd_full(), dtype=dtype, device=device))
 
+    def embed(self, ti: int, tj: int, Rt: torch.Tensor) -> torch.Tensor:
+        """G of the (ti, tj) embedding net at the slots' normalized s = Rt[..., 0]."""
+        if self.compress is None:
+            return self.emb[ti * self.cfg.ntypes + tj](Rt[..., 0:1], self.mm, self.add)
+        f2, coef = self.compress.lookup(Rt[..., 0], ti * self.cfg.ntypes + tj)
+        return calculateCompress(f2, coef)[0].reshape(Rt.shape[:-1] + (-1,))
+
     def forward(self, list_neigh: torch.Tensor, Imagetype_map: torch.Tensor, ImageDR: torch.Tensor,
                 nghost: int = 0, type_present: Optional[List[int]] = None, create_graph: bool = True):
         """list_neigh [B,N,S] 1-based; Imagetype_map [N]; ImageDR [B,N,S,4].
@@ -100,11 +107,7 @@ class DPTorch(torch.nn.Module):
             A = None
             for tj in present:
                 Rt = Ri[:, idx, tj * m:(tj + 1) * m]              # [B, n, m, 4]
-                if self.compress is None:
-                    G = self.emb[ti * T + tj](Rt[..., 0:1], self.mm, self.add)   # [B, n, m, E]
-                else:
-                    f2, coef = self.compress.lookup(Rt[..., 0], ti * T + tj)
-                    G = calculateCompress(f2, coef)[0].reshape(Rt.shape[:-1] + (-1,))
+                G = self.embed(ti, tj, Rt)                                # [B, n, m, E]
                 part = Rt.transpose(-2, -1) @ G                     # [B, n, 4, E]
                 A = part if A is None else A + part
             A = A / (m * len(present))
