@@ -406,7 +406,9 @@ struct Gate16<false> {
 
 // LDS layout of a fused block: [staged weights (LW only)][neighbor tiles of
 // each wave: T*mnl float4][per-wave bucket counts: MAXT ints][bwd only: per
-// wave dA of the current atom [32 features][4] floats, virial 9 doubles].
+// wave dA of the current atom [32 features][4] floats, virial 9 doubles]
+// [BF16 fwd only: per wave the tile's Ri^T [4][32] and G^T [32][32] in bf16].
+constexpr int XT_ELEMS = (4 + 32) * 32;
 struct BlockLds {
     const float *vec;
     const unsigned char *mat;
@@ -414,6 +416,7 @@ struct BlockLds {
     int *nvs;
     float4 *dAs;
     double *vacc;
+    __bf16 *xt;
 };
 
 template <bool BF16, bool LW, bool BWD>
@@ -439,6 +442,8 @@ __device__ __forceinline__ BlockLds block_lds(const FusedParams &p, float4 *smem
     float4 *dAall = reinterpret_cast<float4 *>(nvall + FUSED_WAVES * MAXT);
     b.dAs = dAall + wid * 32;
     b.vacc = reinterpret_cast<double *>(dAall + FUSED_WAVES * 32) + wid * 9;
+    // forward: the transposed tile sits where the backward keeps dA (16-byte aligned)
+    b.xt = reinterpret_cast<__bf16 *>(dAall) + (size_t)wid * XT_ELEMS;
     (void)BWD;
     return b;
 }
@@ -448,6 +453,7 @@ inline size_t fused_lds_bytes(int T, int cap) {
     size_t b = (LW ? weight_lds_bytes<BF16>(T) : 0) + sizeof(float4) * FUSED_WAVES * (size_t)T * cap +
                sizeof(int) * FUSED_WAVES * MAXT;
     if (BWD) b += sizeof(float4) * FUSED_WAVES * 32 + sizeof(double) * FUSED_WAVES * 9;
+    if (!BWD && BF16) b += sizeof(__bf16) * FUSED_WAVES * XT_ELEMS;
     return b;
 }
 
@@ -468,11 +474,21 @@ dp_fused_fwd_kernel(FusedParams p) {
         const AtomGeo g = atom_geo(p, i);
         if (!compact(p, g, i, lane, nb, nvs, nullptr) && lane == 0) atomicCAS(p.status, 0, (int)(i + 1));
         __builtin_amdgcn_wave_barrier();
+        // BF16: A = sum over tiles of Ri^T G on the matrix cores. Each tile stages Ri^T [4][32]
+        // and G^T [32][32] (bf16) in the wave's LDS; two 32x32x16 MFMAs (neighbors 0-15, 16-31)
+        // accumulate D[w][e] with the neighbors as the reduction index, so lane e of half 0
+        // ends the atom holding A[0..3][e] in D[0..3]. FP32: per-lane FMAs and wave sums.
+        floatx16 D;
         float Aacc[4][16];
+        if constexpr (BF16) {
 #pragma unroll
-        for (int w = 0; w < 4; w++)
+            for (int r = 0; r < 16; r++) D[r] = 0.0f;
+        } else {
 #pragma unroll
-            for (int r = 0; r < 16; r++) Aacc[w][r] = 0.0f;
+            for (int w = 0; w < 4; w++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) Aacc[w][r] = 0.0f;
+        }
         for (int t = 0; t < p.T; t++) {
             const int nv = nvs[t];
             const int pair = ti * p.T + t;
@@ -487,39 +503,75 @@ dp_fused_fwd_kernel(FusedParams p) {
                 float h1[16], t2[16], t3[16], G[16];
                 embed_fwd<E, BF16, LW>(L.vec, L.mat, tp, lane, Ri[0], h1, t2, t3, G);
                 const float live = (k < nv) ? 1.0f : 0.0f;
+                if constexpr (BF16) {
+                    __bf16 *rit = L.xt, *gt = L.xt + 4 * 32;
+                    if (h == 0) {
 #pragma unroll
-                for (int w = 0; w < 4; w++)
+                        for (int w = 0; w < 4; w++) rit[w * 32 + n] = (__bf16)(Ri[w] * live);
+                    }
 #pragma unroll
-                    for (int rr = 0; rr < NR; rr++) Aacc[w][rr] += (Ri[w] * live) * G[rr];
+                    for (int rr = 0; rr < NR; rr++) {
+                        const int f = feat(rr, h);
+                        if (f < E) gt[f * 32 + n] = (__bf16)G[rr];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    // rows w >= 4 and columns e >= E of D are never read: their lanes read row 0
+                    const int ia = n < 4 ? n : 0, jb = n < E ? n : 0;
+                    const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(rit + ia * 32 + 8 * h);
+                    const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(rit + ia * 32 + 16 + 8 * h);
+                    const bf16x8 b0 = *reinterpret_cast<const bf16x8 *>(gt + jb * 32 + 8 * h);
+                    const bf16x8 b1 = *reinterpret_cast<const bf16x8 *>(gt + jb * 32 + 16 + 8 * h);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, D, 0, 0, 0);
+                    D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, D, 0, 0, 0);
+                    __builtin_amdgcn_wave_barrier();
+                } else {
+#pragma unroll
+                    for (int w = 0; w < 4; w++)
+#pragma unroll
+                        for (int rr = 0; rr < NR; rr++) Aacc[w][rr] += (Ri[w] * live) * G[rr];
+                }
             }
         }
-        // reduce over the neighbors (lanes of each half)
-#pragma unroll
-        for (int w = 0; w < 4; w++)
-#pragma unroll
-            for (int rr = 0; rr < NR; rr++) Aacc[w][rr] = half_sum(Aacc[w][rr]);
-        // lane n of half h writes register n's feature (n < NR)
         float out[4] = {0.f, 0.f, 0.f, 0.f};
+        int f = -1, rr_f = 0, h_f = 0;        // feature this lane writes, and its (register, half)
+        if constexpr (BF16) {
+            if (h == 0 && n < E) {
+                f = n;
+                rr_f = 4 * (n >> 3) + (n & 3);
+                h_f = (n >> 2) & 1;
 #pragma unroll
-        for (int rr = 0; rr < NR; rr++)
-            if (n == rr) {
-#pragma unroll
-                for (int w = 0; w < 4; w++) out[w] = Aacc[w][rr];
+                for (int w = 0; w < 4; w++) out[w] = D[w];
             }
-        if (n < NR) {
-            const int f = feat(n, h);
-            if (f < E) {
-                float pad = 0.0f;
-                for (int t = 0; t < p.T; t++) {
-                    const float npad = (float)(p.m - nvs[t]);
-                    pad += npad * vec_ptr(L.vec, ti * p.T + t, V_GPAD, h)[n];
+        } else {
+            // reduce over the neighbors (lanes of each half)
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+#pragma unroll
+                for (int rr = 0; rr < NR; rr++) Aacc[w][rr] = half_sum(Aacc[w][rr]);
+            // lane n of half h writes register n's feature (n < NR)
+#pragma unroll
+            for (int rr = 0; rr < NR; rr++)
+                if (n == rr) {
+#pragma unroll
+                    for (int w = 0; w < 4; w++) out[w] = Aacc[w][rr];
                 }
-#pragma unroll
-                for (int w = 0; w < 4; w++) out[w] += tinfo[TI_SPAD + w] * pad;
-                float *Ao = p.A + (size_t)i * 4 * E;
-#pragma unroll
-                for (int w = 0; w < 4; w++) Ao[w * E + f] = out[w] * p.scale;
+            if (n < NR && feat(n, h) < E) {
+                f = feat(n, h);
+                rr_f = n;
+                h_f = h;
             }
+        }
+        if (f >= 0) {
+            float pad = 0.0f;
+            for (int t = 0; t < p.T; t++) {
+                const float npad = (float)(p.m - nvs[t]);
+                pad += npad * vec_ptr(L.vec, ti * p.T + t, V_GPAD, h_f)[rr_f];
+            }
+#pragma unroll
+            for (int w = 0; w < 4; w++) out[w] += tinfo[TI_SPAD + w] * pad;
+            float *Ao = p.A + (size_t)i * 4 * E;
+#pragma unroll
+            for (int w = 0; w < 4; w++) Ao[w * E + f] = out[w] * p.scale;
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -755,7 +807,8 @@ int launch_lw(FusedParams &p, bool fwd, hipStream_t st) {
 template <int E, bool BF16>
 int launch(FusedParams &p, bool fwd, hipStream_t st) {
     const int mode = g_launch[fwd ? 0 : 1].lds_weights;
-    const bool fits = fused_lds_bytes<BF16, true, true>(p.T, p.cap) <= 64 * 1024;
+    const bool fits = (fwd ? fused_lds_bytes<BF16, true, false>(p.T, p.cap)
+                           : fused_lds_bytes<BF16, true, true>(p.T, p.cap)) <= 64 * 1024;
     MATPL_CHECK_ARG(mode != 1 || fits, "dp_fused: weights do not fit in LDS for this T and list width");
     if (mode == 1 || (mode < 0 && fits)) return launch_lw<E, BF16, true>(p, fwd, st);
     return launch_lw<E, BF16, false>(p, fwd, st);
