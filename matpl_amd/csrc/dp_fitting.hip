@@ -83,6 +83,19 @@ struct Gates16<false> {
     __device__ __forceinline__ float get(int r) const { return v[r]; }
 };
 
+// FP16 pairs (BF16 mode): a gate is (1 - t^2) or rdt (1 - t^2), and multiplies a gradient that is
+// rounded to a bf16 MFMA operand right after, so fp16's 11-bit significand adds error eight times
+// below that rounding. Halving the gate registers brings the kernel under 256 VGPRs (two waves per
+// SIMD instead of one), which hides the weight-load latency of the dependent layer chain.
+struct GatesH {
+    typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+    half2_t v[8];
+    __device__ __forceinline__ void set(int r, float g0, float g1) {
+        v[r >> 1] = half2_t{(_Float16)g0, (_Float16)g1};
+    }
+    __device__ __forceinline__ float get(int r) const { return (float)v[r >> 1][r & 1]; }
+};
+
 // Value of lane l ^ 32 (same lane of the other half-wave), via v_permlane32_swap.
 __device__ __forceinline__ float other_half(float v) {
     const unsigned u = __float_as_uint(v);
@@ -126,25 +139,26 @@ __device__ __forceinline__ const float *vptr(const float *vec, int layer, int wh
 }
 
 template <bool BF16, int L>
-__global__ void __launch_bounds__(WAVE)
+__global__ void __launch_bounds__(WAVE, 2)
 dp_fitting_kernel(FitParams p) {
     const int lane = threadIdx.x;
     const int n = lane & 31, h = lane >> 5;
-    // A and the dA accumulation of the tile's 32 atoms; one row of 4E values per
-    // atom, row stride 4E + 1 (odd) so that lanes reading the same element of
-    // different atoms hit different banks
+    // A of the tile's 32 atoms, one row of 4E values per atom, row stride 4E + 1
+    // (odd) so that lanes reading the same element of different atoms hit
+    // different banks. The layer-0 backward overwrites column p of A with dA
+    // once the DR block that reads column p is done (blocks go in increasing p),
+    // so dA needs no buffer of its own: 12.9 KB of LDS per wave for E = 25.
     extern __shared__ float fit_smem[];
     const int E = p.E, RS = 4 * E + 1;
-    float *As = fit_smem, *dAs = fit_smem + 32 * RS;
+    float *As = fit_smem, *dAs = fit_smem;
 #define AS(a, w, c) As[(a) * RS + (w) * E + (c)]
 #define DAS(a, w, c) dAs[(a) * RS + (w) * E + (c)]
     for (int tile = blockIdx.x; tile < p.ntiles; tile += gridDim.x) {
         const int t = p.tiles[3 * tile], a0 = p.tiles[3 * tile + 1], cnt = p.tiles[3 * tile + 2];
-        // stage A and clear dA
+        // stage A
         // (atom, offset) advance by 64 elements per step without a division (4E >= 64)
         for (int x = lane, an = 0, wc = lane; x < 32 * 4 * E; x += WAVE) {
             As[an * RS + wc] = an < cnt ? p.A[(int64_t)a0 * 4 * E + x] : 0.0f;
-            dAs[an * RS + wc] = 0.0f;
             wc += WAVE;
             if (wc >= 4 * E) { wc -= 4 * E; an++; }
         }
@@ -231,7 +245,8 @@ dp_fitting_kernel(FitParams p) {
                                &drv);
             }
         }
-        Gates16<false> gk[L][2];    // gates per layer: (1 - t^2) for layer 0, rdt (1 - t^2) after
+        // gates per layer: (1 - t^2) for layer 0, rdt (1 - t^2) after
+        typename std::conditional<BF16, GatesH, Gates16<false>>::type gk[L][2];
         float x[2][16];
 #pragma unroll
         for (int ob = 0; ob < 2; ob++)
@@ -337,11 +352,11 @@ dp_fitting_kernel(FitParams p) {
             }
             if (h == 0 && p0 < E) {
 #pragma unroll
-                for (int w = 0; w < 4; w++) DAS(n, w, p0) += s0[w];
+                for (int w = 0; w < 4; w++) DAS(n, w, p0) = s0[w];
             }
             if (h == 1 && p1 < E) {
 #pragma unroll
-                for (int w = 0; w < 4; w++) DAS(n, w, p1) += s1[w];
+                for (int w = 0; w < 4; w++) DAS(n, w, p1) = s1[w];
             }
         };
         if constexpr (BF16) {
@@ -435,7 +450,7 @@ MATPL_EXPORT int matpl_dp_fitting(const float *A, const int32_t *tiles, int ntil
     p.type_stride_wh = type_stride_wh; p.type_stride_vec = type_stride_vec;
     p.ntiles = ntiles; p.E = E; p.D = E * M2; p.Dh = (E * M2) / 2; p.nb0 = (E * M2 + 31) / 32; p.L = nlayers;
     const int grid = ntiles < 8192 ? ntiles : 8192;
-    const size_t lds = sizeof(float) * 2 * 32 * (4 * E + 1);
+    const size_t lds = sizeof(float) * 32 * (4 * E + 1);
     hipStream_t st = (hipStream_t)stream;
 #define MATPL_FIT_CASE(LL)                                                                              \
     case LL:                                                                                            \
