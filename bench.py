@@ -254,6 +254,56 @@ def dp_train_leg(steps=5):
     return out
 
 
+def compress_table(w, dev, order=3, dx=0.01, s_max_raw=1.0):
+    """The reference's DP compress defaults (ckpt_compress.py:24-26: dx 0.01, order 3); s tabulated up to
+    the normalized s of a 1.0 A bond (the shortest HfO2 bond is above 1.9 A), then 10x coarser."""
+    import torch
+    from matpl_amd.compress import build_table
+    from matpl_amd.dp import DPTorch
+    plain = DPTorch(w, torch.float64, dev)
+    ct = build_table(list(plain.emb), w.davg, w.dstd, s_max_raw, dx=dx, order=order, device=dev)
+    del plain
+    return ct
+
+
+def compressed_md_leg(frame, w, vel, dev, ct, precision, steps, warmup, tune=True):
+    """The same MD workload with the embedding evaluated from the compression table (the reference's
+    DP compress) inside the fused kernels: G and dG/ds by Horner instead of the three tanh layers."""
+    import torch
+    from matpl_amd import autotune
+    from matpl_amd.md import DPMD
+    N = frame.natoms
+    md = DPMD(frame, w, vel, precision=precision, skin=0.5, mnl=160, dt=1.0, rebuild_every=10, device=dev,
+              compress=ct)
+    md.check()
+    launch = None
+    if tune:
+        key = autotune.workload_key(w.cfg.emb_sizes[0], w.cfg.ntypes, md.mnl, w.cfg.m, N,
+                                    f"{precision}-compress{ct.order}")
+        launch = autotune.tune_fused(md.compute, key, log=log)
+    for _ in range(warmup):
+        md.step()
+    while md.nstep % md.rebuild_every:
+        md.step()
+    torch.cuda.synchronize()
+    md.check()
+    md.capture()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        done += md.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    md.check()
+    ms = (t1 - t0) * 1e3 / done
+    log(f"fused {precision} compressed (order {ct.order}, dx {ct.dx}): {ms:.3f} ms/step")
+    del md
+    torch.cuda.empty_cache()
+    return {"value": round(N * done / (t1 - t0), 1), "ms_per_step": round(ms, 4), "steps": done,
+            "order": ct.order, "dx": ct.dx, "table_rows": ct.n_fine + ct.n_coarse, "launch": launch}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,6 +321,9 @@ def main():
                          "XCD order 0/1), no tuning")
     ap.add_argument("--no-nep", action="store_true", help="skip the NEP legs (SiC 1,728 atoms; 128-frame training)")
     ap.add_argument("--no-train", action="store_true", help="skip the DP training leg (256 LGPS frames)")
+    ap.add_argument("--compress", action="store_true",
+                    help="also time the MD workload with the compressed embedding (the reference's DP compress, "
+                         "tables read by the fused kernels)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MiB",
                     help="after the run, copy a buffer of this size 3 times (a known-traffic kernel for "
                          "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -362,6 +415,12 @@ def main():
                     "kernel": "dp_fused_bwd_kernel", "avg_ms": round(kt * 1e3, 4),
                     "in_cutoff_neighbors": nvalid}
 
+    compressed, ct = None, None
+    if args.compress:
+        ct = compress_table(w, dev)
+        compressed = compressed_md_leg(frame, w, vel, dev, ct, args.precision, args.steps, args.warmup,
+                                       tune=not (args.no_tune or args.launch))
+
     # FP64 drop-in baseline on the same GPU, and the accuracy gate on this workload
     fp64 = None
     force_rmse = None
@@ -373,6 +432,13 @@ def main():
         e_rel = abs(chk.energy() - float(ref.E.sum().item())) / abs(float(ref.E.sum().item()))
         log(f"force RMSE vs FP64 = {force_rmse:.3e} eV/A, |dE|/E = {e_rel:.2e}")
         del chk
+        if compressed is not None:
+            chk = DPMD(frame, w, vel, precision=args.precision, skin=0.5, mnl=160, device=dev, compress=ct)
+            compressed["force_rmse_vs_fp64_eV_per_A"] = float(torch.sqrt(((chk.F.double() - F64) ** 2).mean()).item())
+            compressed["energy_rel_err_vs_fp64"] = abs(chk.energy() - float(ref.E.sum().item())) / abs(
+                float(ref.E.sum().item()))
+            log(f"compressed: force RMSE vs FP64 = {compressed['force_rmse_vs_fp64_eV_per_A']:.3e} eV/A")
+            del chk
         ref.step()
         torch.cuda.synchronize()
         ta = time.perf_counter()
@@ -382,6 +448,8 @@ def main():
         tb = time.perf_counter()
         fp64_ms = (tb - ta) * 1e3 / args.fp64_steps
         fp64 = {"value": N * args.fp64_steps / (tb - ta), "ms_per_step": round(fp64_ms, 3), "steps": args.fp64_steps}
+        if compressed is not None:
+            compressed["speedup_vs_fp64_dropin"] = round(compressed["value"] / fp64["value"], 2)
         log(f"fp64 drop-in: {fp64_ms:.2f} ms/step")
         del ref
         torch.cuda.empty_cache()
@@ -417,6 +485,7 @@ def main():
         "nep": nep,
         "nep_train": nep_train,
         "dp_train": dp_train,
+        "compressed_md": compressed,
     }
     print(json.dumps(out), flush=True)
 

@@ -226,6 +226,34 @@ int matpl_dp_fused_backward(const double *pos, const double *lattice, const int3
                             float *fslot, float *fcenter, float *force, double *virial, int32_t *status,
                             void *stream);
 /*
+ * The fused forward / backward with a compressed embedding: G(s) and dG/ds
+ * from piecewise polynomials instead of the net (the reference's DP compress,
+ * dp_dp.py:350-355, 572-601 with the table of src/user/ckpt_compress.py).
+ * ctab [T*T][rows][8][order+1][4] float (16-byte aligned): for pair p, row k
+ * and feature f = 4q + c, the coefficients (highest power first) of the
+ * interval's polynomial in x = s - knot_k at [p][k][q][.][c]; features >= E
+ * hold zeros. Knots: x0 + k dx for k <= n_fine, then steps of 10 dx; s below
+ * x0 or past the last row extrapolates the first or last interval.
+ * order 3 or 5. mat is not read; vec supplies the empty-slot vector
+ * G(s_pad), which the caller fills from the same table. Other arguments as
+ * matpl_dp_fused_forward / matpl_dp_fused_backward.
+ */
+int matpl_dp_fused_forward_compressed(const double *pos, const double *lattice, const int32_t *atom_frame,
+                                      const int64_t *frame_start, const int32_t *list, const int32_t *shift,
+                                      const int32_t *counts, const int32_t *type_map, const float *vec,
+                                      const float *ctab, int order, int rows, int n_fine, double x0, double dx,
+                                      const float *tinfo, int64_t natoms, int ntypes, int mnl, int m, int E, int bf16,
+                                      double rcut, double rcut_smooth, double scale, float *A, int32_t *status,
+                                      void *stream);
+int matpl_dp_fused_backward_compressed(const double *pos, const double *lattice, const int32_t *atom_frame,
+                                       const int64_t *frame_start, const int32_t *list, const int32_t *shift,
+                                       const int32_t *counts, const int32_t *type_map, const float *vec,
+                                       const float *ctab, int order, int rows, int n_fine, double x0, double dx,
+                                       const float *tinfo, int64_t natoms, int ntypes, int mnl, int m, int E,
+                                       int bf16, double rcut, double rcut_smooth, double scale, const float *dA,
+                                       const int32_t *rev, float *fslot, float *fcenter, float *force,
+                                       double *virial, int32_t *status, void *stream);
+/*
  * Launch shape of the fused forward (kernel 0) or backward (kernel 1) kernel
  * for later calls in this process: the persistent grid's block cap
  * (default 4096), the weights staged in LDS (1), read from global memory (0)

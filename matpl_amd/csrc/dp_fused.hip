@@ -100,6 +100,12 @@ struct FusedParams {
     float rmin, rmax, scale;    // scale = 1 / (m * type_nums)
     float iw;                   // 1 / (rmax - rmin)
     int xcd;                    // 1: XCD-aware block order (grid a multiple of NXCD)
+    // compressed embedding (table mode): G(s) by piecewise polynomials, K coefficients per
+    // feature and interval, highest power first, laid out [T*T][rows][8 groups][K][4 features]
+    // (group q holds features 4q..4q+3, so register block r/4 of half h reads group 2(r/4)+h)
+    const float4 *ctab;
+    int crows, cnf;             // table rows; rows of the fine part
+    float cx0, cdx, cidx, cxfe; // first knot, fine spacing, 1/dx, end of the fine part
 };
 
 // Launch shape of each fused kernel (0 = forward, 1 = backward), process-wide,
@@ -264,6 +270,50 @@ __device__ __forceinline__ void embed_fwd(const float *vecb, const unsigned char
         t3[r] = (r < NR) ? act<BF16>(acc[r]) : 0.0f;
         G[r] = h2[r] + r3[r] * t3[r];
     }
+}
+
+// Compressed embedding of one tile: G = P_k(s - x_k) and dG/ds = P_k'(s - x_k) for the
+// interval k holding s (intervals of dx up to the end of the fine part, then 10 dx; s outside the
+// table extrapolates the first or last interval), as the reference's calc_compress_polynomial
+// (dp_dp.py:572-601) with the table of ckpt_compress.py. Value and derivative by one Horner pass.
+template <int E, int CK>
+__device__ __forceinline__ void table_eval(const FusedParams &p, int pair, float s, int h, float G[16],
+                                           float Gd[16]) {
+    constexpr int NR = NReg<E>::value;
+    int row;
+    float knot;
+    if (s < p.cxfe) {
+        row = (int)floorf((s - p.cx0) * p.cidx);
+        row = row < 0 ? 0 : row;
+        knot = p.cx0 + (float)row * p.cdx;
+    } else {
+        row = p.cnf + (int)floorf((s - p.cxfe) * (0.1f * p.cidx));
+        row = row > p.crows - 1 ? p.crows - 1 : row;
+        knot = p.cxfe + (float)(row - p.cnf) * (10.0f * p.cdx);
+    }
+    const float x = s - knot;
+    const float4 *rp = p.ctab + ((size_t)pair * p.crows + row) * (8 * CK);
+#pragma unroll
+    for (int rb = 0; rb < NR; rb += 4) {
+        const float4 *c = rp + (2 * (rb >> 2) + h) * CK;
+        float4 v = c[0], d = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int kk = 1; kk < CK; kk++) {
+            const float4 a = c[kk];
+            d.x = d.x * x + v.x; d.y = d.y * x + v.y; d.z = d.z * x + v.z; d.w = d.w * x + v.w;
+            v.x = v.x * x + a.x; v.y = v.y * x + a.y; v.z = v.z * x + a.z; v.w = v.w * x + a.w;
+        }
+        const float vv[4] = {v.x, v.y, v.z, v.w}, dd[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+        for (int c4 = 0; c4 < 4; c4++) {
+            if (rb + c4 < 16) {
+                G[rb + c4] = rb + c4 < NR ? vv[c4] : 0.0f;
+                Gd[rb + c4] = rb + c4 < NR ? dd[c4] : 0.0f;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = (NR + 3) / 4 * 4; r < 16; r++) { G[r] = 0.0f; Gd[r] = 0.0f; }
 }
 
 struct AtomGeo {
@@ -457,7 +507,7 @@ inline size_t fused_lds_bytes(int T, int cap) {
     return b;
 }
 
-template <int E, bool BF16, bool LW>
+template <int E, bool BF16, bool LW, int CK>
 __global__ void __launch_bounds__(FUSED_WAVES * WAVE)
 dp_fused_fwd_kernel(FusedParams p) {
     constexpr int NR = NReg<E>::value;
@@ -500,8 +550,14 @@ dp_fused_fwd_kernel(FusedParams p) {
                 float Ri[4] = {0.f, 0.f, 0.f, 0.f};
                 float r, s, ds;
                 if (k < nv) ri_of(nb[t * p.cap + k], tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
-                float h1[16], t2[16], t3[16], G[16];
-                embed_fwd<E, BF16, LW>(L.vec, L.mat, tp, lane, Ri[0], h1, t2, t3, G);
+                float G[16];
+                if constexpr (CK > 0) {
+                    float Gd[16];
+                    table_eval<E, CK>(p, tp, Ri[0], h, G, Gd);
+                } else {
+                    float h1[16], t2[16], t3[16];
+                    embed_fwd<E, BF16, LW>(L.vec, L.mat, tp, lane, Ri[0], h1, t2, t3, G);
+                }
                 const float live = (k < nv) ? 1.0f : 0.0f;
                 if constexpr (BF16) {
                     __bf16 *rit = L.xt, *gt = L.xt + 4 * 32;
@@ -577,7 +633,7 @@ dp_fused_fwd_kernel(FusedParams p) {
     }
 }
 
-template <int E, bool BF16, bool LW>
+template <int E, bool BF16, bool LW, int CK>
 __global__ void __launch_bounds__(FUSED_WAVES * WAVE)
 dp_fused_bwd_kernel(FusedParams p) {
     constexpr int NR = NReg<E>::value;
@@ -624,13 +680,30 @@ dp_fused_bwd_kernel(FusedParams p) {
                 // re-read the weights every tile instead of keeping them live across tiles
                 int tp = pair;
                 asm volatile("" : "+v"(tp));
-                const float *w1 = vec_ptr(L.vec, tp, V_W1, h);
-                const float *r2v = vec_ptr(L.vec, tp, V_R2, h), *r3v = vec_ptr(L.vec, tp, V_R3, h);
                 const bool live = k < nv;
                 float4 e = live ? nb[t * p.cap + k] : make_float4(1.f, 0.f, 0.f, 0.f);
                 float Ri[4] = {0.f, 0.f, 0.f, 0.f};
                 float r = 1.f, s = 0.f, ds = 0.f;
                 if (live) ri_of(e, tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
+                float dRi[4] = {0.f, 0.f, 0.f, 0.f};
+                float dsum = 0.0f;
+                if constexpr (CK > 0) {
+                    // table mode: dE/ds = sum_f dG_f G'_f(s)
+                    float G[16], Gd[16];
+                    table_eval<E, CK>(p, tp, Ri[0], h, G, Gd);
+#pragma unroll
+                    for (int rr = 0; rr < NR; rr++) {
+                        const float4 d = dAs[feat(rr, h)];
+                        const float dG = d.x * Ri[0] + d.y * Ri[1] + d.z * Ri[2] + d.w * Ri[3];
+                        dRi[0] += d.x * G[rr];
+                        dRi[1] += d.y * G[rr];
+                        dRi[2] += d.z * G[rr];
+                        dRi[3] += d.w * G[rr];
+                        dsum += dG * Gd[rr];
+                    }
+                } else {
+                const float *w1 = vec_ptr(L.vec, tp, V_W1, h);
+                const float *r2v = vec_ptr(L.vec, tp, V_R2, h), *r3v = vec_ptr(L.vec, tp, V_R3, h);
                 float G[16];
                 Gate16<false> g1, g2, g3;
                 {
@@ -648,7 +721,7 @@ dp_fused_bwd_kernel(FusedParams p) {
                     g1.set(q);
                 }
                 // dG = dA^T Ri, dRi = dA G (partial over this half's features)
-                float dG[16], dRi[4] = {0.f, 0.f, 0.f, 0.f};
+                float dG[16];
 #pragma unroll
                 for (int rr = 0; rr < 16; rr++) {
                     const float4 d = rr < NR ? dAs[feat(rr, h)] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -672,10 +745,10 @@ dp_fused_bwd_kernel(FusedParams p) {
                 for (int rr = 0; rr < 16; rr++) dz[rr] = acc[rr] * g2.get(rr);
                 mfma_layer<BF16, LW>(mat_ptr<BF16>(L.mat, tp, M_W2B), lane, dz, acc);
                 // layer 1: h1 = tanh(s w1 + b1) -> ds = sum_f dh1 (1-h1^2) w1
-                float dsum = 0.0f;
 #pragma unroll
                 for (int rr = 0; rr < NR; rr++) dsum += acc[rr] * g1.get(rr);
                 if constexpr (BF16) dsum *= 1.0f / TANH_K;     // w1 is stored pre-scaled
+                }
                 dRi[0] += dsum;
 #pragma unroll
                 for (int w = 0; w < 4; w++) dRi[w] += other_half(dRi[w]);
@@ -778,7 +851,7 @@ gather_kernel(const int32_t *__restrict__ counts, const float4 *__restrict__ fsl
     }
 }
 
-template <int E, bool BF16, bool LW>
+template <int E, bool BF16, bool LW, int CK = 0>
 int launch_lw(FusedParams &p, bool fwd, hipStream_t st) {
     const size_t lds = fwd ? fused_lds_bytes<BF16, LW, false>(p.T, p.cap) : fused_lds_bytes<BF16, LW, true>(p.T, p.cap);
     MATPL_CHECK_ARG(lds <= 64 * 1024, "dp_fused: T*mnl too large for LDS (%zu bytes)", lds);
@@ -793,19 +866,22 @@ int launch_lw(FusedParams &p, bool fwd, hipStream_t st) {
     }
     if (blocks < 1) blocks = 1;
     if (fwd) {
-        hipLaunchKernelGGL((dp_fused_fwd_kernel<E, BF16, LW>), dim3((unsigned)blocks), dim3(FUSED_WAVES * WAVE), lds,
+        hipLaunchKernelGGL((dp_fused_fwd_kernel<E, BF16, LW, CK>), dim3((unsigned)blocks), dim3(FUSED_WAVES * WAVE), lds,
                            st, p);
         return check_launch("dp_fused_fwd_kernel");
     }
-    hipLaunchKernelGGL((dp_fused_bwd_kernel<E, BF16, LW>), dim3((unsigned)blocks), dim3(FUSED_WAVES * WAVE), lds, st,
+    hipLaunchKernelGGL((dp_fused_bwd_kernel<E, BF16, LW, CK>), dim3((unsigned)blocks), dim3(FUSED_WAVES * WAVE), lds, st,
                        p);
     return check_launch("dp_fused_bwd_kernel");
 }
 
 // Weights go to LDS when they and the neighbor tiles fit in 64 KB per block
 // (2 blocks = 8 waves per CU); otherwise they are read from global memory.
+// Table mode (ck = 4 or 6 coefficients) reads no net weights.
 template <int E, bool BF16>
-int launch(FusedParams &p, bool fwd, hipStream_t st) {
+int launch(FusedParams &p, bool fwd, hipStream_t st, int ck) {
+    if (ck == 4) return launch_lw<E, BF16, false, 4>(p, fwd, st);
+    if (ck == 6) return launch_lw<E, BF16, false, 6>(p, fwd, st);
     const int mode = g_launch[fwd ? 0 : 1].lds_weights;
     const bool fits = (fwd ? fused_lds_bytes<BF16, true, false>(p.T, p.cap)
                            : fused_lds_bytes<BF16, true, true>(p.T, p.cap)) <= 64 * 1024;
@@ -814,16 +890,16 @@ int launch(FusedParams &p, bool fwd, hipStream_t st) {
     return launch_lw<E, BF16, false>(p, fwd, st);
 }
 
-int dispatch(FusedParams &p, int E, int bf16, bool fwd, hipStream_t st) {
+int dispatch(FusedParams &p, int E, int bf16, bool fwd, hipStream_t st, int ck = 0) {
     MATPL_CHECK_ARG(p.T >= 1 && p.T <= MAXT, "dp_fused: ntypes must be in [1,%d]", MAXT);
     MATPL_CHECK_ARG(p.mnl >= 1 && p.m >= 1, "dp_fused: bad bucket sizes");
     p.cap = p.m < p.mnl ? p.m : p.mnl;
     if (p.natoms == 0) return OK;
-    MATPL_CHECK_ARG(p.pos && p.lattice && p.list && p.shift && p.counts && p.type_map && p.vec && p.mat && p.tinfo &&
-                        p.status, "dp_fused: null pointer");
-    if (E == 25) return bf16 ? launch<25, true>(p, fwd, st) : launch<25, false>(p, fwd, st);
-    if (E == 32) return bf16 ? launch<32, true>(p, fwd, st) : launch<32, false>(p, fwd, st);
-    if (E == 16) return bf16 ? launch<16, true>(p, fwd, st) : launch<16, false>(p, fwd, st);
+    MATPL_CHECK_ARG(p.pos && p.lattice && p.list && p.shift && p.counts && p.type_map && p.vec &&
+                        (ck ? p.ctab != nullptr : p.mat != nullptr) && p.tinfo && p.status, "dp_fused: null pointer");
+    if (E == 25) return bf16 ? launch<25, true>(p, fwd, st, ck) : launch<25, false>(p, fwd, st, ck);
+    if (E == 32) return bf16 ? launch<32, true>(p, fwd, st, ck) : launch<32, false>(p, fwd, st, ck);
+    if (E == 16) return bf16 ? launch<16, true>(p, fwd, st, ck) : launch<16, false>(p, fwd, st, ck);
     set_error("dp_fused: embedding width %d not compiled (16, 25, 32)", E);
     return ERR_ARGUMENT;
 }
@@ -874,6 +950,69 @@ MATPL_EXPORT int matpl_dp_fused_backward(const double *pos, const double *lattic
         p.fcenter = fcenter;
     }
     return dispatch(p, E, bf16, false, (hipStream_t)stream);
+}
+
+// Table-mode descriptor shared by the two compressed entry points.
+static int set_table(FusedParams &p, const float *ctab, int order, int rows, int n_fine, double x0, double dx,
+                     int E) {
+    MATPL_CHECK_ARG(order == 3 || order == 5, "dp_fused_compressed: order must be 3 or 5");
+    MATPL_CHECK_ARG(rows >= 1 && n_fine >= 0 && n_fine <= rows && dx > 0.0, "dp_fused_compressed: bad table sizes");
+    MATPL_CHECK_ARG(E <= 32, "dp_fused_compressed: embedding width above 32");
+    MATPL_CHECK_ARG(ctab && aligned16(ctab), "dp_fused_compressed: table must be 16-byte aligned");
+    p.ctab = reinterpret_cast<const float4 *>(ctab);
+    p.crows = rows;
+    p.cnf = n_fine;
+    p.cx0 = (float)x0;
+    p.cdx = (float)dx;
+    p.cidx = (float)(1.0 / dx);
+    p.cxfe = (float)(x0 + n_fine * dx);
+    return OK;
+}
+
+MATPL_EXPORT int matpl_dp_fused_forward_compressed(
+    const double *pos, const double *lattice, const int32_t *atom_frame, const int64_t *frame_start,
+    const int32_t *list, const int32_t *shift, const int32_t *counts, const int32_t *type_map, const float *vec,
+    const float *ctab, int order, int rows, int n_fine, double x0, double dx, const float *tinfo, int64_t natoms,
+    int ntypes, int mnl, int m, int E, int bf16, double rcut, double rcut_smooth, double scale, float *A,
+    int32_t *status, void *stream) {
+    FusedParams p{};
+    int rc = set_table(p, ctab, order, rows, n_fine, x0, dx, E);
+    if (rc) return rc;
+    p.pos = pos; p.lattice = lattice; p.atom_frame = atom_frame; p.frame_start = frame_start;
+    p.list = list; p.shift = shift; p.counts = counts; p.type_map = type_map;
+    p.vec = vec; p.tinfo = tinfo; p.A = A; p.status = status;
+    p.natoms = natoms; p.T = ntypes; p.mnl = mnl; p.m = m;
+    p.rc2 = rcut * rcut; p.rmin = (float)rcut_smooth; p.rmax = (float)rcut; p.scale = (float)scale;
+    MATPL_CHECK_ARG(rcut > rcut_smooth, "dp_fused: rcut must exceed rcut_smooth");
+    p.iw = (float)(1.0 / (rcut - rcut_smooth));
+    MATPL_CHECK_ARG(natoms == 0 || A, "dp_fused_forward: null A");
+    return dispatch(p, E, bf16, true, (hipStream_t)stream, order + 1);
+}
+
+MATPL_EXPORT int matpl_dp_fused_backward_compressed(
+    const double *pos, const double *lattice, const int32_t *atom_frame, const int64_t *frame_start,
+    const int32_t *list, const int32_t *shift, const int32_t *counts, const int32_t *type_map, const float *vec,
+    const float *ctab, int order, int rows, int n_fine, double x0, double dx, const float *tinfo, int64_t natoms,
+    int ntypes, int mnl, int m, int E, int bf16, double rcut, double rcut_smooth, double scale, const float *dA,
+    const int32_t *rev, float *fslot, float *fcenter, float *force, double *virial, int32_t *status, void *stream) {
+    FusedParams p{};
+    int rc = set_table(p, ctab, order, rows, n_fine, x0, dx, E);
+    if (rc) return rc;
+    p.pos = pos; p.lattice = lattice; p.atom_frame = atom_frame; p.frame_start = frame_start;
+    p.list = list; p.shift = shift; p.counts = counts; p.type_map = type_map;
+    p.vec = vec; p.tinfo = tinfo; p.dA = dA; p.force = force; p.virial = virial; p.status = status;
+    p.natoms = natoms; p.T = ntypes; p.mnl = mnl; p.m = m;
+    p.rc2 = rcut * rcut; p.rmin = (float)rcut_smooth; p.rmax = (float)rcut; p.scale = (float)scale;
+    MATPL_CHECK_ARG(rcut > rcut_smooth, "dp_fused: rcut must exceed rcut_smooth");
+    p.iw = (float)(1.0 / (rcut - rcut_smooth));
+    MATPL_CHECK_ARG(natoms == 0 || (dA && force && virial), "dp_fused_backward: null output");
+    if (rev) {
+        MATPL_CHECK_ARG(fslot && fcenter && aligned16(fslot), "dp_fused_backward: gather mode needs fslot/fcenter");
+        p.fslot = reinterpret_cast<float4 *>(fslot);
+        p.rev = rev;
+        p.fcenter = fcenter;
+    }
+    return dispatch(p, E, bf16, false, (hipStream_t)stream, order + 1);
 }
 
 MATPL_EXPORT int matpl_dp_fused_gather(const int32_t *counts, const float *fslot, const float *fcenter, float *force,

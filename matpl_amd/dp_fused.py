@@ -115,6 +115,40 @@ def pack_fused(w: DPWeights, precision: str = "bf16", device="cuda") -> FusedWei
                         torch.as_tensor(tinfo, device=device), E, bf16)
 
 
+@dataclass
+class FusedTable:
+    """A compressed embedding (matpl_amd.compress.CompressTable) in the fused kernels' layout."""
+    tab: torch.Tensor      # [T*T, rows, 8, K, 4] float32: pair, interval, feature group, coefficient, feature
+    order: int
+    rows: int
+    n_fine: int
+    x0: float
+    dx: float
+
+
+def table_value_np(ct, pair: int, s: float) -> np.ndarray:
+    """G(s) of one pair from a CompressTable, in float64 (the lookup rule of CompressTable.lookup)."""
+    f2, coef = ct.lookup(torch.tensor([s], dtype=torch.float64), pair)
+    x = float(f2.reshape(-1)[0])
+    c = coef[0].to(torch.float64).cpu().numpy()        # [E, K], highest power first
+    v = c[:, 0].copy()
+    for k in range(1, c.shape[1]):
+        v = v * x + c[:, k]
+    return v
+
+
+def pack_table(ct, E: int, device) -> FusedTable:
+    t = ct.table.detach().to(torch.float64).cpu().numpy()      # [P, rows, E, K]
+    P, R, Et, K = t.shape
+    if Et != E or E > 32:
+        raise ValueError(f"compress table has width {Et}, the model {E}")
+    pad = np.zeros((P, R, 32, K))
+    pad[:, :, :E] = t
+    arr = np.ascontiguousarray(pad.reshape(P, R, 8, 4, K).transpose(0, 1, 2, 4, 3))
+    return FusedTable(torch.as_tensor(arr, dtype=torch.float32, device=device), ct.order, R, ct.n_fine,
+                      float(ct.x0), float(ct.dx))
+
+
 class _Fitting:
     """Per-type fitting nets in FP32 torch with autograd for dE/dA."""
 
@@ -161,12 +195,28 @@ class FusedDP:
     Atoms must be grouped by type (type ranges are contiguous).
     """
 
-    def __init__(self, w: DPWeights, precision: str = "bf16", device="cuda", fitting: str = "mfma"):
-        """fitting: "mfma" (hand-written matrix-core kernel) or "torch" (FP32 torch + autograd)."""
+    def __init__(self, w: DPWeights, precision: str = "bf16", device="cuda", fitting: str = "mfma", compress=None):
+        """fitting: "mfma" (hand-written matrix-core kernel) or "torch" (FP32 torch + autograd).
+        compress: a matpl_amd.compress.CompressTable; the fused kernels then evaluate the embedding
+        from the table (G and dG/ds by Horner) instead of the net, as DPTorch(compress=...) does."""
         self.w = w
         self.cfg = w.cfg
         self.device = torch.device(device)
         self.fw = pack_fused(w, precision, self.device)
+        self.table = None
+        if compress is not None:
+            T = self.cfg.ntypes
+            self.table = pack_table(compress, self.fw.E, self.device)
+            # the empty-slot vector from the same table
+            vec = self.fw.vec.cpu().numpy()
+            ri_pad = -w.davg / w.dstd
+            for p in range(T * T):
+                g = table_value_np(compress, p, float(ri_pad[p // T, 0]))
+                for h in range(2):
+                    for r in range(16):
+                        f = feat(r, h)
+                        vec[p, V_GPAD, h, r] = g[f] if f < self.fw.E else 0.0
+            self.fw.vec = torch.as_tensor(vec, device=self.device)
         self.fit = _Fitting(w, self.device)
         self.fitting = fitting
         self._mfma_fit = {}
@@ -193,10 +243,16 @@ class FusedDP:
         scale = 1.0 / (cfg.m * T)
         A = torch.empty((N, 4, self.fw.E), dtype=torch.float32, device=dev)
         status = torch.zeros(1, dtype=torch.int32, device=dev)
-        args = (_ptr(pos), _ptr(lattice), _ptr(atom_frame), _ptr(frame_start), _ptr(nl_list), _ptr(nl_shift),
-                _ptr(nl_counts), _ptr(type_map), _ptr(self.fw.vec), _ptr(self.fw.mat), _ptr(self.fw.tinfo), N, T, mnl,
-                cfg.m, self.fw.E, int(self.fw.bf16), float(cfg.rcut), float(cfg.rcut_smooth), scale)
-        _lib.call("matpl_dp_fused_forward", *args, _ptr(A), _ptr(status), _stream())
+        head = (_ptr(pos), _ptr(lattice), _ptr(atom_frame), _ptr(frame_start), _ptr(nl_list), _ptr(nl_shift),
+                _ptr(nl_counts), _ptr(type_map), _ptr(self.fw.vec))
+        tail = (_ptr(self.fw.tinfo), N, T, mnl, cfg.m, self.fw.E, int(self.fw.bf16), float(cfg.rcut),
+                float(cfg.rcut_smooth), scale)
+        if self.table is None:
+            args, sfx = head + (_ptr(self.fw.mat),) + tail, ""
+        else:
+            tb = self.table
+            args, sfx = head + (_ptr(tb.tab), tb.order, tb.rows, tb.n_fine, tb.x0, tb.dx) + tail, "_compressed"
+        _lib.call("matpl_dp_fused_forward" + sfx, *args, _ptr(A), _ptr(status), _stream())
         if self.fitting == "mfma":
             key = tuple(type_ranges)
             if key not in self._mfma_fit:
@@ -217,8 +273,8 @@ class FusedDP:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-        _lib.call("matpl_dp_fused_backward", *args, _ptr(dA), _ptr(rev), _ptr(fslot), _ptr(fcenter), _ptr(F), _ptr(V),
-                  _ptr(status), _stream())
+        _lib.call("matpl_dp_fused_backward" + sfx, *args, _ptr(dA), _ptr(rev), _ptr(fslot), _ptr(fcenter), _ptr(F),
+                  _ptr(V), _ptr(status), _stream())
         if self.events is not None:
             e1.record()
             self.events.append((e0, e1))

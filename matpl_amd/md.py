@@ -33,7 +33,8 @@ def _affine(x: torch.Tensor, M: np.ndarray, out=None) -> torch.Tensor:
 
 class DPMD:
     def __init__(self, frame: Frame, w: DPWeights, velocities: np.ndarray, precision="bf16", skin=0.5,
-                 mnl=None, dt=1.0, rebuild_every=10, device="cuda"):
+                 mnl=None, dt=1.0, rebuild_every=10, device="cuda", compress=None):
+        """compress: a matpl_amd.compress.CompressTable for the tabulated embedding."""
         self.dev = torch.device(device)
         self.cfg = w.cfg
         self.N = frame.natoms
@@ -49,7 +50,7 @@ class DPMD:
         self.mnl = mnl or self.cfg.m
         self.dt = dt
         self.rebuild_every = rebuild_every
-        self.model = FusedDP(w, precision, self.dev)
+        self.model = FusedDP(w, precision, self.dev, compress=compress)
         tm = frame.types
         self.type_ranges = [(int(np.searchsorted(tm, t, "left")), int(np.searchsorted(tm, t, "right")))
                             for t in range(self.cfg.ntypes)]
