@@ -35,7 +35,7 @@ BF16_PEAK_TFLOPS = 2500.0
 # each scaled by a copy kernel of known size (profiles/dp_md_hfo2_99636_hbm_traffic.md; re-measure after
 # kernel changes). Counters cannot be read from inside the timed run. The XCD-aware block order (the
 # autotuner's choice here) keeps partner-slot writes within one XCD's L2; the plain order writes more.
-BWD_TRAFFIC_BYTES = {"xcd": 458.6e6, "plain": 587.1e6}
+BWD_TRAFFIC_BYTES = {"xcd": 459.1e6, "plain": 587.1e6}
 
 
 def log(*a):
@@ -118,7 +118,7 @@ def nep_leg(steps=20):
             nb = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, 2, maxr, maxa)
             return model(nb["NN_r"], nb["NL_r"], nb["Ri_r"], nb["NN_a"], nb["NL_a"], nb["Ri_a"], gp["num_atoms"],
                          gp["atom_type_map"], create_graph=False)
-        for _ in range(3):
+        for _ in range(5):
             r = step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -129,10 +129,25 @@ def nep_leg(steps=20):
         res[name] = r
         out[name] = {"value": round(f.natoms * steps / dt_s, 1), "unit": "atom-steps/s",
                      "ms_per_step": round(dt_s * 1e3 / steps, 3)}
+        # the same step captured as one HIP graph (no host synchronisation inside), replayed
+        g = mn.NEPGraph(model, gp, 2, maxr, maxa)
+        g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            rg = g.replay()
+        torch.cuda.synchronize()
+        dg = time.perf_counter() - t0
+        g.check()
+        dF = float((rg[2].detach() - r[2].detach()).abs().max().item())
+        out[name + "_graph"] = {"value": round(f.natoms * steps / dg, 1), "unit": "atom-steps/s",
+                                "ms_per_step": round(dg * 1e3 / steps, 3), "max_abs_dF_vs_eager": dF}
+        del g
     F32, F64 = res["fp32"][2].detach().double(), res["fp64"][2].detach()
     out["force_rmse_fp32_vs_fp64_eV_per_A"] = float(torch.sqrt(((F32 - F64) ** 2).mean()).item())
     out["speedup_fp32_vs_fp64"] = round(out["fp32"]["value"] / out["fp64"]["value"], 2)
     log(f"NEP: fp32 {out['fp32']['ms_per_step']} ms/step, fp64 {out['fp64']['ms_per_step']} ms/step, "
+        f"HIP graph fp32 {out['fp32_graph']['ms_per_step']} / fp64 {out['fp64_graph']['ms_per_step']} ms/step, "
         f"force RMSE {out['force_rmse_fp32_vs_fp64_eV_per_A']:.2e}")
     return out
 

@@ -163,24 +163,25 @@ class NEPTorch(torch.nn.Module):
             q = torch.cat([q, qa], -1)
         return q
 
-    def energies(self, feats, atom_type_map):
+    def energies(self, feats, atom_type_map, type_index=None):
+        """type_index: the atoms of each type (precomputed, e.g. for graph capture); else found here."""
         x = feats * self.q_scaler
         Ei = torch.zeros(feats.shape[0], dtype=feats.dtype, device=feats.device)
         for t, net in enumerate(self.fit):
-            idx = torch.nonzero(atom_type_map == t).reshape(-1)
+            idx = torch.nonzero(atom_type_map == t).reshape(-1) if type_index is None else type_index[t]
             if idx.numel():
                 Ei = Ei.index_put((idx,), net(x[idx]).reshape(-1))
         return Ei
 
     def forward(self, NN_r, NL_r, Ri_r, NN_a, NL_a, Ri_a, num_atom, atom_type_map, is_calc_f=True,
-                create_graph=None):
+                create_graph=None, type_index=None):
         Ri_r = Ri_r.to(self.dtype).detach().requires_grad_()
         Ri_a = Ri_a.to(self.dtype).detach().requires_grad_()
         l3 = self.cfg.l_max[0]
         feats = self.features(Ri_r, NL_r, Ri_a, NL_a, atom_type_map)
-        Ei = self.energies(feats, atom_type_map)
+        Ei = self.energies(feats, atom_type_map, type_index)
         nf = num_atom.numel()
-        frame = torch.repeat_interleave(torch.arange(nf, device=Ei.device), num_atom.reshape(-1).to(Ei.device))
+        frame = nep_ops._frame_index(num_atom, Ei.device)[1]
         Etot = torch.zeros(nf, dtype=Ei.dtype, device=Ei.device).index_add(0, frame, Ei).unsqueeze(-1)
         if not is_calc_f:
             return Etot, Ei, None, None
@@ -200,3 +201,51 @@ class NEPTorch(torch.nn.Module):
             Force = F if Force is None else Force + F
             Virial = V if Virial is None else Virial + V
         return Etot, Ei, -Force, -Virial
+
+
+class NEPGraph:
+    """E+F+V of a fixed set of frames, neighbor search included, replayed as one HIP graph.
+
+    The positions are read from ``packed["position"]`` at every replay (update it in place). The
+    per-type atom lists and frame index are fixed at construction; the neighbor lists are sized by
+    ``max_r`` / ``max_a``, and overflow is reported by ``check()``. An eager call first runs the
+    checked ops on the same inputs; the captured sequence makes no host synchronisation
+    (nep_ops.static_frames)."""
+
+    def __init__(self, model: NEPTorch, packed, ntypes: int, max_r: int, max_a: int):
+        self.model, self.p = model, packed
+        cfg = model.cfg
+        self.args = (cfg.rcut_radial, cfg.rcut_angular, ntypes, max_r, max_a)
+        tm = packed["atom_type_map"]
+        self.type_index = [torch.nonzero(tm == t).reshape(-1) for t in range(ntypes)]
+        self.eager_out = self._run()                     # checked ops on these inputs
+        cur = torch.cuda.current_stream()
+        side = torch.cuda.Stream()
+        side.wait_stream(cur)
+        with torch.cuda.stream(side):
+            with nep_ops.static_frames(packed["num_atoms"]):
+                for _ in range(2):
+                    self._run()
+        cur.wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with nep_ops.static_frames(packed["num_atoms"]) as st:
+            with torch.cuda.graph(self.graph):
+                self.out = self._run()
+        self.frames = st                                 # keeps the frame index and status alive
+
+    def _run(self):
+        p = self.p
+        nb = neighbors(p, *self.args)
+        return self.model(nb["NN_r"], nb["NL_r"], nb["Ri_r"], nb["NN_a"], nb["NL_a"], nb["Ri_a"], p["num_atoms"],
+                          p["atom_type_map"], create_graph=False, type_index=self.type_index)
+
+    def replay(self):
+        """(Etot, Ei, Force, Virial) at the current positions (tensors reused by every replay)."""
+        self.graph.replay()
+        return self.out
+
+    def check(self):
+        s = int(self.frames.status.item())
+        if s:
+            raise OverflowError(f"NEPGraph: more than max_NN neighbors (atom {s - 1})")

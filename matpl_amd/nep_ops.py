@@ -21,6 +21,9 @@ the coefficient-gradient output (zero in force-loss training).
 """
 from __future__ import annotations
 
+from contextlib import contextmanager
+from dataclasses import dataclass
+
 import numpy as np
 import torch
 
@@ -55,9 +58,38 @@ def _expect(t, numel, what):
         raise ValueError(f"CalcOps_cuda NEP ops: {what} has {t.numel()} elements, expected {numel}")
 
 
+@dataclass
+class StaticFrames:
+    num_atom: torch.Tensor
+    start: torch.Tensor      # [F + 1] int64, first atom of each frame
+    af: torch.Tensor         # [N] int32, frame of each atom
+    status: torch.Tensor     # [1] int32, neighbor-overflow flag of the ops called inside (sticky)
+
+
+_STATIC = None
+
+
+@contextmanager
+def static_frames(num_atom: torch.Tensor):
+    """Inside this context the NEP ops make no host synchronisation, so that a sequence of them can be
+    captured in a HIP graph: the frame index of ``num_atom`` (this tensor object) is computed once on
+    entry, input checks that read device values are skipped, and calculate_neighbor records overflow in
+    the context's ``status`` (1 + atom, sticky) instead of raising. The caller runs the same inputs
+    through the checked ops first and reads ``status`` after the work."""
+    global _STATIC
+    start, af = _frame_index(num_atom, num_atom.device)
+    ctx = StaticFrames(num_atom, start, af, torch.zeros(1, dtype=torch.int32, device=num_atom.device))
+    prev = _STATIC
+    _STATIC = ctx
+    try:
+        yield ctx
+    finally:
+        _STATIC = prev
+
+
 def _check_list(NL, natoms):
     """Neighbor indices must address atoms of the packed batch (the kernels index with them)."""
-    if NL.numel() and int(NL.max().item()) >= natoms:
+    if _STATIC is None and NL.numel() and int(NL.max().item()) >= natoms:
         raise ValueError(f"CalcOps_cuda NEP ops: neighbor index {int(NL.max().item())} >= {natoms} atoms")
 
 
@@ -71,7 +103,7 @@ def _prep(c, Ri, NL, atom_map):
         raise ValueError("CalcOps_cuda NEP ops: NL must be [N, S] with one atom_map entry per atom")
     _expect(Ri, NL.numel() * 4, "Ri [N, S, 4]")
     _check_list(NL, NL.shape[0])
-    if atom_map.numel() and int(atom_map.max().item()) >= c.shape[0]:
+    if _STATIC is None and atom_map.numel() and int(atom_map.max().item()) >= c.shape[0]:
         raise ValueError("CalcOps_cuda NEP ops: atom type beyond the coefficient table")
     return (c.to(Ri.dtype).contiguous(), Ri, NL, atom_map)
 
@@ -189,6 +221,8 @@ class NepForce(torch.autograd.Function):
 
 
 def _frame_index(num_atom, device):
+    if _STATIC is not None and num_atom is _STATIC.num_atom:
+        return _STATIC.start, _STATIC.af
     na = num_atom.reshape(-1).to(torch.int64)
     start = torch.zeros(na.numel() + 1, dtype=torch.int64, device=device)
     start[1:] = torch.cumsum(na.to(device), 0)
@@ -210,7 +244,7 @@ class NepVirial(torch.autograd.Function):
         _expect(Ri_d, N * S * 12, "Ri_d [N, S, 4, 3]")
         _expect(Rij, N * S * 3, "Rij [N, S, 3]")
         _check_list(NL, N)
-        if int(num_atom.sum().item()) != N:
+        if _STATIC is None and int(num_atom.sum().item()) != N:
             raise ValueError(f"calculateNepVirial: num_atom sums to {int(num_atom.sum().item())}, list has {N} atoms")
         start, af = _frame_index(num_atom, dE.device)
         nf = start.numel() - 1
@@ -294,7 +328,8 @@ def nep_centre_force(dE, Ri_d):
 def _nep_geo(num_atoms, box, box_orig, num_cell, position):
     dev = position.device
     nf = num_atoms.numel()
-    if int(num_atoms.sum().item()) != position.shape[0] or position.numel() != 3 * position.shape[0]:
+    if (_STATIC is None and int(num_atoms.sum().item()) != position.shape[0]) or \
+            position.numel() != 3 * position.shape[0]:
         raise ValueError("NEP neighbor search: position must be [sum(num_atoms), 3]")
     if box.numel() != 18 * nf or box_orig.numel() != 9 * nf or num_cell.numel() != 3 * nf:
         raise ValueError("NEP neighbor search: box [F*18], box_orig [F*9], num_cell [F*3] expected")
@@ -307,7 +342,8 @@ def calculate_maxneigh(num_atoms, box, box_orig, num_cell, position, cutoff_2b, 
                        atom_type_map, with_type=False):
     start, af, box, bo, nc, pos = _nep_geo(num_atoms, box, box_orig, num_cell, position)
     N = pos.shape[0]
-    if with_type and (atom_type_map.numel() != N or (N and int(atom_type_map.max().item()) >= atom_type_num)):
+    if with_type and (atom_type_map.numel() != N or
+                      (_STATIC is None and N and int(atom_type_map.max().item()) >= atom_type_num)):
         raise ValueError("calculate_maxneigh: atom_type_map must give each atom a type < atom_type_num")
     shape = (N, atom_type_num) if with_type else (N,)
     NN_r = torch.empty(shape, dtype=torch.int64, device=pos.device)
@@ -332,11 +368,11 @@ def calculate_neighbor(num_atoms, atom_type_map, atom_types, box, box_orig, num_
     NL_a = torch.empty((N, max_NN_angular), dtype=torch.int64, device=dev)
     Ri_r = torch.empty((N, max_NN_radial, w), dtype=torch.float64, device=dev)
     Ri_a = torch.empty((N, max_NN_angular, w), dtype=torch.float64, device=dev)
-    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    status = _STATIC.status if _STATIC is not None else torch.zeros(1, dtype=torch.int32, device=dev)
     _lib.call("matpl_nep_neighbor", _ptr(box), _ptr(bo), _ptr(nc), _ptr(start), _ptr(af), None, _ptr(pos),
               float(cutoff_2b), float(cutoff_3b), int(max_NN_radial), int(max_NN_angular), N, 0, 0, int(with_rij),
               _ptr(NN_r), _ptr(NL_r), _ptr(NN_a), _ptr(NL_a), _ptr(Ri_r), _ptr(Ri_a), _ptr(status), _stream())
-    s = int(status.item())
+    s = int(status.item()) if _STATIC is None else 0
     if s:
         raise OverflowError(f"calculate_neighbor: more than max_NN neighbors (atom {s - 1})")
     it, dt = num_atoms.dtype, position.dtype
