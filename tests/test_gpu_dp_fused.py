@@ -17,8 +17,8 @@ from matpl_amd.neighbor import find_neighbore_gpu, reverse_map
 pytestmark = pytest.mark.gpu
 
 
-def _setup(frame, T, m, seed):
-    cfg = DPConfig(ntypes=T, m=m)
+def _setup(frame, T, m, seed, E=25):
+    cfg = DPConfig(ntypes=T, m=m, emb_sizes=[E, E, E])
     lst, dR = oracle.dp_find_neighbor(frame.lattice, frame.frac(), frame.types, T, m, cfg.rcut)
     davg, dstd = oracle.dp_calc_stat(lst, dR, frame.types, T, cfg.rcut_smooth, cfg.rcut)
     return DPWeights.random(cfg, seed=seed, davg=davg, dstd=dstd)
@@ -143,3 +143,31 @@ def test_fitting_kernel_matches_torch_autograd(dev, precision, tol):
     torch.cuda.synchronize()
     assert torch.allclose(Ei, Ei_ref, atol=tol * float(Ei_ref.abs().max()), rtol=0)
     assert torch.allclose(dA, dA_ref, atol=tol * float(dA_ref.abs().max()), rtol=0)
+
+
+WIDTH_CASES = {
+    "hfo2_324_E16": (lambda: st.fluorite_hfo2((3, 3, 3), seed=21), 2, 120, 16),
+    "hfo2_324_E32": (lambda: st.fluorite_hfo2((3, 3, 3), seed=21), 2, 120, 32),
+    "lgps_300_T4": (lambda: st.lgps_frames(nframes=1, nmin=300, nmax=300, seed=3)[0], 4, 150, 25),
+}
+
+
+@pytest.mark.parametrize("name", list(WIDTH_CASES))
+def test_fused_other_widths_and_types(dev, name):
+    """The other compiled embedding widths (16, 32) and four atom types (Li-Ge-P-S, 16 type pairs,
+    weights read from global memory: they exceed the LDS budget) against the FP64 oracle: FP32 within
+    1e-6 |E| and 1e-5 eV/A per force component, BF16 within force RMSE 1e-3 eV/A."""
+    maker, T, m, E = WIDTH_CASES[name]
+    f = maker()
+    w = _setup(f, T, m, seed=7, E=E)
+    ref = oracle.dp_evaluate(f, w)
+    print(f"case fused {name}", flush=True)
+    Ei, F, V = _fused(f, w, "fp32", dev)
+    assert abs(Ei.sum() - ref["E"]) <= 1e-6 * abs(ref["E"])
+    assert np.abs(F - ref["F"]).max() <= 1e-5, np.abs(F - ref["F"]).max()
+    np.testing.assert_allclose(V.T, ref["V"].reshape(3, 3), atol=1e-5 * np.abs(ref["V"]).max())
+    Ei, F, V = _fused(f, w, "bf16", dev)
+    rmse = np.sqrt(((F - ref["F"]) ** 2).mean())
+    print(f"{name}: bf16 force RMSE {rmse:.2e}", flush=True)
+    assert rmse <= 1e-3
+    assert abs(Ei.sum() - ref["E"]) <= 1e-3 * abs(ref["E"])
