@@ -349,36 +349,70 @@ __device__ __forceinline__ int64_t neighbor_disp(const FusedParams &p, const Ato
 // (dx, dy, dz, bucket slot k as float bits); nv[t] = count. Used bucket
 // entries beyond the cutoff get a zero per-slot force when fslot is given.
 // Returns false on overflow.
+//
+// The used entries of all buckets are taken as one sequence (bucket t's
+// entries at offsets off[t]..off[t+1]), two per lane per pass, and every load
+// of a pass (list, shift, then the neighbors' positions) is issued before any
+// is used: a skin list of ~105 entries in two buckets takes one pass of two
+// dependent load rounds instead of three bucket-by-bucket passes of two each.
+// Within a bucket the entries keep their order, so nb is the same as a
+// bucket-by-bucket compaction.
 __device__ __forceinline__ bool compact(const FusedParams &p, const AtomGeo &g, int64_t i, int lane, float4 *nb,
                                         int *nv, float4 *fslot) {
-    bool ok_all = true;
-    for (int t = 0; t < p.T; t++) {
-        const int cnt = p.counts[i * p.T + t];
-        int n = 0;
-        for (int k0 = 0; k0 < cnt; k0 += WAVE) {
-            const int k = k0 + lane;
-            bool ok = false;
-            float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (k < cnt) {
-                double dx, dy, dz;
-                const int64_t slot = (i * p.T + t) * (int64_t)p.mnl + k;
-                neighbor_disp(p, g, slot, dx, dy, dz);
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                ok = r2 < p.rc2 && r2 > 1e-10;
-                e = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(k));
-                if (!ok && fslot) {
-                    const int r = p.rev[slot];       // -1 if the list is not symmetric (reported by its status)
-                    if (r >= 0) fslot[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+    int off[MAXT + 1], n[MAXT];
+    off[0] = 0;
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) {
+        off[t + 1] = off[t] + (t < p.T ? p.counts[i * p.T + t] : 0);
+        n[t] = 0;
+    }
+    const int total = off[MAXT];
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int e0 = 0; e0 < total; e0 += 2 * WAVE) {
+        int tt[2];
+        int64_t slot[2];
+        double dx[2], dy[2], dz[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int e = e0 + u * WAVE + lane;
+            int t = 0;
+#pragma unroll
+            for (int q = 1; q < MAXT; q++) t += (e >= off[q]) ? 1 : 0;
+            tt[u] = e < total ? t : -1;
+            slot[u] = (i * p.T + t) * (int64_t)p.mnl + (e - off[t]);
+            dx[u] = dy[u] = dz[u] = 0.0;
+            if (e < total) neighbor_disp(p, g, slot[u], dx[u], dy[u], dz[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const double r2 = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+            const bool ok = tt[u] >= 0 && r2 < p.rc2 && r2 > 1e-10;
+            const int k = (int)(slot[u] - (i * p.T + (tt[u] < 0 ? 0 : tt[u])) * (int64_t)p.mnl);
+            if (tt[u] >= 0 && !ok && fslot) {
+                const int r = p.rev[slot[u]];    // -1 if the list is not symmetric (reported by its status)
+                if (r >= 0) fslot[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            const float4 e = make_float4((float)dx[u], (float)dy[u], (float)dz[u], __int_as_float(k));
+#pragma unroll
+            for (int t = 0; t < MAXT; t++) {
+                if (t < p.T) {
+                    const bool mine = ok && tt[u] == t;
+                    const uint64_t bal = __ballot(mine);
+                    const int idx = n[t] + __popcll(bal & below);
+                    if (mine && idx < p.cap) nb[t * p.cap + idx] = e;
+                    n[t] += __popcll(bal);
                 }
             }
-            const uint64_t bal = __ballot(ok);
-            const int idx = n + __popcll(bal & ((1ull << lane) - 1ull));
-            if (ok && idx < p.cap) nb[t * p.cap + idx] = e;
-            n += __popcll(bal);
         }
-        // more than m in-cutoff neighbors is an error the caller reports; only cap of them are used
-        if (lane == 0) nv[t] = n < p.cap ? n : p.cap;
-        if (n > p.m) ok_all = false;
+    }
+    bool ok_all = true;
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) {
+        if (t < p.T) {
+            // more than m in-cutoff neighbors is an error the caller reports; only cap of them are used
+            if (lane == 0) nv[t] = n[t] < p.cap ? n[t] : p.cap;
+            if (n[t] > p.m) ok_all = false;
+        }
     }
     return ok_all;
 }
@@ -512,7 +546,9 @@ __global__ void __launch_bounds__(FUSED_WAVES * WAVE)
 dp_fused_fwd_kernel(FusedParams p) {
     constexpr int NR = NReg<E>::value;
     extern __shared__ float4 smem[];
-    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    // wid is the same on every lane: readfirstlane lets the compiler keep the atom index and the
+    // per-atom bucket counts in scalar registers
+    const int lane = threadIdx.x & (WAVE - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const BlockLds L = block_lds<BF16, LW, false>(p, smem, wid);
     float4 *nb = L.nb;
     int *nvs = L.nvs;
@@ -638,7 +674,9 @@ __global__ void __launch_bounds__(FUSED_WAVES * WAVE)
 dp_fused_bwd_kernel(FusedParams p) {
     constexpr int NR = NReg<E>::value;
     extern __shared__ float4 smem[];
-    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    // wid is the same on every lane: readfirstlane lets the compiler keep the atom index and the
+    // per-atom bucket counts in scalar registers
+    const int lane = threadIdx.x & (WAVE - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const BlockLds L = block_lds<BF16, LW, true>(p, smem, wid);
     float4 *nb = L.nb;
     int *nvs = L.nvs;
