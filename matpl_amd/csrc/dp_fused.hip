@@ -697,14 +697,20 @@ dp_fused_bwd_kernel(FusedParams p) {
             if (lane < 9) vacc[lane] = 0.0;
             vframe = frame;
         }
-        compact(p, g, i, lane, nb, nvs, p.fslot);
+        // the atom's dA (two values per lane) is loaded before the compaction so that its latency
+        // overlaps the neighbor loads
+        float dav[2];
         {
             const float *dAi = p.dA + (size_t)i * 4 * E;
-            for (int x = lane; x < 128; x += WAVE) {
-                const int f = x >> 2, w = x & 3;
-                reinterpret_cast<float *>(dAs)[x] = f < E ? dAi[w * E + f] * p.scale : 0.0f;
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int x = lane + u * WAVE, f = x >> 2, w = x & 3;
+                dav[u] = f < E ? dAi[w * E + f] : 0.0f;
             }
         }
+        compact(p, g, i, lane, nb, nvs, p.fslot);
+#pragma unroll
+        for (int u = 0; u < 2; u++) reinterpret_cast<float *>(dAs)[lane + u * WAVE] = dav[u] * p.scale;
         __builtin_amdgcn_wave_barrier();
         float fc[3] = {0.f, 0.f, 0.f};     // centre-force accumulator (half 0 lanes)
         float vf[9];                       // this atom's virial partial (half 0 lanes)
@@ -720,6 +726,9 @@ dp_fused_bwd_kernel(FusedParams p) {
                 asm volatile("" : "+v"(tp));
                 const bool live = k < nv;
                 float4 e = live ? nb[t * p.cap + k] : make_float4(1.f, 0.f, 0.f, 0.f);
+                const int64_t slot = (i * p.T + t) * (int64_t)p.mnl + __float_as_int(e.w);
+                // gather mode: the partner slot, loaded here so that its latency hides behind the tile
+                const int rs = (p.fslot && live && h == 0) ? p.rev[slot] : -1;
                 float Ri[4] = {0.f, 0.f, 0.f, 0.f};
                 float r = 1.f, s = 0.f, ds = 0.f;
                 if (live) ri_of(e, tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
@@ -803,11 +812,9 @@ dp_fused_bwd_kernel(FusedParams p) {
                     float f[3];
 #pragma unroll
                     for (int a = 0; a < 3; a++) f[a] = -(x[a] * cf + gw[a + 1] * s * ir);
-                    const int64_t slot = (i * p.T + t) * (int64_t)p.mnl + __float_as_int(e.w);
                     if (p.fslot) {
                         // gather mode: the force on j goes to the partner slot (j's entry naming i),
                         // so that j's gather reads its own entries contiguously
-                        const int rs = p.rev[slot];
                         if (rs >= 0) p.fslot[rs] = make_float4(f[0], f[1], f[2], 0.f);
                     } else {
                         const int64_t j = g.fbase + p.list[slot] - 1;
