@@ -876,18 +876,32 @@ dp_fused_bwd_kernel(FusedParams p) {
 __global__ void __launch_bounds__(256)
 gather_kernel(const int32_t *__restrict__ counts, const float4 *__restrict__ fslot, const float *__restrict__ fcenter,
               float *__restrict__ force, int T, int mnl, int64_t n) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    for (int64_t i = (int64_t)blockIdx.x * 4 + threadIdx.x / WAVE; i < n; i += (int64_t)gridDim.x * 4) {
+    // the wave index is uniform: the atom index and its bucket offsets stay in scalar registers
+    const int lane = threadIdx.x & (WAVE - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    for (int64_t i = (int64_t)blockIdx.x * 4 + wid; i < n; i += (int64_t)gridDim.x * 4) {
+        // all buckets as one sequence, two entries per lane per pass, the pass's loads issued together
+        int off[MAXT + 1];
+        off[0] = 0;
+#pragma unroll
+        for (int t = 0; t < MAXT; t++) off[t + 1] = off[t] + (t < T ? counts[i * T + t] : 0);
+        const int total = off[MAXT];
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-        for (int t = 0; t < T; t++) {
-            const int cnt = counts[i * T + t];
-            const float4 *fr = fslot + (i * T + t) * (int64_t)mnl;
-            for (int k = lane; k < cnt; k += WAVE) {
-                const float4 f = fr[k];
-                a0 += f.x; a1 += f.y; a2 += f.z;
+        for (int e0 = 0; e0 < total; e0 += 2 * WAVE) {
+            float4 f[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int e = e0 + u * WAVE + lane;
+                int t = 0, base = 0;
+#pragma unroll
+                for (int q = 1; q < MAXT; q++) {
+                    if (e >= off[q]) { t = q; base = off[q]; }
+                }
+                f[u] = e < total ? fslot[(i * T + t) * (int64_t)mnl + (e - base)] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
+#pragma unroll
+            for (int u = 0; u < 2; u++) { a0 += f[u].x; a1 += f[u].y; a2 += f[u].z; }
         }
-        a0 = wave_sum(a0); a1 = wave_sum(a1); a2 = wave_sum(a2);
+        a0 = wave_sum_dpp(a0); a1 = wave_sum_dpp(a1); a2 = wave_sum_dpp(a2);
         if (lane == 0) {
             force[3 * i] = fcenter[3 * i] + a0;
             force[3 * i + 1] = fcenter[3 * i + 1] + a1;
