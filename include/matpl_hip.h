@@ -374,16 +374,20 @@ int matpl_nep_virial(int f64, const int64_t *NL, const void *dE, const void *Rij
 int matpl_nep_centre_force(int f64, const void *dE, const void *Ri_d, int64_t N, int S, void *F, void *stream);
 
 /*
- * Per-frame virial [nframes][9] alone (zeroed, then accumulated): the first
- * output of matpl_nep_virial without the per-atom rows, for callers that do not
- * use atom_virial (the reference's NEP model returns only the frame virial,
- * src/model/nep_net.py:438, 613-622). Atom i belongs to frame atom_frame[i] (int32 [N],
- * frames contiguous); each list entry counts toward its centre atom's frame (lists
- * never leave a frame, where this equals the sum of the per-atom rows). Same pair terms as launch_calculate_nepvirial
+ * Per-frame virial [nframes][9] alone: the first output of matpl_nep_virial
+ * without its per-atom rows (indexed by neighbor), for callers that do not use
+ * atom_virial (the reference's NEP model returns only the frame virial,
+ * src/model/nep_net.py:438, 613-622). Frame f holds atoms
+ * frame_start[f]..frame_start[f+1] (int64 [nframes+1]); each list entry counts
+ * toward its centre atom's frame (lists never leave a frame, where this equals
+ * the sum of the per-atom rows). rows [N][9] is a workspace that receives each
+ * atom's centre-row sum; the frame sums are formed from it without atomics
+ * (deterministic). Same pair terms as launch_calculate_nepvirial
  * (src/op/include/calculate_nepfeat.h:21).
  */
 int matpl_nep_frame_virial(int f64, const int64_t *NL, const void *dE, const void *Rij, const void *Ri_d,
-                           const int32_t *atom_frame, int64_t nframes, int64_t N, int S, void *virial, void *stream);
+                           const int64_t *frame_start, int64_t nframes, int64_t N, int S, void *rows, void *virial,
+                           void *stream);
 
 /*
  * Backward of the whole force (centre + neighbor terms) with respect to dE:

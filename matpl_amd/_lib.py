@@ -64,7 +64,7 @@ SIGNATURES = {
     "matpl_nep_force": (I32, [I32, P, P, P, P, I64, I32, P]),
     "matpl_nep_virial": (I32, [I32, P, P, P, P, P, I64, I64, I32, P, P, P]),
     "matpl_nep_centre_force": (I32, [I32, P, P, I64, I32, P, P]),
-    "matpl_nep_frame_virial": (I32, [I32, P, P, P, P, P, I64, I64, I32, P, P]),
+    "matpl_nep_frame_virial": (I32, [I32, P, P, P, P, P, I64, I64, I32, P, P, P]),
     "matpl_nep_force_grad": (I32, [I32, P, P, P, P, I64, I32, P]),
     "matpl_nep_virial_grad": (I32, [I32, P, P, P, P, P, P, I64, I32, P]),
     # MD helpers

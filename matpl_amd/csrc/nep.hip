@@ -269,10 +269,25 @@ nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int
             }
             continue;
         }
-        const int ir = wave_incl_scan(cr), ia_ = wave_incl_scan(ca);
+        // list offsets: with one replica (the box is large enough) each lane has at most one hit per
+        // list, and a ballot gives the prefix; several replicas need the scan
+        int ir, ia_, sumr, suma;
+        if (nrep == 1) {
+            const uint64_t br = __ballot(cr > 0), ba = __ballot(ca > 0);
+            const uint64_t below = (1ull << lane) - 1ull;
+            ir = __popcll(br & below) + cr;
+            ia_ = __popcll(ba & below) + ca;
+            sumr = __popcll(br);
+            suma = __popcll(ba);
+        } else {
+            ir = wave_incl_scan(cr);
+            ia_ = wave_incl_scan(ca);
+            sumr = __shfl(ir, WAVE - 1, WAVE);
+            suma = __shfl(ia_, WAVE - 1, WAVE);
+        }
         const int64_t offr = baser + ir - cr, offa = basea + ia_ - ca;
-        baser += __shfl(ir, WAVE - 1, WAVE);
-        basea += __shfl(ia_, WAVE - 1, WAVE);
+        baser += sumr;
+        basea += suma;
         if (mode == 1) continue;
         if (baser > maxr || basea > maxa) { overflow = true; continue; }
         // pass 2: write in (n2, replica) order
@@ -818,49 +833,36 @@ __global__ void __launch_bounds__(256) nep_centre_kernel(const T *__restrict__ d
 
 // Per-frame virial only: sum over the frame's list entries (j >= 0) of
 // (dE . Ri_d)_a Rij_c, the row sum of the per-atom virial without forming it.
-// One wavefront per run of `apw` consecutive atoms, lanes over slots; partial
-// sums go to the frame's 9 entries (zeroed by the caller) when the frame changes.
+// Two passes without atomics (deterministic): one wavefront per atom, lanes
+// over slots, writes the atom's centre row rows[i][9]; nep_virial_reduce_kernel
+// then sums each frame's rows.
 template <typename T>
-__global__ void __launch_bounds__(256) nep_frame_virial_kernel(const int64_t *__restrict__ NL,
-                                                               const T *__restrict__ dE, const T *__restrict__ Rij,
-                                                               const T *__restrict__ Ri_d,
-                                                               const int32_t *__restrict__ atom_frame, int64_t N,
-                                                               int S, int apw, T *__restrict__ virial) {
+__global__ void __launch_bounds__(256) nep_centre_virial_kernel(const int64_t *__restrict__ NL,
+                                                                const T *__restrict__ dE, const T *__restrict__ Rij,
+                                                                const T *__restrict__ Ri_d, int64_t N, int S,
+                                                                T *__restrict__ rows) {
     const int lane = threadIdx.x & (WAVE - 1);
-    const int64_t i0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE * apw;
-    if (i0 >= N) return;
-    const int64_t i1 = i0 + apw < N ? i0 + apw : N;
+    const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / WAVE;
+    if (i >= N) return;
     T v[9];
 #pragma unroll
     for (int c = 0; c < 9; c++) v[c] = T(0);
-    int cur = atom_frame[i0];
-    auto flush = [&]() {
+    for (int k = lane; k < S; k += WAVE) {
+        const int64_t s = i * S + k;
+        if (NL[s] < 0) continue;
+        const T *e = dE + 4 * s, *d = Ri_d + 12 * s, *r = Rij + 3 * s;
 #pragma unroll
-        for (int c = 0; c < 9; c++) {
-            const T x = wave_sum(v[c]);
-            if (lane == 0 && x != T(0)) atomic_add(virial + 9 * (int64_t)cur + c, x);
-            v[c] = T(0);
-        }
-    };
-    for (int64_t i = i0; i < i1; i++) {
-        const int fr = atom_frame[i];
-        if (fr != cur) {
-            flush();
-            cur = fr;
-        }
-        for (int k = lane; k < S; k += WAVE) {
-            const int64_t s = i * S + k;
-            if (NL[s] < 0) continue;
-            const T *e = dE + 4 * s, *d = Ri_d + 12 * s, *r = Rij + 3 * s;
+        for (int a = 0; a < 3; a++) {
+            const T fa = e[0] * d[a] + e[1] * d[3 + a] + e[2] * d[6 + a] + e[3] * d[9 + a];
 #pragma unroll
-            for (int a = 0; a < 3; a++) {
-                const T fa = e[0] * d[a] + e[1] * d[3 + a] + e[2] * d[6 + a] + e[3] * d[9 + a];
-#pragma unroll
-                for (int c = 0; c < 3; c++) v[3 * a + c] += fa * r[c];
-            }
+            for (int c = 0; c < 3; c++) v[3 * a + c] += fa * r[c];
         }
     }
-    flush();
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+        const T x = wave_sum(v[c]);
+        if (lane == 0) rows[9 * i + c] = x;
+    }
 }
 
 // per-frame sum of atom_virial rows: one block per (frame, component)
@@ -989,15 +991,16 @@ int centre_launch(const T *dE, const T *Ri_d, int64_t N, int S, T *F, hipStream_
 }
 
 template <typename T>
-int frame_virial_launch(const int64_t *NL, const T *dE, const T *Rij, const T *Ri_d, const int32_t *atom_frame,
-                        int64_t nframes, int64_t N, int S, T *virial, hipStream_t st) {
-    int rc = fill_async<T>(virial, T(0), nframes * 9, st);
-    if (rc || N == 0 || S == 0) return rc;
-    const int apw = 8;
-    const int64_t waves = ceil_div(N, apw);
-    hipLaunchKernelGGL(nep_frame_virial_kernel<T>, dim3((unsigned)ceil_div(waves, 4)), dim3(256), 0, st, NL, dE, Rij,
-                       Ri_d, atom_frame, N, S, apw, virial);
-    return check_launch("nep_frame_virial_kernel");
+int frame_virial_launch(const int64_t *NL, const T *dE, const T *Rij, const T *Ri_d, const int64_t *frame_start,
+                        int64_t nframes, int64_t N, int S, T *rows, T *virial, hipStream_t st) {
+    if (N == 0 || S == 0) return fill_async<T>(virial, T(0), nframes * 9, st);
+    hipLaunchKernelGGL(nep_centre_virial_kernel<T>, dim3((unsigned)ceil_div(N, 4)), dim3(256), 0, st, NL, dE, Rij,
+                       Ri_d, N, S, rows);
+    int rc = check_launch("nep_centre_virial_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(nep_virial_reduce_kernel<T>, dim3(9, (unsigned)nframes), dim3(256), 0, st, rows, frame_start,
+                       virial);
+    return check_launch("nep_virial_reduce_kernel");
 }
 
 }  // namespace
@@ -1016,17 +1019,17 @@ MATPL_EXPORT int matpl_nep_centre_force(int f64, const void *dE, const void *Ri_
 }
 
 MATPL_EXPORT int matpl_nep_frame_virial(int f64, const int64_t *NL, const void *dE, const void *Rij, const void *Ri_d,
-                                        const int32_t *atom_frame, int64_t nframes, int64_t N, int S, void *virial,
-                                        void *stream) {
+                                        const int64_t *frame_start, int64_t nframes, int64_t N, int S, void *rows,
+                                        void *virial, void *stream) {
     MATPL_CHECK_ARG(N >= 0 && S >= 0 && nframes >= 0, "nep_frame_virial: bad sizes");
     if (nframes == 0) return OK;
-    MATPL_CHECK_ARG(virial && (N == 0 || S == 0 || (NL && dE && Rij && Ri_d && atom_frame)),
+    MATPL_CHECK_ARG(virial && (N == 0 || S == 0 || (NL && dE && Rij && Ri_d && frame_start && rows)),
                     "nep_frame_virial: null pointer");
     hipStream_t st = (hipStream_t)stream;
     return f64 ? frame_virial_launch<double>(NL, (const double *)dE, (const double *)Rij, (const double *)Ri_d,
-                                             atom_frame, nframes, N, S, (double *)virial, st)
+                                             frame_start, nframes, N, S, (double *)rows, (double *)virial, st)
                : frame_virial_launch<float>(NL, (const float *)dE, (const float *)Rij, (const float *)Ri_d,
-                                            atom_frame, nframes, N, S, (float *)virial, st);
+                                            frame_start, nframes, N, S, (float *)rows, (float *)virial, st);
 }
 
 MATPL_EXPORT int matpl_nep_neighbor(const double *box, const double *box_orig, const int64_t *num_cell,

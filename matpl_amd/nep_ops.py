@@ -289,8 +289,9 @@ class NepFrameVirial(torch.autograd.Function):
             raise ValueError(f"nep frame virial: num_atom sums to {af.numel()}, list has {N} atoms")
         nf = start.numel() - 1
         virial = torch.empty((nf, 9), dtype=dE.dtype, device=dE.device)
-        _lib.call("matpl_nep_frame_virial", _f64(dE), _ptr(NL), _ptr(dE), _ptr(Rij), _ptr(Ri_d), _ptr(af), nf, N, S,
-                  _ptr(virial), _stream())
+        rows = torch.empty((N, 9), dtype=dE.dtype, device=dE.device)
+        _lib.call("matpl_nep_frame_virial", _f64(dE), _ptr(NL), _ptr(dE), _ptr(Rij), _ptr(Ri_d), _ptr(start), nf, N,
+                  S, _ptr(rows), _ptr(virial), _stream())
         ctx.save_for_backward(NL, Rij, Ri_d, af)
         ctx.dE_shape = dE.shape
         return virial
