@@ -61,7 +61,7 @@ def build_workload(reps=REPS, device="cuda"):
 def flops_per_neighbor_bwd(E=25):
     """Algorithmic flops of the backward kernel per in-cutoff neighbor: the recomputed embedding forward
     (E + 2E^2 MACs) and the A contraction (4E), then dG (4E), dRi (4E), the two E x E layer backprops (2E^2)
-    and the layer-1 input gradient (E); 2 flops per MAC (5,550 for E = 25)."""
+    and the layer-1 input gradient (E); 2 flops per MAC (5,700 for E = 25)."""
     fwd = E + 2 * E * E + 4 * E          # embedding MACs (1->E, E->E, E->E) + contraction
     bwd = 4 * E + 2 * E * E + E + 4 * E  # dG, dRi, two E->E backprops, layer-1 input grad
     return 2 * (fwd + bwd)
