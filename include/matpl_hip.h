@@ -229,10 +229,9 @@ int matpl_dp_fused_backward(const double *pos, const double *lattice, const int3
  * The fused forward / backward with a compressed embedding: G(s) and dG/ds
  * from piecewise polynomials instead of the net (the reference's DP compress,
  * dp_dp.py:350-355, 572-601 with the table of src/user/ckpt_compress.py).
- * ctab [T*T][rows][8][order+1][4] float (16-byte aligned): for pair p, row k
- * and feature f = 4q + c, the coefficients (highest power first) of the
- * interval's polynomial in x = s - knot_k at [p][k][q][.][c]; features >= E
- * hold zeros. Knots: x0 + k dx for k <= n_fine, then steps of 10 dx; s below
+ * ctab [T*T][rows][32][order+1] float (16-byte aligned): for pair p, row k
+ * and feature f, the coefficients (highest power first) of the interval's
+ * polynomial in x = s - knot_k; features E..31 hold zeros. Knots: x0 + k dx for k <= n_fine, then steps of 10 dx; s below
  * x0 or past the last row extrapolates the first or last interval.
  * order 3 or 5. mat is not read; vec supplies the empty-slot vector
  * G(s_pad), which the caller fills from the same table. Other arguments as

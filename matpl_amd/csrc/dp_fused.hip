@@ -101,9 +101,9 @@ struct FusedParams {
     float iw;                   // 1 / (rmax - rmin)
     int xcd;                    // 1: XCD-aware block order (grid a multiple of NXCD)
     // compressed embedding (table mode): G(s) by piecewise polynomials, K coefficients per
-    // feature and interval, highest power first, laid out [T*T][rows][8 groups][K][4 features]
-    // (group q holds features 4q..4q+3, so register block r/4 of half h reads group 2(r/4)+h)
-    const float4 *ctab;
+    // feature and interval, highest power first, laid out [T*T][rows][32 features][K], so that
+    // the 32 lanes of a half-wave read one interval's row in one coalesced load
+    const float *ctab;
     int crows, cnf;             // table rows; rows of the fine part
     float cx0, cdx, cidx, cxfe; // first knot, fine spacing, 1/dx, end of the fine part
 };
@@ -272,14 +272,12 @@ __device__ __forceinline__ void embed_fwd(const float *vecb, const unsigned char
     }
 }
 
-// Compressed embedding of one tile: G = P_k(s - x_k) and dG/ds = P_k'(s - x_k) for the
-// interval k holding s (intervals of dx up to the end of the fine part, then 10 dx; s outside the
-// table extrapolates the first or last interval), as the reference's calc_compress_polynomial
-// (dp_dp.py:572-601) with the table of ckpt_compress.py. Value and derivative by one Horner pass.
-template <int E, int CK>
-__device__ __forceinline__ void table_eval(const FusedParams &p, int pair, float s, int h, float G[16],
-                                           float Gd[16]) {
-    constexpr int NR = NReg<E>::value;
+// Compressed embedding: G = P_k(s - x_k) and dG/ds = P_k'(s - x_k) for the interval k holding s
+// (intervals of dx up to the end of the fine part, then 10 dx; s outside the table extrapolates the
+// first or last interval), as the reference's calc_compress_polynomial (dp_dp.py:572-601) with the
+// table of ckpt_compress.py. table_row finds the interval: the float offset of its row and s - x_k.
+template <int CK>
+__device__ __forceinline__ void table_row(const FusedParams &p, int pair, float s, int &rowoff, float &x) {
     int row;
     float knot;
     if (s < p.cxfe) {
@@ -291,29 +289,49 @@ __device__ __forceinline__ void table_eval(const FusedParams &p, int pair, float
         row = row > p.crows - 1 ? p.crows - 1 : row;
         knot = p.cxfe + (float)(row - p.cnf) * (10.0f * p.cdx);
     }
-    const float x = s - knot;
-    const float4 *rp = p.ctab + ((size_t)pair * p.crows + row) * (8 * CK);
+    x = s - knot;
+    rowoff = (pair * p.crows + row) * (32 * CK);
+}
+
+// Value and derivative of one feature's polynomial (coefficients c[0..CK), highest power first) at
+// x, by one Horner pass.
+template <int CK>
+__device__ __forceinline__ void horner(const float *c, float x, float &v, float &d) {
+    float cc[CK];
+    if constexpr (CK == 4) {
+        const float4 q = *reinterpret_cast<const float4 *>(c);
+        cc[0] = q.x; cc[1] = q.y; cc[2] = q.z; cc[3] = q.w;
+    } else {
 #pragma unroll
-    for (int rb = 0; rb < NR; rb += 4) {
-        const float4 *c = rp + (2 * (rb >> 2) + h) * CK;
-        float4 v = c[0], d = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int kk = 1; kk < CK; kk++) {
-            const float4 a = c[kk];
-            d.x = d.x * x + v.x; d.y = d.y * x + v.y; d.z = d.z * x + v.z; d.w = d.w * x + v.w;
-            v.x = v.x * x + a.x; v.y = v.y * x + a.y; v.z = v.z * x + a.z; v.w = v.w * x + a.w;
-        }
-        const float vv[4] = {v.x, v.y, v.z, v.w}, dd[4] = {d.x, d.y, d.z, d.w};
-#pragma unroll
-        for (int c4 = 0; c4 < 4; c4++) {
-            if (rb + c4 < 16) {
-                G[rb + c4] = rb + c4 < NR ? vv[c4] : 0.0f;
-                Gd[rb + c4] = rb + c4 < NR ? dd[c4] : 0.0f;
-            }
+        for (int u = 0; u < CK / 2; u++) {
+            const float2 q = reinterpret_cast<const float2 *>(c)[u];
+            cc[2 * u] = q.x;
+            cc[2 * u + 1] = q.y;
         }
     }
+    v = cc[0];
+    d = 0.0f;
 #pragma unroll
-    for (int r = (NR + 3) / 4 * 4; r < 16; r++) { G[r] = 0.0f; Gd[r] = 0.0f; }
+    for (int kk = 1; kk < CK; kk++) {
+        d = d * x + v;
+        v = v * x + cc[kk];
+    }
+}
+
+// The features this lane holds in the accumulator layout (register r: feature f(r, h)).
+template <int E, int CK>
+__device__ __forceinline__ void table_eval(const FusedParams &p, int pair, float s, int h, float G[16],
+                                           float Gd[16]) {
+    constexpr int NR = NReg<E>::value;
+    int rowoff;
+    float x;
+    table_row<CK>(p, pair, s, rowoff, x);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        G[r] = 0.0f;
+        Gd[r] = 0.0f;
+        if (r < NR) horner<CK>(p.ctab + rowoff + feat(r, h) * CK, x, G[r], Gd[r]);
+    }
 }
 
 struct AtomGeo {
@@ -491,8 +509,10 @@ struct Gate16<false> {
 // LDS layout of a fused block: [staged weights (LW only)][neighbor tiles of
 // each wave: T*mnl float4][per-wave bucket counts: MAXT ints][bwd only: per
 // wave dA of the current atom [32 features][4] floats, virial 9 doubles]
-// [BF16 fwd only: per wave the tile's Ri^T [4][32] and G^T [32][32] in bf16].
-constexpr int XT_ELEMS = (4 + 32) * 32;
+// [BF16 fwd only: per wave the tile's Ri^T [4][32] and G^T [32][32 or 40] in bf16].
+// (table mode: G^T rows of 40 instead of 32, which spreads the coalesced evaluation's writes over
+// the LDS banks, and 32 (row offset, x) pairs)
+constexpr int XT_ELEMS = 4 * 32 + 32 * 40 + 2 * 64;
 struct BlockLds {
     const float *vec;
     const unsigned char *mat;
@@ -587,32 +607,55 @@ dp_fused_fwd_kernel(FusedParams p) {
                 float r, s, ds;
                 if (k < nv) ri_of(nb[t * p.cap + k], tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
                 float G[16];
-                if constexpr (CK > 0) {
+                if constexpr (CK > 0 && !BF16) {
                     float Gd[16];
                     table_eval<E, CK>(p, tp, Ri[0], h, G, Gd);
-                } else {
+                } else if constexpr (CK == 0) {
                     float h1[16], t2[16], t3[16];
                     embed_fwd<E, BF16, LW>(L.vec, L.mat, tp, lane, Ri[0], h1, t2, t3, G);
                 }
                 const float live = (k < nv) ? 1.0f : 0.0f;
                 if constexpr (BF16) {
+                    constexpr int GS = CK > 0 ? 40 : 32;      // G^T row stride
                     __bf16 *rit = L.xt, *gt = L.xt + 4 * 32;
                     if (h == 0) {
 #pragma unroll
                         for (int w = 0; w < 4; w++) rit[w * 32 + n] = (__bf16)(Ri[w] * live);
                     }
+                    if constexpr (CK > 0) {
+                        // table mode: half h evaluates neighbor 2 it + h, lane n its feature n, reading the
+                        // interval's row in one coalesced load per half-wave; every neighbor of the tile is
+                        // evaluated (a dead one at row 0, x = 0) so that G^T holds finite values only
+                        float2 *tb = reinterpret_cast<float2 *>(L.xt + 4 * 32 + 32 * 40);
+                        if (h == 0) {
+                            int rowoff = 0;
+                            float x = 0.0f;
+                            if (k < nv) table_row<CK>(p, tp, Ri[0], rowoff, x);
+                            tb[n] = make_float2(__int_as_float(rowoff), x);
+                        }
+                        __builtin_amdgcn_wave_barrier();
+#pragma unroll 4
+                        for (int it = 0; it < TILE / 2; it++) {
+                            const int nn = 2 * it + h;
+                            const float2 q = tb[nn];
+                            float v, d;
+                            horner<CK>(p.ctab + __float_as_int(q.x) + n * CK, q.y, v, d);
+                            if (n < E) gt[n * GS + nn] = (__bf16)v;
+                        }
+                    } else {
 #pragma unroll
-                    for (int rr = 0; rr < NR; rr++) {
-                        const int f = feat(rr, h);
-                        if (f < E) gt[f * 32 + n] = (__bf16)G[rr];
+                        for (int rr = 0; rr < NR; rr++) {
+                            const int f = feat(rr, h);
+                            if (f < E) gt[f * GS + n] = (__bf16)G[rr];
+                        }
                     }
                     __builtin_amdgcn_wave_barrier();
                     // rows w >= 4 and columns e >= E of D are never read: their lanes read row 0
                     const int ia = n < 4 ? n : 0, jb = n < E ? n : 0;
                     const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(rit + ia * 32 + 8 * h);
                     const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(rit + ia * 32 + 16 + 8 * h);
-                    const bf16x8 b0 = *reinterpret_cast<const bf16x8 *>(gt + jb * 32 + 8 * h);
-                    const bf16x8 b1 = *reinterpret_cast<const bf16x8 *>(gt + jb * 32 + 16 + 8 * h);
+                    const bf16x8 b0 = *reinterpret_cast<const bf16x8 *>(gt + jb * GS + 8 * h);
+                    const bf16x8 b1 = *reinterpret_cast<const bf16x8 *>(gt + jb * GS + 16 + 8 * h);
                     D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, D, 0, 0, 0);
                     D = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, D, 0, 0, 0);
                     __builtin_amdgcn_wave_barrier();
@@ -1018,7 +1061,7 @@ static int set_table(FusedParams &p, const float *ctab, int order, int rows, int
     MATPL_CHECK_ARG(rows >= 1 && n_fine >= 0 && n_fine <= rows && dx > 0.0, "dp_fused_compressed: bad table sizes");
     MATPL_CHECK_ARG(E <= 32, "dp_fused_compressed: embedding width above 32");
     MATPL_CHECK_ARG(ctab && aligned16(ctab), "dp_fused_compressed: table must be 16-byte aligned");
-    p.ctab = reinterpret_cast<const float4 *>(ctab);
+    p.ctab = ctab;
     p.crows = rows;
     p.cnf = n_fine;
     p.cx0 = (float)x0;

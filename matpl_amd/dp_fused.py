@@ -118,7 +118,7 @@ def pack_fused(w: DPWeights, precision: str = "bf16", device="cuda") -> FusedWei
 @dataclass
 class FusedTable:
     """A compressed embedding (matpl_amd.compress.CompressTable) in the fused kernels' layout."""
-    tab: torch.Tensor      # [T*T, rows, 8, K, 4] float32: pair, interval, feature group, coefficient, feature
+    tab: torch.Tensor      # [T*T, rows, 32, K] float32: pair, interval, feature (padded to 32), coefficient
     order: int
     rows: int
     n_fine: int
@@ -142,10 +142,11 @@ def pack_table(ct, E: int, device) -> FusedTable:
     P, R, Et, K = t.shape
     if Et != E or E > 32:
         raise ValueError(f"compress table has width {Et}, the model {E}")
+    if K != ct.order + 1:
+        raise ValueError(f"compress table of order {ct.order} has {K} coefficients")
     pad = np.zeros((P, R, 32, K))
     pad[:, :, :E] = t
-    arr = np.ascontiguousarray(pad.reshape(P, R, 8, 4, K).transpose(0, 1, 2, 4, 3))
-    return FusedTable(torch.as_tensor(arr, dtype=torch.float32, device=device), ct.order, R, ct.n_fine,
+    return FusedTable(torch.as_tensor(pad, dtype=torch.float32, device=device), ct.order, R, ct.n_fine,
                       float(ct.x0), float(ct.dx))
 
 
