@@ -250,7 +250,8 @@ cell_kernel(Lattice lat, Grid g, const double *__restrict__ fw, const int32_t *_
             int64_t n, int32_t *__restrict__ list, double *__restrict__ dR, int32_t *__restrict__ counts,
             int32_t *__restrict__ shift, int32_t *__restrict__ status) {
     extern __shared__ int32_t smem[];
-    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    // wid is the same on every lane: the atom, its cell and the cells' ranges stay scalar
+    const int lane = threadIdx.x & (WAVE - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     int32_t *bucket = smem + wid * (T * m + 2 * T);   // [T][m] hit indices
     int32_t *cnt = bucket + T * m;                    // [T] stored
     int32_t *cntM = cnt + T;                          // [T] counted at rc_m
@@ -296,19 +297,36 @@ cell_kernel(Lattice lat, Grid g, const double *__restrict__ fw, const int32_t *_
             const int cy = (ci[1] + g.off[1][b] + g.nc[1]) % g.nc[1];
             const int col = (cx * g.nc[1] + cy) * g.nc[2];
             const int s0 = cell_start[col + zr0[c]], s1 = cell_start[col + zr1[c] + 1];
-            for (int p = s0 + lane; p < s1; p += WAVE) {
-                const int j = cell_atoms[p];
-                double dx1 = fw[3 * j] - fi0, dx2 = fw[3 * j + 1] - fi1, dx3 = fw[3 * j + 2] - fi2;
-                near_image(dx1); near_image(dx2); near_image(dx3);
-                double dx, dy, dz;
-                frac_to_cart(L, dx1, dx2, dx3, dx, dy, dz);
-                const double dd = dx * dx + dy * dy + dz * dz;
-                if (dd < rc2 && dd > 1e-8) {
-                    const int tj = type_map[j];
-                    atomicAdd(cntM + tj, 1);
-                    if (dd < rci2) {
-                        const int k = atomicAdd(cnt + tj, 1);
-                        if (k < m) bucket[tj * m + k] = j;
+            // two candidates per lane per pass, both passes' loads issued before either is used
+            for (int p0 = s0; p0 < s1; p0 += 2 * WAVE) {
+                int jj[2];
+                double fj[2][3];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int p = p0 + u * WAVE + lane;
+                    jj[u] = p < s1 ? cell_atoms[p] : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int j = jj[u] < 0 ? 0 : jj[u];
+                    fj[u][0] = fw[3 * j]; fj[u][1] = fw[3 * j + 1]; fj[u][2] = fw[3 * j + 2];
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int j = jj[u];
+                    if (j < 0) continue;
+                    double dx1 = fj[u][0] - fi0, dx2 = fj[u][1] - fi1, dx3 = fj[u][2] - fi2;
+                    near_image(dx1); near_image(dx2); near_image(dx3);
+                    double dx, dy, dz;
+                    frac_to_cart(L, dx1, dx2, dx3, dx, dy, dz);
+                    const double dd = dx * dx + dy * dy + dz * dz;
+                    if (dd < rc2 && dd > 1e-8) {
+                        const int tj = type_map[j];
+                        atomicAdd(cntM + tj, 1);
+                        if (dd < rci2) {
+                            const int k = atomicAdd(cnt + tj, 1);
+                            if (k < m) bucket[tj * m + k] = j;
+                        }
                     }
                 }
             }
