@@ -9,8 +9,11 @@
 //         in-cutoff ones are compacted into LDS as the normalized
 //         Ri = ((s, s x/r, s y/r, s z/r) - davg) / dstd in FP32, and the
 //         embedding net G = emb_{ti,tj}(Ri_0) runs on 32-neighbor tiles on the
-//         matrix cores. The contraction A = sum Ri^T G is accumulated in FP32
-//         per lane and reduced across the wave; the empty slots of each type
+//         matrix cores. The contraction A = sum Ri^T G runs on the matrix
+//         cores too in BF16 mode (each tile's Ri^T and G^T are staged in LDS,
+//         neighbors as the reduction index), in FP32 mode per lane with wave
+//         sums. With a compression table G comes from the interval's
+//         polynomial instead of the net. The empty slots of each type
 //         bucket (m - n_t of them, plus neighbors beyond rcut) add the closed
 //         form (m - n_t) * [s_pad,0,0,0]^T G_{ti,tj}(s_pad), s_pad =
 //         -davg_0/dstd_0 (dp_dp.py:293-297 normalizes empty slots too).
