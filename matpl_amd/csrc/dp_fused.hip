@@ -731,6 +731,12 @@ dp_fused_bwd_kernel(FusedParams p) {
     const int h = lane >> 5, n = lane & 31;
     if (lane < 9) vacc[lane] = 0.0;
     int vframe = -1;
+    // virial partials of half-0 lanes, FP32, over this wave's atoms of the current frame (a few
+    // hundred terms per lane); reduced across the wave and added to the FP64 frame sum when the
+    // frame changes and at the end
+    float vf[9];
+#pragma unroll
+    for (int v = 0; v < 9; v++) vf[v] = 0.0f;
 
     for (int64_t i = logical_block(p.xcd) * FUSED_WAVES + wid; i < p.natoms; i += (int64_t)gridDim.x * FUSED_WAVES) {
         const int ti = p.type_map[i];
@@ -738,7 +744,14 @@ dp_fused_bwd_kernel(FusedParams p) {
         const AtomGeo g = atom_geo(p, i);
         const int frame = p.atom_frame ? p.atom_frame[i] : 0;
         if (frame != vframe) {
-            // flush this wave's virial of the previous frame
+            // flush this wave's virial of the previous frame: lane partials -> FP64 -> frame sum
+#pragma unroll
+            for (int v = 0; v < 9; v++) {
+                const float x = wave_sum_dpp(vf[v]);
+                if (lane == 0) vacc[v] += (double)x;
+                vf[v] = 0.0f;
+            }
+            __builtin_amdgcn_wave_barrier();
             if (vframe >= 0 && lane < 9 && vacc[lane] != 0.0) atomic_add(p.virial + 9 * vframe + lane, vacc[lane]);
             if (lane < 9) vacc[lane] = 0.0;
             vframe = frame;
@@ -759,9 +772,6 @@ dp_fused_bwd_kernel(FusedParams p) {
         for (int u = 0; u < 2; u++) reinterpret_cast<float *>(dAs)[lane + u * WAVE] = dav[u] * p.scale;
         __builtin_amdgcn_wave_barrier();
         float fc[3] = {0.f, 0.f, 0.f};     // centre-force accumulator (half 0 lanes)
-        float vf[9];                       // this atom's virial partial (half 0 lanes)
-#pragma unroll
-        for (int v = 0; v < 9; v++) vf[v] = 0.0f;
         for (int t = 0; t < p.T; t++) {
             const int nv = nvs[t];
             const int pair = ti * p.T + t;
@@ -885,14 +895,15 @@ dp_fused_bwd_kernel(FusedParams p) {
                 else atomic_add(p.force + 3 * i + a, x);
             }
         }
-        // the atom's virial (FP32 over its ~100 slots) joins the wave's FP64 frame sum
-#pragma unroll
-        for (int v = 0; v < 9; v++) {
-            const float x = wave_sum_dpp(vf[v]);
-            if (lane == 0) vacc[v] += (double)x;
-        }
         __builtin_amdgcn_wave_barrier();
     }
+    // the last frame's lane partials
+#pragma unroll
+    for (int v = 0; v < 9; v++) {
+        const float x = wave_sum_dpp(vf[v]);
+        if (lane == 0) vacc[v] += (double)x;
+    }
+    __builtin_amdgcn_wave_barrier();
     // virial: wave sums -> one set of atomics per block and frame
     __shared__ int vfr[FUSED_WAVES];
     if (lane == 0) vfr[wid] = vframe;
