@@ -889,7 +889,7 @@ dp_fused_bwd_kernel(FusedParams p) {
         }
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            const float x = wave_sum_dpp(fc[a]);
+            const float x = half_sum(fc[a]);      // only half-0 lanes hold centre-force terms
             if (lane == 0) {
                 if (p.fcenter) p.fcenter[3 * i + a] = x;
                 else atomic_add(p.force + 3 * i + a, x);
