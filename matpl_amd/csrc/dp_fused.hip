@@ -731,12 +731,14 @@ dp_fused_bwd_kernel(FusedParams p) {
     const int h = lane >> 5, n = lane & 31;
     if (lane < 9) vacc[lane] = 0.0;
     int vframe = -1;
-    // virial partials of half-0 lanes, FP32, over this wave's atoms of the current frame (a few
-    // hundred terms per lane); reduced across the wave and added to the FP64 frame sum when the
-    // frame changes and at the end
+    // virial of half-0 lanes: an FP32 partial per atom (vf, the lane's share of the atom's slots),
+    // added to an FP64 lane sum over this wave's atoms of the current frame (vd); the lane sums are
+    // reduced across the wave into the frame sum only when the frame changes and at the end, and
+    // they do not depend on which atoms share a wave beyond FP64 rounding
     float vf[9];
+    double vd[9];
 #pragma unroll
-    for (int v = 0; v < 9; v++) vf[v] = 0.0f;
+    for (int v = 0; v < 9; v++) { vf[v] = 0.0f; vd[v] = 0.0; }
 
     for (int64_t i = logical_block(p.xcd) * FUSED_WAVES + wid; i < p.natoms; i += (int64_t)gridDim.x * FUSED_WAVES) {
         const int ti = p.type_map[i];
@@ -744,12 +746,12 @@ dp_fused_bwd_kernel(FusedParams p) {
         const AtomGeo g = atom_geo(p, i);
         const int frame = p.atom_frame ? p.atom_frame[i] : 0;
         if (frame != vframe) {
-            // flush this wave's virial of the previous frame: lane partials -> FP64 -> frame sum
+            // flush this wave's virial of the previous frame: lane sums -> frame sum
 #pragma unroll
             for (int v = 0; v < 9; v++) {
-                const float x = wave_sum_dpp(vf[v]);
-                if (lane == 0) vacc[v] += (double)x;
-                vf[v] = 0.0f;
+                const double x = wave_sum(vd[v]);
+                if (lane == 0) vacc[v] += x;
+                vd[v] = 0.0;
             }
             __builtin_amdgcn_wave_barrier();
             if (vframe >= 0 && lane < 9 && vacc[lane] != 0.0) atomic_add(p.virial + 9 * vframe + lane, vacc[lane]);
@@ -895,13 +897,18 @@ dp_fused_bwd_kernel(FusedParams p) {
                 else atomic_add(p.force + 3 * i + a, x);
             }
         }
+#pragma unroll
+        for (int v = 0; v < 9; v++) {
+            vd[v] += (double)vf[v];
+            vf[v] = 0.0f;
+        }
         __builtin_amdgcn_wave_barrier();
     }
-    // the last frame's lane partials
+    // the last frame's lane sums
 #pragma unroll
     for (int v = 0; v < 9; v++) {
-        const float x = wave_sum_dpp(vf[v]);
-        if (lane == 0) vacc[v] += (double)x;
+        const double x = wave_sum(vd[v]);
+        if (lane == 0) vacc[v] += x;
     }
     __builtin_amdgcn_wave_barrier();
     // virial: wave sums -> one set of atomics per block and frame
