@@ -171,3 +171,44 @@ def test_fused_other_widths_and_types(dev, name):
     print(f"{name}: bf16 force RMSE {rmse:.2e}", flush=True)
     assert rmse <= 1e-3
     assert abs(Ei.sum() - ref["E"]) <= 1e-3 * abs(ref["E"])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_fused_several_frames(dev, precision):
+    """Frames packed along the atom axis (atom_frame / frame_start, a lattice per frame, list indices
+    within the frame, boxes small enough that several images of a neighbor appear): per-frame energies,
+    forces and virials against the FP64 oracle of each frame (one atom type, so the packed atoms stay
+    grouped by type). FP32 within 1e-6 |E| and 1e-5 eV/A; BF16 within force RMSE 1e-3 eV/A."""
+    from matpl_amd.neighbor import find_neighbore_batch
+    frames = [st.fcc_cu((3, 3, 3), seed=1), st.fcc_cu((3, 3, 4), a=3.65, seed=2), st.fcc_cu((4, 3, 3), seed=3)]
+    T, m = 1, 100
+    w = _setup(frames[0], T, m, seed=9)
+    print(f"case fused {len(frames)} frames {precision}", flush=True)
+    nl = find_neighbore_batch([f.lattice for f in frames], [f.frac() for f in frames], [f.types for f in frames],
+                              T, m, w.cfg.rcut, want_dR=False, want_shift=True, device=dev)
+    sizes = [f.natoms for f in frames]
+    fs = torch.as_tensor(np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64), device=dev)
+    af = torch.as_tensor(np.repeat(np.arange(len(frames), dtype=np.int32), sizes), device=dev)
+    lat = torch.as_tensor(np.stack([np.asarray(f.lattice).reshape(9) for f in frames]), device=dev)
+    pos = torch.cat([nl.frac_wrapped[int(fs[k]):int(fs[k + 1])] @ lat[k].reshape(3, 3)
+                     for k in range(len(frames))]).contiguous()
+    tm = torch.zeros(sum(sizes), dtype=torch.int32, device=dev)
+    rev, rst = reverse_map(nl, tm, m, frame_start=fs, atom_frame=af)
+    assert int(rst.item()) == 0
+    model = FusedDP(w, precision, dev)
+    Ei, F, V, status = model(pos, lat, tm, nl.list, nl.shift, nl.counts, m, atom_frame=af, frame_start=fs,
+                             nframes=len(frames), type_ranges=[(0, sum(sizes))], rev=rev)
+    torch.cuda.synchronize()
+    assert int(status.item()) == 0
+    Ei, F, V = Ei.double().cpu().numpy(), F.double().cpu().numpy(), V.cpu().numpy()
+    for k, f in enumerate(frames):
+        a, b = int(fs[k]), int(fs[k + 1])
+        ref = oracle.dp_evaluate(f, w)
+        if precision == "fp32":
+            assert abs(Ei[a:b].sum() - ref["E"]) <= 1e-6 * abs(ref["E"]), k
+            assert np.abs(F[a:b] - ref["F"]).max() <= 1e-5, (k, np.abs(F[a:b] - ref["F"]).max())
+            np.testing.assert_allclose(V[k].reshape(3, 3).T, ref["V"].reshape(3, 3),
+                                       atol=1e-5 * np.abs(ref["V"]).max())
+        else:
+            assert np.sqrt(((F[a:b] - ref["F"]) ** 2).mean()) <= 1e-3, k
+            assert abs(Ei[a:b].sum() - ref["E"]) <= 1e-3 * abs(ref["E"]), k
