@@ -773,6 +773,10 @@ dp_fused_bwd_kernel(FusedParams p) {
 #pragma unroll
         for (int u = 0; u < 2; u++) reinterpret_cast<float *>(dAs)[lane + u * WAVE] = dav[u] * p.scale;
         __builtin_amdgcn_wave_barrier();
+        // A operands of dG = dA Ri^T (32 features x 4, K split in two steps of 2): lane (n, h)
+        // holds dA[n][h] and dA[n][2 + h]
+        const float da0 = reinterpret_cast<const float *>(dAs)[4 * n + h];
+        const float da1 = reinterpret_cast<const float *>(dAs)[4 * n + 2 + h];
         float fc[3] = {0.f, 0.f, 0.f};     // centre-force accumulator (half 0 lanes)
         for (int t = 0; t < p.T; t++) {
             const int nv = nvs[t];
@@ -792,6 +796,13 @@ dp_fused_bwd_kernel(FusedParams p) {
                 if (live) ri_of(e, tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
                 float dRi[4] = {0.f, 0.f, 0.f, 0.f};
                 float dsum = 0.0f;
+                // dG = dA Ri^T on the matrix cores in FP32: rows = features (accumulator layout),
+                // columns = the tile's neighbors; dead lanes have Ri = 0 and so dG = 0
+                floatx16 acc;
+#pragma unroll
+                for (int rr = 0; rr < 16; rr++) acc[rr] = 0.0f;
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(da0, h ? Ri[1] : Ri[0], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(da1, h ? Ri[3] : Ri[2], acc, 0, 0, 0);
                 if constexpr (CK > 0) {
                     // table mode: dE/ds = sum_f dG_f G'_f(s)
                     float G[16], Gd[16];
@@ -799,53 +810,46 @@ dp_fused_bwd_kernel(FusedParams p) {
 #pragma unroll
                     for (int rr = 0; rr < NR; rr++) {
                         const float4 d = dAs[feat(rr, h)];
-                        const float dG = d.x * Ri[0] + d.y * Ri[1] + d.z * Ri[2] + d.w * Ri[3];
                         dRi[0] += d.x * G[rr];
                         dRi[1] += d.y * G[rr];
                         dRi[2] += d.z * G[rr];
                         dRi[3] += d.w * G[rr];
-                        dsum += dG * Gd[rr];
+                        dsum += acc[rr] * Gd[rr];
                     }
                 } else {
                 const float *w1 = vec_ptr(L.vec, tp, V_W1, h);
-                const float *r2v = vec_ptr(L.vec, tp, V_R2, h), *r3v = vec_ptr(L.vec, tp, V_R3, h);
                 float G[16];
                 Gate16<false> g1, g2, g3;
                 {
                     float h1[16], t2[16], t3[16];
                     embed_fwd<E, BF16, LW>(L.vec, L.mat, tp, lane, Ri[0], h1, t2, t3, G);
+                    // the resnet scales r3, r2 are folded into the backward matrices
                     float q[16];
 #pragma unroll
-                    for (int rr = 0; rr < 16; rr++) q[rr] = rr < NR ? r3v[rr] * (1.0f - t3[rr] * t3[rr]) : 0.0f;
+                    for (int rr = 0; rr < 16; rr++) q[rr] = rr < NR ? 1.0f - t3[rr] * t3[rr] : 0.0f;
                     g3.set(q);
 #pragma unroll
-                    for (int rr = 0; rr < 16; rr++) q[rr] = rr < NR ? r2v[rr] * (1.0f - t2[rr] * t2[rr]) : 0.0f;
+                    for (int rr = 0; rr < 16; rr++) q[rr] = rr < NR ? 1.0f - t2[rr] * t2[rr] : 0.0f;
                     g2.set(q);
 #pragma unroll
                     for (int rr = 0; rr < 16; rr++) q[rr] = rr < NR ? w1[rr] * (1.0f - h1[rr] * h1[rr]) : 0.0f;
                     g1.set(q);
                 }
-                // dG = dA^T Ri, dRi = dA G (partial over this half's features)
-                float dG[16];
+                // dRi = dA G (partial over this half's features)
 #pragma unroll
-                for (int rr = 0; rr < 16; rr++) {
-                    const float4 d = rr < NR ? dAs[feat(rr, h)] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    dG[rr] = d.x * Ri[0] + d.y * Ri[1] + d.z * Ri[2] + d.w * Ri[3];
+                for (int rr = 0; rr < NR; rr++) {
+                    const float4 d = dAs[feat(rr, h)];
                     dRi[0] += d.x * G[rr];
                     dRi[1] += d.y * G[rr];
                     dRi[2] += d.z * G[rr];
                     dRi[3] += d.w * G[rr];
                 }
-                // layer 3: G = h2 + r3 t3 -> dh2 = dG + W3 (dG r3 (1-t3^2))
+                // layer 3: G = h2 + r3 t3 -> dh2 = dG + (W3 diag r3) (dG (1-t3^2))
                 float dz[16];
-                floatx16 acc;
 #pragma unroll
-                for (int rr = 0; rr < 16; rr++) {
-                    dz[rr] = dG[rr] * g3.get(rr);
-                    acc[rr] = dG[rr];
-                }
+                for (int rr = 0; rr < 16; rr++) dz[rr] = acc[rr] * g3.get(rr);
                 mfma_layer<BF16, LW>(mat_ptr<BF16>(L.mat, tp, M_W3B), lane, dz, acc);
-                // layer 2: h2 = h1 + r2 t2 -> dh1 = dh2 + W2 (dh2 r2 (1-t2^2))
+                // layer 2: h2 = h1 + r2 t2 -> dh1 = dh2 + (W2 diag r2) (dh2 (1-t2^2))
 #pragma unroll
                 for (int rr = 0; rr < 16; rr++) dz[rr] = acc[rr] * g2.get(rr);
                 mfma_layer<BF16, LW>(mat_ptr<BF16>(L.mat, tp, M_W2B), lane, dz, acc);

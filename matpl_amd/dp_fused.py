@@ -94,6 +94,9 @@ def pack_fused(w: DPWeights, precision: str = "bf16", device="cuda") -> FusedWei
                 if f < E:
                     for k, v in vecs.items():
                         vec[p, k, h, r] = v[f]
+        # the backward matrices carry the resnet scale of their layer's output feature f
+        # (dh_in = dh_out + (W diag r) (dh_out (1 - t^2)))
+        r2s, r3s = vecs[V_R2], vecs[V_R3]
         for lane in range(64):
             h, i = lane // 32, lane % 32
             for r in range(16):
@@ -101,8 +104,8 @@ def pack_fused(w: DPWeights, precision: str = "bf16", device="cuda") -> FusedWei
                 if f < E and i < E:
                     mat[p, M_W2F, lane, r] = W2[f, i]
                     mat[p, M_W3F, lane, r] = W3[f, i]
-                    mat[p, M_W2B, lane, r] = W2[i, f]
-                    mat[p, M_W3B, lane, r] = W3[i, f]
+                    mat[p, M_W2B, lane, r] = W2[i, f] * r2s[f]
+                    mat[p, M_W3B, lane, r] = W3[i, f] * r3s[f]
     if bf16:
         # tanh(z) is evaluated as 1 - 2 / (2^y + 1) with y = TANH_K z: pre-scale the forward weights and
         # biases of every tanh layer (the backward matrices and the padding vector stay unscaled)
