@@ -417,6 +417,18 @@ int matpl_md_kick(double *v, double *x, const float *F, const double *inv_mass, 
 /* out[i] = in[i] . M for row vectors (M host [9]); Cartesian <-> fractional. */
 int matpl_md_affine(const double *in, double *out, const double *M_host, int64_t n, void *stream);
 
+/* ------------------------------------------------ tall-by-small products */
+/*
+ * y [K][b] = x [K][a] . W [a][b], FP32, 1 <= a, b <= 64: x row-major with row
+ * stride ldx >= a, W element (j, c) at W[j * w_row + c * w_col], y contiguous.
+ * lowp != 0: operands and result rounded to bfloat16 (products and sums in FP32).
+ * Replaces the layer products torch.matmul(x, weight) of EmbeddingNet /
+ * FittingNet (src/model/dp_embedding.py:73-94, 140-152) in the batched DP
+ * training step, where the reference leaves them to the BLAS library.
+ */
+int matpl_tall_mm(const float *x, int64_t ldx, const float *W, int64_t w_row, int64_t w_col, float *y, int64_t K,
+                  int a, int b, int lowp, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

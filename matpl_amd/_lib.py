@@ -70,6 +70,7 @@ SIGNATURES = {
     # MD helpers
     "matpl_md_kick": (I32, [P, P, P, P, F64, I64, I32, P]),
     "matpl_md_affine": (I32, [P, P, P, I64, P]),
+    "matpl_tall_mm": (I32, [P, I64, P, I64, I64, P, I64, I32, I32, I32, P]),
 }
 
 

@@ -5,13 +5,39 @@ hipBLAS picks a single-tile kernel for x^T g when x and g are tall and thin
 (the weight gradient of a layer over all atoms / slots): one tile loops over
 all K rows serially. ``_TallMM`` replaces x @ W so that the weight gradient,
 at every derivative order, goes through ``_tall_tn``, which splits K into row
-blocks of a batched product and sums them.
+blocks of a batched product and sums them. The products x @ W themselves, for
+FP32 device tensors with a, b <= 64 (every embedding layer and the fitting
+net's hidden layers, forward and input gradients), run on the library's
+tall-by-small kernel (matpl_tall_mm), which reads and writes FP32 and, in BF16
+mode, rounds operands and result in registers; wider products stay with torch.
 """
 from __future__ import annotations
 
 import torch
 
+from . import _lib
+from .calcops import _ptr, _stream
+
 _SPLIT_ROWS = 512
+_SMALL_MAX = 64
+
+
+def _small(x, W) -> bool:
+    """x [K, a] @ W [a, b] goes to the HIP tall-by-small kernel: FP32 device tensors, a, b <= 64."""
+    return (x.is_cuda and x.dtype == torch.float32 and W.dtype == torch.float32 and x.dim() == 2
+            and 1 <= x.shape[1] <= _SMALL_MAX and 1 <= W.shape[1] <= _SMALL_MAX)
+
+
+def _small_mm(x, W, lowp: bool):
+    """x @ W on matpl_tall_mm (FP32 in and out; lowp: BF16 operands and result, FP32 sums). W may be
+    any strided view (a transposed weight needs no copy); x needs unit column stride."""
+    if x.stride(1) != 1 or x.stride(0) < x.shape[1]:
+        x = x.contiguous()
+    K, a = x.shape
+    y = torch.empty((K, W.shape[1]), dtype=torch.float32, device=x.device)
+    _lib.call("matpl_tall_mm", _ptr(x), max(x.stride(0), a), _ptr(W), W.stride(0), W.stride(1), _ptr(y), K, a,
+              W.shape[1], int(lowp), _stream())
+    return y
 
 
 def _tall_tn(x, g):
@@ -39,7 +65,7 @@ class _TallMM(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, W):
         ctx.save_for_backward(x, W)
-        return x @ W
+        return _small_mm(x, W, False) if _small(x, W) else x @ W
 
     @staticmethod
     def backward(ctx, g):
@@ -64,6 +90,8 @@ class _TallMMLow(torch.autograd.Function):
     def forward(ctx, x, W, dt):
         ctx.save_for_backward(x, W)
         ctx.dt = dt
+        if dt == torch.bfloat16 and _small(x, W):
+            return _small_mm(x, W, True)
         return (x.to(dt) @ W.to(dt)).float()
 
     @staticmethod
