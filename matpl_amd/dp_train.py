@@ -39,7 +39,7 @@ from .bsmm import bmm_tn
 from .dp import DPTorch, _Net
 from .dpnet import DPWeights
 from .neighbor import env_mat, find_neighbore_batch, image_dr
-from .tallmm import add_bias, tall_mm, tall_mm_bf16
+from .tallmm import add_bias, tall_mm_bf16, tall_mm_f32
 
 
 @dataclass
@@ -102,7 +102,7 @@ class DPTrain(torch.nn.Module):
             raise ValueError("precision must be 'bf16' or 'fp32'")
         cfg = w.cfg
         self.cfg = cfg
-        self.mm = tall_mm_bf16 if precision == "bf16" else tall_mm
+        self.mm = tall_mm_bf16 if precision == "bf16" else tall_mm_f32
         T = cfg.ntypes
         f32 = torch.float32
         self.emb = torch.nn.ModuleList([_Net(w.emb[p], 0, f32, device) for p in range(T * T)])

@@ -129,7 +129,7 @@ class _FitNet(torch.nn.Module):
     def forward(self, x):
         L = len(self.W)
         for l in range(L):
-            z = _TallMM.apply(x, self.W[l]) + self.b[l]
+            z = _TallMM.apply(x, self.W[l], False) + self.b[l]
             if l == L - 1:
                 return z
             h = torch.tanh(z)

@@ -5,11 +5,12 @@ hipBLAS picks a single-tile kernel for x^T g when x and g are tall and thin
 (the weight gradient of a layer over all atoms / slots): one tile loops over
 all K rows serially. ``_TallMM`` replaces x @ W so that the weight gradient,
 at every derivative order, goes through ``_tall_tn``, which splits K into row
-blocks of a batched product and sums them. The products x @ W themselves, for
-FP32 device tensors with a, b <= 64 (every embedding layer and the fitting
-net's hidden layers, forward and input gradients), run on the library's
-tall-by-small kernel (matpl_tall_mm), which reads and writes FP32 and, in BF16
-mode, rounds operands and result in registers; wider products stay with torch.
+blocks of a batched product and sums them. In ``tall_mm_f32`` and
+``tall_mm_bf16`` the products x @ W themselves, for FP32 device tensors with
+a, b <= 64 (every DP embedding layer and the fitting net's hidden layers,
+forward and input gradients), run on the library's tall-by-small kernel
+(matpl_tall_mm), which reads and writes FP32 and, in BF16 mode, rounds operands
+and result in registers; wider products stay with torch.
 """
 from __future__ import annotations
 
@@ -60,25 +61,34 @@ def _tall_tn(x, g):
 class _TallMM(torch.autograd.Function):
     """x @ W for tall x [K, a], small W [a, b]; the weight gradient goes through _tall_tn and the
     input gradient through _TallMM again, so every product of any derivative order is either
-    tall-by-small or split over K."""
+    tall-by-small or split over K. hip: the products x @ W that qualify run on matpl_tall_mm."""
 
     @staticmethod
-    def forward(ctx, x, W):
+    def forward(ctx, x, W, hip):
         ctx.save_for_backward(x, W)
-        return _small_mm(x, W, False) if _small(x, W) else x @ W
+        ctx.hip = hip
+        return _small_mm(x, W, False) if hip and _small(x, W) else x @ W
 
     @staticmethod
     def backward(ctx, g):
         x, W = ctx.saved_tensors
-        gx = _TallMM.apply(g, W.t()) if ctx.needs_input_grad[0] else None
+        gx = _TallMM.apply(g, W.t(), ctx.hip) if ctx.needs_input_grad[0] else None
         gW = _tall_tn(x, g) if ctx.needs_input_grad[1] else None
-        return gx, gW
+        return gx, gW, None
 
 
 def tall_mm(x, W):
     """x @ W for x [..., a] (any leading shape) and W [a, b]."""
     lead = x.shape[:-1]
-    return _TallMM.apply(x.reshape(-1, x.shape[-1]), W).reshape(*lead, W.shape[-1])
+    return _TallMM.apply(x.reshape(-1, x.shape[-1]), W, False).reshape(*lead, W.shape[-1])
+
+
+def tall_mm_f32(x, W):
+    """tall_mm whose FP32 products with a, b <= 64 run on matpl_tall_mm (for row counts in the
+    hundreds of thousands, as in the batched DP training step; torch's product is quicker for a few
+    thousand rows)."""
+    lead = x.shape[:-1]
+    return _TallMM.apply(x.reshape(-1, x.shape[-1]), W, True).reshape(*lead, W.shape[-1])
 
 
 class _TallMMLow(torch.autograd.Function):

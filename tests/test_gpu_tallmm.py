@@ -2,12 +2,12 @@
 Bounds: an FP32 sum of a products is within a * 2^-23 * sum|x_j W_jc| of the exact value (FP32 mode);
 in lowp mode the operands are bfloat16 values, the result is a bfloat16 value within one bfloat16 ulp
 plus that FP32 sum bound of the exact product, and within two ulps plus twice the bound of torch's BF16
-product with FP32 accumulation. Ragged row counts, a, b from 1 to 64; autograd through tall_mm /
-tall_mm_bf16 (first and second order) matches the float64 product's."""
+product with FP32 accumulation. Ragged row counts, a, b from 1 to 64; autograd through
+tall_mm_f32 / tall_mm_bf16 (first and second order) matches the float64 product's."""
 import pytest
 import torch
 
-from matpl_amd.tallmm import _small_mm, tall_mm, tall_mm_bf16
+from matpl_amd.tallmm import _small_mm, tall_mm_bf16, tall_mm_f32
 
 pytestmark = pytest.mark.gpu
 
@@ -50,7 +50,7 @@ def test_bf16_matches_torch_bf16(dev, K, a, b):
     assert bool(((y.double() - tb.double()).abs() <= 2 * ulp + 2 * sumerr).all())
 
 
-@pytest.mark.parametrize("fn,tol", [(tall_mm, 1e-5), (tall_mm_bf16, 3e-2)])
+@pytest.mark.parametrize("fn,tol", [(tall_mm_f32, 1e-5), (tall_mm_bf16, 3e-2)])
 def test_autograd_matches_torch(dev, fn, tol):
     print(f"case tall_mm autograd {fn.__name__}", flush=True)
     g = torch.Generator(device="cpu").manual_seed(5)

@@ -12,7 +12,7 @@ def test_tall_products_match_plain_autograd():
         x = torch.randn(K, 7, dtype=torch.float64, requires_grad=True)
         W = torch.randn(7, 5, dtype=torch.float64, requires_grad=True)
         outs = []
-        for f in (lambda a, b: _TallMM.apply(a, b), lambda a, b: a @ b):
+        for f in (lambda a, b: _TallMM.apply(a, b, False), lambda a, b: a @ b):
             xx, WW = x.detach().clone().requires_grad_(), W.detach().clone().requires_grad_()
             y = torch.tanh(f(xx, WW))
             (gx,) = torch.autograd.grad(y.pow(2).sum(), xx, create_graph=True)
@@ -22,7 +22,7 @@ def test_tall_products_match_plain_autograd():
             assert torch.allclose(a, b, rtol=1e-12, atol=1e-10)
     xs = torch.randn(2 * _SPLIT_ROWS + 9, 3, dtype=torch.float64, requires_grad=True)
     Ws = torch.randn(3, 2, dtype=torch.float64, requires_grad=True)
-    assert torch.autograd.gradgradcheck(lambda a, b: _TallMM.apply(a, b), (xs, Ws))
+    assert torch.autograd.gradgradcheck(lambda a, b: _TallMM.apply(a, b, False), (xs, Ws))
 
 
 def test_bias_add_and_column_sum():
