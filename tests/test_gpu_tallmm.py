@@ -65,7 +65,7 @@ def test_autograd_matches_torch(dev, fn, tol):
     gx2, gW2 = torch.autograd.grad((y2 * up.double()).sum(), (x2, W2), create_graph=True)
     ggW2 = torch.autograd.grad((gx2 ** 2).sum(), W2)[0]
     for a, b in ((y, y2), (gx, gx2), (gW, gW2), (ggW, ggW2)):
-        assert float((a.double() - b).norm() / b.norm()) <= tol
+        assert float((a.detach().double() - b.detach()).norm() / b.detach().norm()) <= tol
 
 
 @pytest.mark.parametrize("lowp", [False, True])
