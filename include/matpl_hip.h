@@ -196,6 +196,12 @@ int matpl_bsmm(int f64, int mode, const void *P1, const void *P2, void *out, int
  * frame) give the normalized Ri (FP32), the embedding net output G on the
  * matrix cores (bf16 != 0: BF16 in / FP32 accumulate; 0: FP32), and
  * A[i] = scale * (sum Ri^T G + empty-slot term) [N][4][E], E in {16,25,32}.
+ * frame_types [F] (uint32, or NULL): bit t set when type t is in the frame's
+ * atom_type list. Only those types' buckets add an empty-slot term, and the
+ * frame's scale is 1 / (m * popcount) (dp_dp.py:150-165, 197, 413); every
+ * type of a frame's atoms must be in its list, and no list may be empty.
+ * With NULL every model type takes part and scale (= 1 / (m * ntypes)) is
+ * used.
  * vec/mat/tinfo are the packed net parameters (matpl_amd/dp_fused.py).
  * Backward: given dA = dEtot/dA, force [N][3] (float) receives -dE/dx and
  * virial [F][9] (float64, accumulated: zero it first) the
@@ -214,12 +220,14 @@ int matpl_bsmm(int f64, int mode, const void *P1, const void *P2, void *out, int
  * with more than m in-cutoff neighbors of one type.
  */
 int matpl_dp_fused_forward(const double *pos, const double *lattice, const int32_t *atom_frame,
-                           const int64_t *frame_start, const int32_t *list, const int32_t *shift,
+                           const int64_t *frame_start, const uint32_t *frame_types, const int32_t *list,
+                           const int32_t *shift,
                            const int32_t *counts, const int32_t *type_map, const float *vec, const void *mat,
                            const float *tinfo, int64_t natoms, int ntypes, int mnl, int m, int E, int bf16,
                            double rcut, double rcut_smooth, double scale, float *A, int32_t *status, void *stream);
 int matpl_dp_fused_backward(const double *pos, const double *lattice, const int32_t *atom_frame,
-                            const int64_t *frame_start, const int32_t *list, const int32_t *shift,
+                            const int64_t *frame_start, const uint32_t *frame_types, const int32_t *list,
+                           const int32_t *shift,
                             const int32_t *counts, const int32_t *type_map, const float *vec, const void *mat,
                             const float *tinfo, int64_t natoms, int ntypes, int mnl, int m, int E, int bf16,
                             double rcut, double rcut_smooth, double scale, const float *dA, const int32_t *rev,
@@ -238,14 +246,16 @@ int matpl_dp_fused_backward(const double *pos, const double *lattice, const int3
  * matpl_dp_fused_forward / matpl_dp_fused_backward.
  */
 int matpl_dp_fused_forward_compressed(const double *pos, const double *lattice, const int32_t *atom_frame,
-                                      const int64_t *frame_start, const int32_t *list, const int32_t *shift,
+                                      const int64_t *frame_start, const uint32_t *frame_types, const int32_t *list,
+                           const int32_t *shift,
                                       const int32_t *counts, const int32_t *type_map, const float *vec,
                                       const float *ctab, int order, int rows, int n_fine, double x0, double dx,
                                       const float *tinfo, int64_t natoms, int ntypes, int mnl, int m, int E, int bf16,
                                       double rcut, double rcut_smooth, double scale, float *A, int32_t *status,
                                       void *stream);
 int matpl_dp_fused_backward_compressed(const double *pos, const double *lattice, const int32_t *atom_frame,
-                                       const int64_t *frame_start, const int32_t *list, const int32_t *shift,
+                                       const int64_t *frame_start, const uint32_t *frame_types, const int32_t *list,
+                           const int32_t *shift,
                                        const int32_t *counts, const int32_t *type_map, const float *vec,
                                        const float *ctab, int order, int rows, int n_fine, double x0, double dx,
                                        const float *tinfo, int64_t natoms, int ntypes, int mnl, int m, int E,

@@ -11,12 +11,13 @@ change their speed but not their results:
 ``tune_fused`` times each candidate on the caller's own workload (coordinate
 descent: the forward kernel's shape, then the backward's), keeps the fastest,
 sets it in the library (matpl_dp_fused_set_launch, process-wide) and caches it
-in a JSON file keyed by device, net width, types, list width, atom count
-(rounded to a power of two) and precision. A later call with the same key
+in a JSON file keyed by device, library build (content hash), net width, types,
+list width, atom count (rounded to a power of two) and precision. A later call with the same key
 reuses the cached shape without timing.
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import math
 import os
@@ -65,10 +66,26 @@ def parse_launch(spec: str) -> Dict:
     return {"forward": f, "backward": b, "fixed": True}
 
 
+_BUILD_ID = None
+
+
+def build_id() -> str:
+    """Content hash of the loaded libmatpl_hip.so: a shape tuned for one build of the fused kernels
+    is not reused for another."""
+    global _BUILD_ID
+    if _BUILD_ID is None:
+        h = hashlib.sha1()
+        with open(_lib.LIB_PATH, "rb") as f:
+            for chunk in iter(lambda: f.read(1 << 20), b""):
+                h.update(chunk)
+        _BUILD_ID = h.hexdigest()[:12]
+    return _BUILD_ID
+
+
 def workload_key(E: int, T: int, mnl: int, m: int, natoms: int, precision: str) -> str:
     dev = device_id()
     nb = 1 << max(0, int(round(math.log2(max(natoms, 1)))))
-    return f"{dev}|E{E}|T{T}|mnl{mnl}|m{m}|n~{nb}|{precision}"
+    return f"{dev}|lib{build_id()}|E{E}|T{T}|mnl{mnl}|m{m}|n~{nb}|{precision}"
 
 
 def _load() -> Dict:

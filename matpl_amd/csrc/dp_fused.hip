@@ -18,6 +18,11 @@
 //         form (m - n_t) * [s_pad,0,0,0]^T G_{ti,tj}(s_pad), s_pad =
 //         -davg_0/dstd_0 (dp_dp.py:293-297 normalizes empty slots too).
 //         Out: A [N][4][E] scaled by 1/(m * type_nums) (dp_dp.py:413).
+//         type_nums and the buckets that take part are per frame: the
+//         reference forms the (ti, tj) pairs only over the types of the
+//         frame's atom_type list (dp_dp.py:150-165, 197), so a type absent
+//         from that list adds no empty-slot term and does not count in
+//         type_nums (frame_types; without it every model type takes part).
 //   bwd : given dA = dE/dA, recomputes the embedding forward for each tile,
 //         back-propagates through the net (same matrix-core layout), forms
 //         dE/dRi per neighbor, contracts it with the analytic dRi/dx
@@ -81,6 +86,7 @@ struct FusedParams {
     const double *lattice;      // [F][9], row d = lattice vector d
     const int32_t *atom_frame;  // [N] frame of each atom (nullptr: all frame 0)
     const int64_t *frame_start; // [F+1] first atom of each frame (nullptr: 0)
+    const uint32_t *frame_types;// [F] bit t set when type t is in the frame's type list (nullptr: all T)
     const int32_t *list;        // [N][T][mnl] 1-based within frame, 0 = empty
     const int32_t *shift;       // [N][T][mnl] packed image
     const int32_t *counts;      // [N][T] entries used per bucket
@@ -100,7 +106,7 @@ struct FusedParams {
     int T, mnl, m;
     int cap;                    // LDS tile capacity per type = min(m, mnl)
     double rc2;                 // model cutoff^2 (FP64 test)
-    float rmin, rmax, scale;    // scale = 1 / (m * type_nums)
+    float rmin, rmax, scale;    // scale = 1 / (m * T), used when frame_types is null
     float iw;                   // 1 / (rmax - rmin)
     int xcd;                    // 1: XCD-aware block order (grid a multiple of NXCD)
     // compressed embedding (table mode): G(s) by piecewise polynomials, K coefficients per
@@ -438,6 +444,18 @@ __device__ __forceinline__ bool compact(const FusedParams &p, const AtomGeo &g, 
     return ok_all;
 }
 
+// The type list of atom i's frame: the mask of the neighbor types whose buckets take part, and the
+// descriptor scale 1 / (m * type_nums) of dp_dp.py:413 (type_nums = types in the list).
+__device__ __forceinline__ uint32_t frame_type_mask(const FusedParams &p, int64_t i, float &scale) {
+    if (!p.frame_types) {
+        scale = p.scale;
+        return 0xffffffffu;
+    }
+    const uint32_t mk = p.frame_types[p.atom_frame ? p.atom_frame[i] : 0] & ((1u << p.T) - 1u);
+    scale = 1.0f / (float)(p.m * __popc(mk));
+    return mk;
+}
+
 // Normalized Ri of a compacted neighbor (FP32).
 __device__ __forceinline__ void ri_of(const float4 e, const float *ti, float rmin, float rmax, float iw, float Ri[4],
                                       float &r, float &s, float &ds) {
@@ -700,8 +718,11 @@ dp_fused_fwd_kernel(FusedParams p) {
             }
         }
         if (f >= 0) {
+            float scale;
+            const uint32_t tmask = frame_type_mask(p, i, scale);
             float pad = 0.0f;
             for (int t = 0; t < p.T; t++) {
+                if (!((tmask >> t) & 1u)) continue;     // the bucket of a type outside the frame's list
                 const float npad = (float)(p.m - nvs[t]);
                 pad += npad * vec_ptr(L.vec, ti * p.T + t, V_GPAD, h_f)[rr_f];
             }
@@ -709,7 +730,7 @@ dp_fused_fwd_kernel(FusedParams p) {
             for (int w = 0; w < 4; w++) out[w] += tinfo[TI_SPAD + w] * pad;
             float *Ao = p.A + (size_t)i * 4 * E;
 #pragma unroll
-            for (int w = 0; w < 4; w++) Ao[w * E + f] = out[w] * p.scale;
+            for (int w = 0; w < 4; w++) Ao[w * E + f] = out[w] * scale;
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -770,8 +791,10 @@ dp_fused_bwd_kernel(FusedParams p) {
             }
         }
         compact(p, g, i, lane, nb, nvs, p.fslot);
+        float scale;
+        frame_type_mask(p, i, scale);
 #pragma unroll
-        for (int u = 0; u < 2; u++) reinterpret_cast<float *>(dAs)[lane + u * WAVE] = dav[u] * p.scale;
+        for (int u = 0; u < 2; u++) reinterpret_cast<float *>(dAs)[lane + u * WAVE] = dav[u] * scale;
         __builtin_amdgcn_wave_barrier();
         // A operands of dG = dA Ri^T (32 features x 4, K split in two steps of 2): lane (n, h)
         // holds dA[n][h] and dA[n][2 + h]
@@ -1037,13 +1060,15 @@ int dispatch(FusedParams &p, int E, int bf16, bool fwd, hipStream_t st, int ck =
 using namespace matpl;
 
 MATPL_EXPORT int matpl_dp_fused_forward(const double *pos, const double *lattice, const int32_t *atom_frame,
-                                        const int64_t *frame_start, const int32_t *list, const int32_t *shift,
+                                        const int64_t *frame_start, const uint32_t *frame_types,
+                                        const int32_t *list, const int32_t *shift,
                                         const int32_t *counts, const int32_t *type_map, const float *vec,
                                         const void *mat, const float *tinfo, int64_t natoms, int ntypes, int mnl,
                                         int m, int E, int bf16, double rcut, double rcut_smooth, double scale,
                                         float *A, int32_t *status, void *stream) {
     FusedParams p{};
     p.pos = pos; p.lattice = lattice; p.atom_frame = atom_frame; p.frame_start = frame_start;
+    p.frame_types = frame_types;
     p.list = list; p.shift = shift; p.counts = counts; p.type_map = type_map;
     p.vec = vec; p.mat = mat; p.tinfo = tinfo; p.A = A; p.status = status;
     p.natoms = natoms; p.T = ntypes; p.mnl = mnl; p.m = m;
@@ -1055,7 +1080,8 @@ MATPL_EXPORT int matpl_dp_fused_forward(const double *pos, const double *lattice
 }
 
 MATPL_EXPORT int matpl_dp_fused_backward(const double *pos, const double *lattice, const int32_t *atom_frame,
-                                         const int64_t *frame_start, const int32_t *list, const int32_t *shift,
+                                         const int64_t *frame_start, const uint32_t *frame_types,
+                                         const int32_t *list, const int32_t *shift,
                                          const int32_t *counts, const int32_t *type_map, const float *vec,
                                          const void *mat, const float *tinfo, int64_t natoms, int ntypes, int mnl,
                                          int m, int E, int bf16, double rcut, double rcut_smooth, double scale,
@@ -1063,6 +1089,7 @@ MATPL_EXPORT int matpl_dp_fused_backward(const double *pos, const double *lattic
                                          float *force, double *virial, int32_t *status, void *stream) {
     FusedParams p{};
     p.pos = pos; p.lattice = lattice; p.atom_frame = atom_frame; p.frame_start = frame_start;
+    p.frame_types = frame_types;
     p.list = list; p.shift = shift; p.counts = counts; p.type_map = type_map;
     p.vec = vec; p.mat = mat; p.tinfo = tinfo; p.dA = dA; p.force = force; p.virial = virial; p.status = status;
     p.natoms = natoms; p.T = ntypes; p.mnl = mnl; p.m = m;
@@ -1098,7 +1125,8 @@ static int set_table(FusedParams &p, const float *ctab, int order, int rows, int
 
 MATPL_EXPORT int matpl_dp_fused_forward_compressed(
     const double *pos, const double *lattice, const int32_t *atom_frame, const int64_t *frame_start,
-    const int32_t *list, const int32_t *shift, const int32_t *counts, const int32_t *type_map, const float *vec,
+    const uint32_t *frame_types, const int32_t *list, const int32_t *shift, const int32_t *counts,
+    const int32_t *type_map, const float *vec,
     const float *ctab, int order, int rows, int n_fine, double x0, double dx, const float *tinfo, int64_t natoms,
     int ntypes, int mnl, int m, int E, int bf16, double rcut, double rcut_smooth, double scale, float *A,
     int32_t *status, void *stream) {
@@ -1106,6 +1134,7 @@ MATPL_EXPORT int matpl_dp_fused_forward_compressed(
     int rc = set_table(p, ctab, order, rows, n_fine, x0, dx, E);
     if (rc) return rc;
     p.pos = pos; p.lattice = lattice; p.atom_frame = atom_frame; p.frame_start = frame_start;
+    p.frame_types = frame_types;
     p.list = list; p.shift = shift; p.counts = counts; p.type_map = type_map;
     p.vec = vec; p.tinfo = tinfo; p.A = A; p.status = status;
     p.natoms = natoms; p.T = ntypes; p.mnl = mnl; p.m = m;
@@ -1118,7 +1147,8 @@ MATPL_EXPORT int matpl_dp_fused_forward_compressed(
 
 MATPL_EXPORT int matpl_dp_fused_backward_compressed(
     const double *pos, const double *lattice, const int32_t *atom_frame, const int64_t *frame_start,
-    const int32_t *list, const int32_t *shift, const int32_t *counts, const int32_t *type_map, const float *vec,
+    const uint32_t *frame_types, const int32_t *list, const int32_t *shift, const int32_t *counts,
+    const int32_t *type_map, const float *vec,
     const float *ctab, int order, int rows, int n_fine, double x0, double dx, const float *tinfo, int64_t natoms,
     int ntypes, int mnl, int m, int E, int bf16, double rcut, double rcut_smooth, double scale, const float *dA,
     const int32_t *rev, float *fslot, float *fcenter, float *force, double *virial, int32_t *status, void *stream) {
@@ -1126,6 +1156,7 @@ MATPL_EXPORT int matpl_dp_fused_backward_compressed(
     int rc = set_table(p, ctab, order, rows, n_fine, x0, dx, E);
     if (rc) return rc;
     p.pos = pos; p.lattice = lattice; p.atom_frame = atom_frame; p.frame_start = frame_start;
+    p.frame_types = frame_types;
     p.list = list; p.shift = shift; p.counts = counts; p.type_map = type_map;
     p.vec = vec; p.tinfo = tinfo; p.dA = dA; p.force = force; p.virial = virial; p.status = status;
     p.natoms = natoms; p.T = ntypes; p.mnl = mnl; p.m = m;

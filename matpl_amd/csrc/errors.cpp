@@ -19,4 +19,4 @@ void set_error(const char *fmt, ...) {
 
 MATPL_EXPORT const char *matpl_last_error(void) { return matpl::g_error; }
 
-MATPL_EXPORT int matpl_abi_version(void) { return 1; }
+MATPL_EXPORT int matpl_abi_version(void) { return 2; }

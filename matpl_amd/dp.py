@@ -10,8 +10,6 @@ baseline the fused path is measured against.
 """
 from __future__ import annotations
 
-from typing import List, Optional, Tuple
-
 import torch
 
 from . import calcops
@@ -86,15 +84,25 @@ class DPTorch(torch.nn.Module):
         return calculateCompress(f2, coef)[0].reshape(Rt.shape[:-1] + (-1,))
 
     def forward(self, list_neigh: torch.Tensor, Imagetype_map: torch.Tensor, ImageDR: torch.Tensor,
-                nghost: int = 0, type_present: Optional[List[int]] = None, create_graph: bool = True):
+                nghost: int = 0, type_present=None, create_graph: bool = True):
         """list_neigh [B,N,S] 1-based; Imagetype_map [N]; ImageDR [B,N,S,4].
+
+        type_present: the image's atom_type list as T flags (dp_dp.py:150-165, 197; None = every
+        model type), or, for frames packed along the atom axis, a [N, T] bool tensor holding each
+        atom's frame's flags; only the listed types' buckets take part and A is divided by m
+        times their number (dp_dp.py:413).
 
         Returns Etot [B,1], Ei [B,N], Force [B,N+nghost,3], Virial [B,9].
         """
         cfg = self.cfg
         B, N, S = list_neigh.shape
         T, m = cfg.ntypes, cfg.m
-        present = list(range(T)) if type_present is None else [t for t in range(T) if type_present[t]]
+        per_atom = isinstance(type_present, torch.Tensor) and type_present.dim() == 2
+        if per_atom:
+            amask = type_present.to(device=ImageDR.device, dtype=torch.bool)
+            present = [t for t in range(T) if bool(amask[:, t].any())]
+        else:
+            present = list(range(T)) if type_present is None else [t for t in range(T) if type_present[t]]
         Ri, Ri_d = env_mat(ImageDR.to(self.dtype), Imagetype_map, self.davg_full, self.dstd_full,
                            cfg.rcut_smooth, cfg.rcut)
         Ri.requires_grad_()
@@ -109,8 +117,13 @@ class DPTorch(torch.nn.Module):
                 Rt = Ri[:, idx, tj * m:(tj + 1) * m]              # [B, n, m, 4]
                 G = self.embed(ti, tj, Rt)                                # [B, n, m, E]
                 part = Rt.transpose(-2, -1) @ G                     # [B, n, 4, E]
+                if per_atom:
+                    part = part * amask[idx, tj].to(self.dtype).view(1, -1, 1, 1)
                 A = part if A is None else A + part
-            A = A / (m * len(present))
+            if per_atom:
+                A = A / (m * amask[idx].sum(1).to(self.dtype)).view(1, -1, 1, 1)
+            else:
+                A = A / (m * len(present))
             DR = A.transpose(-2, -1) @ A[..., :cfg.M2]              # [B, n, E, M2]
             Ei_t = self.fit[ti](DR.reshape(B, idx.numel(), -1), self.mm, self.add)   # [B, n, 1]
             Ei = Ei.index_copy(1, idx, Ei_t)

@@ -136,6 +136,11 @@ def write_dp_checkpoint(w: DPWeights, atom_types: Sequence[int], path: str,
                 sd[f"{name}.{p}.layers.{l}.weight"] = t(W)
                 if b is not None:
                     sd[f"{name}.{p}.layers.{l}.bias"] = t(b).reshape(1, -1)
+                # EmbeddingNet registers a resnet_dt on every layer when the flag is set
+                # (dp_embedding.py:65-70), also where its forward never reads it (the 1 -> E layer):
+                # those get ones, the mean of MatPL's initialisation there
+                if r is None and name == "embedding_net" and cfg.emb_resnet_dt:
+                    r = np.ones(W.shape[1])
                 if r is not None:
                     sd[f"{name}.{p}.layers.{l}.resnet_dt"] = t(r).reshape(1, -1)
     if energy_shift is None:

@@ -37,6 +37,26 @@ def _get(W, rows, cols):
     return out
 
 
+def segments(type_ranges):
+    """[(type, first, end)] runs of atoms of one type: from [(first, end)] per type (index = type),
+    or as given when already triples (frames packed along the atom axis, each sorted by type)."""
+    rs = list(type_ranges)
+    if rs and len(rs[0]) == 3:
+        return [tuple(int(x) for x in r) for r in rs]
+    return [(t, int(a), int(b)) for t, (a, b) in enumerate(rs)]
+
+
+def type_runs(type_map: np.ndarray):
+    """[(type, first, end)]: the maximal runs of equal type in a per-atom type array."""
+    tm = np.asarray(type_map).reshape(-1)
+    if tm.size == 0:
+        return []
+    cut = np.nonzero(np.diff(tm))[0] + 1
+    starts = np.concatenate([[0], cut])
+    ends = np.concatenate([cut, [tm.size]])
+    return [(int(tm[a]), int(a), int(b)) for a, b in zip(starts, ends)]
+
+
 class FittingMFMA:
     def __init__(self, w: DPWeights, type_ranges, precision="bf16", device="cuda"):
         cfg = w.cfg
@@ -107,7 +127,7 @@ class FittingMFMA:
         self.vec = torch.as_tensor(vec.astype(np.float32), device=device).contiguous()
         self.strides = (w0f[0].size, w0b[0].size, whf[0].size, vec[0].size)
         tiles = []
-        for t, (a, b) in enumerate(type_ranges):
+        for t, a, b in segments(type_ranges):
             for s in range(a, b, 32):
                 tiles.append((t, s, min(32, b - s)))
         self.ntiles = len(tiles)

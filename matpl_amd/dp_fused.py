@@ -23,7 +23,7 @@ import torch
 
 from . import _lib
 from .calcops import _ptr, _stream
-from .dp_fitting import FittingMFMA
+from .dp_fitting import FittingMFMA, segments, type_runs
 from .dpnet import DPWeights
 
 NVEC, NMAT, NTI = 7, 4, 16
@@ -171,7 +171,7 @@ class _Fitting:
         A = A.detach().requires_grad_()
         Ei = torch.empty(A.shape[0], dtype=A.dtype, device=A.device)
         parts = []
-        for t, (a, b) in enumerate(type_ranges):
+        for t, a, b in segments(type_ranges):
             if b <= a:
                 continue
             At = A[a:b]
@@ -193,10 +193,36 @@ class _Fitting:
         return Ei, dA
 
 
+def frame_types_of(type_map: torch.Tensor, atom_frame: Optional[torch.Tensor], nframes: int, ntypes: int):
+    """[F, T] bool: the types that occur in each frame, i.e. the per-image atom_type list the
+    reference's data loader hands DP.forward (dp_dp.py:150-165, 197)."""
+    tm = type_map.reshape(-1).to(torch.int64)
+    fr = torch.zeros_like(tm) if atom_frame is None else atom_frame.reshape(-1).to(torch.int64)
+    occ = torch.zeros(nframes * ntypes, dtype=torch.bool, device=tm.device)
+    occ[fr * ntypes + tm] = True
+    return occ.view(nframes, ntypes)
+
+
+def type_masks(type_present, nframes: int, ntypes: int) -> np.ndarray:
+    """[F] uint32 bit masks from a [T] (every frame) or [F, T] presence table."""
+    tp = np.asarray(type_present.cpu() if isinstance(type_present, torch.Tensor) else type_present).astype(bool)
+    if tp.ndim == 1:
+        tp = np.broadcast_to(tp, (nframes, tp.shape[0]))
+    if tp.shape != (nframes, ntypes):
+        raise ValueError(f"type_present must be [{ntypes}] or [{nframes}, {ntypes}], got {tp.shape}")
+    masks = (tp.astype(np.uint64) << np.arange(ntypes, dtype=np.uint64)).sum(1).astype(np.uint32)
+    if (masks == 0).any():
+        raise ValueError("type_present: every frame needs at least one type in its list")
+    return masks
+
+
 class FusedDP:
     """DP E/F/V for one (or a batch of) frame(s) given a neighbor list.
 
-    Atoms must be grouped by type (type ranges are contiguous).
+    Frames are packed along the atom axis, each frame's atoms contiguous. The fitting runs over
+    runs of atoms of one type (``type_ranges``: [(first, end)] per type, or [(type, first, end)]
+    runs; default the runs of ``type_map``), so atoms grouped by type within each frame keep
+    its tiles full.
     """
 
     def __init__(self, w: DPWeights, precision: str = "bf16", device="cuda", fitting: str = "mfma", compress=None):
@@ -226,29 +252,33 @@ class FusedDP:
         self._mfma_fit = {}
         self.precision = precision
         self.events = None      # when a list: (start, end) HIP events around each backward launch
+        self._masks = {}        # checked per-frame type masks on the device, by content and atoms
 
     def __call__(self, pos: torch.Tensor, lattice: torch.Tensor, type_map: torch.Tensor, nl_list: torch.Tensor,
                  nl_shift: torch.Tensor, nl_counts: torch.Tensor, mnl: int, atom_frame: Optional[torch.Tensor] = None,
                  frame_start: Optional[torch.Tensor] = None, nframes: int = 1, type_ranges=None,
-                 rev: Optional[torch.Tensor] = None):
+                 rev: Optional[torch.Tensor] = None, type_present=None):
         """Returns (Ei [N], F [N,3] float32, V [nframes, 9] float64, status).
 
         With ``rev`` (neighbor.reverse_map) the neighbor forces are gathered
         without atomics; otherwise they are scattered with atomics.
+        ``type_present`` ([T] or [nframes, T] bools, e.g. ``frame_types_of``) is each frame's
+        atom_type list (dp_dp.py:150-165, 197): only its types' buckets take part and the
+        descriptor is divided by m times their number (dp_dp.py:413). None: every model type
+        (the LAMMPS pair style's use, where the list is the model's).
         """
         cfg = self.cfg
         N = pos.shape[0]
         T = cfg.ntypes
         dev = pos.device
         if type_ranges is None:
-            tm = type_map.cpu().numpy()
-            type_ranges = [(int(np.searchsorted(tm, t, "left")), int(np.searchsorted(tm, t, "right")))
-                           for t in range(T)]
+            type_ranges = type_runs(type_map.cpu().numpy())
         scale = 1.0 / (cfg.m * T)
+        masks = None if type_present is None else self._type_masks(type_present, type_map, atom_frame, nframes)
         A = torch.empty((N, 4, self.fw.E), dtype=torch.float32, device=dev)
         status = torch.zeros(1, dtype=torch.int32, device=dev)
-        head = (_ptr(pos), _ptr(lattice), _ptr(atom_frame), _ptr(frame_start), _ptr(nl_list), _ptr(nl_shift),
-                _ptr(nl_counts), _ptr(type_map), _ptr(self.fw.vec))
+        head = (_ptr(pos), _ptr(lattice), _ptr(atom_frame), _ptr(frame_start), _ptr(masks), _ptr(nl_list),
+                _ptr(nl_shift), _ptr(nl_counts), _ptr(type_map), _ptr(self.fw.vec))
         tail = (_ptr(self.fw.tinfo), N, T, mnl, cfg.m, self.fw.E, int(self.fw.bf16), float(cfg.rcut),
                 float(cfg.rcut_smooth), scale)
         if self.table is None:
@@ -258,7 +288,7 @@ class FusedDP:
             args, sfx = head + (_ptr(tb.tab), tb.order, tb.rows, tb.n_fine, tb.x0, tb.dx) + tail, "_compressed"
         _lib.call("matpl_dp_fused_forward" + sfx, *args, _ptr(A), _ptr(status), _stream())
         if self.fitting == "mfma":
-            key = tuple(type_ranges)
+            key = tuple(segments(type_ranges))
             if key not in self._mfma_fit:
                 self._mfma_fit[key] = FittingMFMA(self.w, type_ranges, self.precision, dev)
             Ei, dA = self._mfma_fit[key](A)
@@ -286,3 +316,21 @@ class FusedDP:
             _lib.call("matpl_dp_fused_gather", _ptr(nl_counts), _ptr(fslot), _ptr(fcenter), _ptr(F), T, mnl, N,
                       _stream())
         return Ei, F, V, status
+
+    def _type_masks(self, type_present, type_map, atom_frame, nframes):
+        """The per-frame type masks on the device. The first call with a given table and atom
+        arrays checks that every type of a frame's atoms is in its list (the reference has no
+        rule for atoms outside it); later calls reuse the checked tensor without a host sync."""
+        masks = type_masks(type_present, nframes, self.cfg.ntypes)
+        key = (masks.tobytes(), type_map.data_ptr(), None if atom_frame is None else atom_frame.data_ptr(),
+               type_map.numel())
+        t = self._masks.get(key)
+        if t is None:
+            occ = frame_types_of(type_map, atom_frame, nframes, self.cfg.ntypes).cpu().numpy()
+            bits = (occ.astype(np.uint64) << np.arange(self.cfg.ntypes, dtype=np.uint64)).sum(1).astype(np.uint32)
+            bad = np.nonzero(bits & ~masks)[0]
+            if bad.size:
+                raise ValueError(f"type_present: frame {int(bad[0])} has atoms of a type outside its list")
+            t = torch.as_tensor(masks.view(np.int32), device=type_map.device)
+            self._masks = {key: t}
+        return t

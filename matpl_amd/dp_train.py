@@ -15,8 +15,8 @@ differentiates through all of it. ``DPTorch`` in FP64 is that path here (the pla
     Ri_pad = -davg/dstd, and are added back in closed form: (m - mc) Ri_pad^T G(Ri_pad[0]) per
     bucket (the same sum the reference forms slot by slot);
   * descriptors Ri, Ri_d in FP32 from the HIP env-mat kernel;
-  * embedding and fitting products in BF16 on the matrix cores with FP32 accumulation and FP32
-    activations ("bf16"), or all in FP32 ("fp32");
+  * embedding and fitting layer products on BF16-rounded operands with FP32 VALU FMA
+    accumulation and FP32 activations ("bf16"), or all in FP32 ("fp32");
   * the per-atom contractions A = Ri^T G and DR = A^T A' (and all their derivatives) on the
     batched small-product kernels (matpl_amd.bsmm);
   * the centre force by a HIP reduction; neighbor forces by CalcOps_cuda.calculateForce, whose
@@ -57,11 +57,18 @@ class TrainBatch:
     F_label: torch.Tensor        # [N, 3] float64
     list_full: torch.Tensor = None       # [1, N, T*m] (kept for the plain path)
     image_dr_full: torch.Tensor = None
+    type_present: torch.Tensor = None    # [F, T] bool: each frame's atom_type list (dp_dp.py:150-165)
+
+    def atom_type_present(self) -> torch.Tensor:
+        """[N, T] bool: the type list of each atom's frame."""
+        return self.type_present[self.frame]
 
 
-def pack_batch(frames, cfg, device="cuda", seed=0, keep_full=False) -> TrainBatch:
+def pack_batch(frames, cfg, device="cuda", seed=0, keep_full=False, type_present=None) -> TrainBatch:
     """Neighbor lists of all frames (one batched launch), packed with global indices, and
-    seeded synthetic labels (energies ~ N(0, 0.1) eV/atom, forces ~ N(0, 0.5) eV/A)."""
+    seeded synthetic labels (energies ~ N(0, 0.1) eV/atom, forces ~ N(0, 0.5) eV/A).
+    type_present [F, T]: each frame's atom_type list; default the types of its atoms, as the
+    reference's loader gives it per image."""
     T, m = cfg.ntypes, cfg.m
     nl = find_neighbore_batch([f.lattice for f in frames], [f.frac() for f in frames], [f.types for f in frames],
                               T, m, cfg.rcut, device=device)
@@ -81,6 +88,16 @@ def pack_batch(frames, cfg, device="cuda", seed=0, keep_full=False) -> TrainBatc
                    [torch.nonzero(tm == t).reshape(-1) for t in range(T)], mc,
                    glob[:, :, :mc].reshape(1, N, T * mc).contiguous(),
                    idr[:, :, :mc].reshape(1, N, T * mc, 4).contiguous(), E_label, F_label)
+    if type_present is None:
+        tp = np.zeros((len(frames), T), bool)
+        for k, f in enumerate(frames):
+            tp[k, np.unique(f.types)] = True
+    else:
+        tp = np.asarray(type_present, bool).reshape(len(frames), T)
+        for k, f in enumerate(frames):
+            if not tp[k, np.unique(f.types)].all():
+                raise ValueError(f"type_present: frame {k} has atoms of a type outside its list")
+    b.type_present = torch.as_tensor(tp, device=device)
     if keep_full:
         b.list_full = glob.reshape(1, N, T * m).contiguous()
         b.image_dr_full = idr.reshape(1, N, T * m, 4).contiguous()
@@ -121,12 +138,20 @@ class DPTrain(torch.nn.Module):
         Ri, Ri_d = env_mat(batch.image_dr_c.to(torch.float32), batch.type_map, davg_c, dstd_c, cfg.rcut_smooth,
                            cfg.rcut)
         Ri.requires_grad_()
-        present = [t for t in range(T) if batch.by_type[t].numel()]
+        # each frame's type list (dp_dp.py:150-165): a bucket takes part only for atoms whose frame
+        # lists its type, and A is divided by m times the frame's number of listed types (:413)
+        tp = batch.type_present
+        listed = [t for t in range(T) if bool(tp[:, t].any())]
+        every = [bool(tp[:, t].all()) for t in range(T)]
+        amask = batch.atom_type_present().to(torch.float32)                    # [N, T]
+        inv = 1.0 / (m * amask.sum(1))                                           # [N]
         Ei = torch.zeros(N, dtype=torch.float32, device=Ri.device)
-        for ti in present:
+        for ti in range(T):
             idx = batch.by_type[ti]
+            if idx.numel() == 0:
+                continue
             A = None
-            for tj in present:
+            for tj in listed:
                 net = self.emb[ti * T + tj]
                 Rt = Ri[0, idx, tj * mc:(tj + 1) * mc]                      # [n, mc, 4]
                 G = net(Rt[..., 0:1], self.mm, add_bias)                             # [n, mc, E]
@@ -134,8 +159,10 @@ class DPTrain(torch.nn.Module):
                 if m > mc:
                     gpad = net(self.ri_pad[ti, 0].view(1, 1), self.mm)        # [1, E]
                     part = part + (m - mc) * (self.ri_pad[ti].view(4, 1) * gpad)
+                if not every[tj]:
+                    part = part * amask[idx, tj].view(-1, 1, 1)
                 A = part if A is None else A + part
-            A = A / (m * len(present))
+            A = A * inv[idx].view(-1, 1, 1)
             DR = bmm_tn(A, A[..., :cfg.M2])
             Ei = Ei.index_copy(0, idx, self.fit[ti](DR.reshape(idx.numel(), -1), self.mm, add_bias).reshape(-1))
         Etot = torch.zeros(batch.nframes, dtype=Ei.dtype, device=Ei.device).index_add(0, batch.frame, Ei)
@@ -172,7 +199,8 @@ def plain_grads(model: DPTorch, batch: TrainBatch, chunk_frames=None):
         lst = batch.list_full[:, lo:hi]
         lst = torch.where(lst > 0, lst - lo, lst)
         tm = batch.type_map[lo:hi]
-        Etot_atoms, Ei, F, V = model(lst, tm, batch.image_dr_full[:, lo:hi], create_graph=True)
+        Etot_atoms, Ei, F, V = model(lst, tm, batch.image_dr_full[:, lo:hi],
+                                     type_present=batch.atom_type_present()[lo:hi], create_graph=True)
         Etot = torch.zeros(b - a, dtype=Ei.dtype, device=Ei.device).index_add(0, batch.frame[lo:hi] - a, Ei[0])
         le = (((Etot - batch.E_label[a:b]) / batch.frame_natoms[a:b]) ** 2).sum() / F_all
         lf = ((F[0] - batch.F_label[lo:hi]) ** 2).sum() / (3 * batch.natoms)

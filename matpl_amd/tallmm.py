@@ -92,9 +92,10 @@ def tall_mm_f32(x, W):
 
 
 class _TallMMLow(torch.autograd.Function):
-    """x @ W with both operands rounded to ``dt`` (bfloat16) for the product on the matrix cores
-    (FP32 accumulation, result rounded to ``dt``, returned as FP32); the input gradient is the same
-    low-precision product, the weight gradient an FP32 split product."""
+    """x @ W with both operands rounded to ``dt`` (bfloat16): BF16-rounded operands multiplied
+    with FP32 VALU FMAs (matpl_tall_mm for widths <= 64, else torch's product), result rounded to
+    ``dt`` and returned as FP32; the input gradient is the same low-precision product, the weight
+    gradient an FP32 split product."""
 
     @staticmethod
     def forward(ctx, x, W, dt):

@@ -110,6 +110,28 @@ def test_write_read_round_trip(tmp_path):
     assert ck["atom_type_order"].size == 2 and ck["davg"].shape == (2, 2 * 120 * 4)
 
 
+@pytest.mark.parametrize("emb_rdt,fit_rdt", [(True, True), (False, True), (True, False), (False, False)])
+def test_written_state_dict_loads_strictly_into_matpl_modules(tmp_path, emb_rdt, fit_rdt):
+    """MatPL loads a checkpoint with model.load_state_dict(checkpoint['state_dict']) in strict mode
+    (dp_network.py:225), into modules that own a resnet_dt on every embedding layer when the flag is
+    set (dp_embedding.py:65-70) and on fitting layers 1..n-2 (dp_embedding.py:123-126)."""
+    cfg = DPConfig(ntypes=2, m=120, emb_resnet_dt=emb_rdt, fit_resnet_dt=fit_rdt)
+    w = DPWeights.random(cfg, seed=4)
+    path = str(tmp_path / "strict.ckpt")
+    write_dp_checkpoint(w, [72, 8], path)
+    ones = lambda n: np.ones(n)
+    emb = [[(W, b, (r if r is not None else ones(W.shape[1])) if emb_rdt else None) for (W, b, r) in net]
+           for net in w.emb]
+    fit = [[(W, b, r if (fit_rdt and 0 < l < len(net) - 1) else None) for l, (W, b, r) in enumerate(net)]
+           for net in w.fit]
+    model = _DP(emb, fit).double()
+    missing, unexpected = model.load_state_dict(load_checkpoint(path)["state_dict"], strict=True)
+    assert not missing and not unexpected
+    for p in range(4):
+        for l, (W, b, r) in enumerate(w.emb[p]):
+            assert np.array_equal(model.embedding_net[p].layers[l].weight.detach().numpy(), W)
+
+
 def test_rejects_other_layouts(tmp_path):
     w = _weights()
     path = str(tmp_path / "bad.ckpt")

@@ -57,7 +57,7 @@ def test_library_loads_and_exports_everything():
     L = ctypes.CDLL(_lib.LIB_PATH)
     missing = [s for s in declared_symbols() if not hasattr(L, s)]
     assert not missing, missing
-    assert _lib.lib().matpl_abi_version() == 1
+    assert _lib.lib().matpl_abi_version() == 2
 
 
 def test_missing_library_fails_loudly(monkeypatch):

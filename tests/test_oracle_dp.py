@@ -266,3 +266,39 @@ def test_operator_contract_dense_scatter():
     gV = rng.normal(size=(B, 9))
     gv = oracle.dp_virial_grad_op(nl0, Rij, Rid, gV)
     assert abs((oracle.dp_virial_op(nl0, x, Rij, Rid, 0)[0] * gV).sum() - (gv * x).sum()) < 1e-9
+
+
+def _drop_type(w, t_drop):
+    """The (T-1)-type model that keeps every type but t_drop: its nets, davg/dstd rows."""
+    T = w.cfg.ntypes
+    keep = [t for t in range(T) if t != t_drop]
+    cfg = DPConfig(ntypes=T - 1, m=w.cfg.m, rcut=w.cfg.rcut, rcut_smooth=w.cfg.rcut_smooth)
+    emb = [w.emb[a * T + b] for a in keep for b in keep]
+    return DPWeights(cfg, emb, [w.fit[t] for t in keep], w.davg[keep], w.dstd[keep]), keep
+
+
+@pytest.mark.parametrize("t_drop", [0, 1, 2])
+def test_absent_type_equals_reduced_model(t_drop):
+    """A frame without type t, evaluated with t left out of its type list (dp_dp.py:150-165, 197,
+    413), equals the (T-1)-type model with t's nets, bucket and statistics removed; with t listed
+    (every model type) it does not, because the empty t bucket then adds its padding term and
+    type_nums counts t."""
+    base = st.fluorite_hfo2((2, 2, 2), seed=41)
+    keep2 = [t for t in range(3) if t != t_drop]
+    f3 = st.Frame(base.lattice, base.pos, np.asarray(keep2, np.int32)[base.types], ["A", "B", "C"])
+    cfg = DPConfig(ntypes=3, m=100)
+    rng = np.random.default_rng(7)
+    davg = np.c_[rng.uniform(0.05, 0.2, 3), np.zeros((3, 3))]
+    dstd = np.c_[rng.uniform(0.05, 0.2, 3), rng.uniform(0.05, 0.2, (3, 3))]
+    w3 = DPWeights.random(cfg, seed=12, davg=davg, dstd=dstd)
+    tp = np.ones(3, np.int32)
+    tp[t_drop] = 0
+    r3 = oracle.dp_evaluate(f3, w3, type_present=tp)
+    w2, keep = _drop_type(w3, t_drop)
+    f2 = st.Frame(base.lattice, base.pos, base.types.copy(), ["A", "B"])
+    r2 = oracle.dp_evaluate(f2, w2)
+    assert abs(r3["E"] - r2["E"]) <= 1e-12 * abs(r2["E"])
+    np.testing.assert_allclose(r3["F"], r2["F"], atol=1e-12)
+    np.testing.assert_allclose(r3["V"], r2["V"], atol=1e-11 * np.abs(r2["V"]).max())
+    full = oracle.dp_evaluate(f3, w3)
+    assert abs(full["E"] - r2["E"]) > 1e-6 * abs(r2["E"])
