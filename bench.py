@@ -67,29 +67,82 @@ def flops_per_neighbor_bwd(E=25):
     return 2 * (fwd + bwd)
 
 
-def cpu_baseline():
-    """FP64 C oracle (OpenMP) E+F+V on a bounded sample of the same workload."""
+def _cpu_info():
+    """CPU model and counts of the host the baseline runs on (lscpu's fields, read from /proc/cpuinfo)."""
+    model, phys = None, set()
+    pid = cid = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    pid = v
+                elif k == "core id":
+                    cid = v
+                elif not k and pid is not None:
+                    phys.add((pid, cid))
+                    pid = cid = None
+    except OSError:
+        pass
+    if pid is not None:
+        phys.add((pid, cid))
+    return {"cpu_model": model, "host_physical_cores": len(phys) or None, "host_logical_cpus": os.cpu_count(),
+            "cpus_allowed": len(os.sched_getaffinity(0))}
+
+
+def _time_oracle(frame, w, seconds):
     import oracle
-    from matpl_amd import structures as st
-    from matpl_amd.dpnet import DPConfig, DPWeights
-    reps = (8, 8, 8)
-    f = st.fluorite_hfo2(reps, a=5.08, sigma=0.1, seed=1003)
-    cfg = DPConfig(ntypes=2, rcut=6.0, rcut_smooth=5.5, m=120)
-    lst, dR = oracle.dp_find_neighbor(f.lattice, f.frac(), f.types, 2, cfg.m, cfg.rcut)
-    davg, dstd = oracle.dp_calc_stat(lst, dR, f.types, 2, cfg.rcut_smooth, cfg.rcut)
-    w = DPWeights.random(cfg, seed=2023, davg=davg, dstd=dstd)
     t0 = time.perf_counter()
     nev = 0
     while True:
-        oracle.dp_evaluate(f, w)
+        oracle.dp_evaluate(frame, w)
         nev += 1
-        if time.perf_counter() - t0 > 10.0:
+        if time.perf_counter() - t0 > seconds:
             break
-    dt = time.perf_counter() - t0
-    return {"value": f.natoms * nev / dt, "unit": "atom-steps/s", "cores": oracle.lib().oracle_num_threads(),
-            "kind": "port",
-            "sample": f"fluorite HfO2 {reps[0]}x{reps[1]}x{reps[2]} cells ({f.natoms} atoms), same model, "
-                      f"{nev} full E+F+V evaluations incl. neighbor search, FP64 C oracle with OpenMP"}
+    return frame.natoms * nev / (time.perf_counter() - t0), nev
+
+
+def cpu_baseline():
+    """The FP64 C oracle (a restatement of the reference's CPU path: NeighConst.f90 search, dp_dp.py
+    env mat, nets and scatter; OpenMP over atoms) timed for full E+F+V evaluations, neighbor search
+    included, on bounded samples of two workloads: HfO2 8x8x8 cells (6,144 atoms; a sample of the
+    99,636-atom MD workload, same model) and fcc Cu 3x3x3 (108 atoms), each with all the threads the
+    process may use and with one thread. About 20 s of CPU work in all."""
+    import oracle
+    from matpl_amd import structures as st
+    from matpl_amd.dpnet import DPConfig, DPWeights
+    L = oracle.lib()
+    nthreads = L.oracle_num_threads()
+    out = {"unit": "atom-steps/s", "kind": "port", **_cpu_info()}
+    samples = {}
+    for name, f, cfg in (("hfo2_6144", st.fluorite_hfo2((8, 8, 8), a=5.08, sigma=0.1, seed=1003),
+                          DPConfig(ntypes=2, rcut=6.0, rcut_smooth=5.5, m=120)),
+                         ("cu_108", st.fcc_cu((3, 3, 3), seed=1000), DPConfig(ntypes=1, rcut=6.0, rcut_smooth=5.5,
+                                                                                m=100))):
+        T = cfg.ntypes
+        lst, dR = oracle.dp_find_neighbor(f.lattice, f.frac(), f.types, T, cfg.m, cfg.rcut)
+        davg, dstd = oracle.dp_calc_stat(lst, dR, f.types, T, cfg.rcut_smooth, cfg.rcut)
+        w = DPWeights.random(cfg, seed=2023, davg=davg, dstd=dstd)
+        r = {"natoms": f.natoms}
+        for key, nt, secs in (("all_threads", nthreads, 6.0), ("one_thread", 1, 4.0)):
+            L.oracle_set_num_threads(nt)
+            v, nev = _time_oracle(f, w, secs)
+            r[key] = {"value": round(v, 1), "threads": nt, "evaluations": nev}
+        L.oracle_set_num_threads(nthreads)
+        samples[name] = r
+    h = samples["hfo2_6144"]
+    out.update({"value": h["all_threads"]["value"], "cores": nthreads,
+                "sample": "fluorite HfO2 8x8x8 cells (6144 atoms), the MD workload's model, full E+F+V "
+                          "evaluations incl. neighbor search, FP64 C oracle with OpenMP; one-thread and "
+                          "fcc Cu 108-atom (the CPU-runnable DP configuration) figures under 'samples'",
+                "samples": samples})
+    log(f"CPU baseline ({out['cpu_model']}, {nthreads} threads): HfO2 6144 {h['all_threads']['value']:.4g} / "
+        f"1 thread {h['one_thread']['value']:.4g} atom-steps/s; Cu 108 "
+        f"{samples['cu_108']['all_threads']['value']:.4g} / {samples['cu_108']['one_thread']['value']:.4g}")
+    return out
 
 
 def nep_leg(steps=20):
@@ -173,6 +226,7 @@ def nep_train_leg(steps=3):
     out = {"workload": f"NEP training gradient, 128 Ag-Au-Si-C frames ({natoms} atoms), rc 6/4 A, n_max 4/4, "
                        "basis 12, l_max [4,2,0], 30 hidden, E+F+V loss backward to all parameters incl. c2/c3",
            "natoms": natoms}
+    grads = {}
     for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
         model = mn.NEPTorch(w, dt, dev)
         params = [q for q in model.parameters() if q.requires_grad]
@@ -183,7 +237,7 @@ def nep_train_leg(steps=3):
                                 gp["num_atoms"], gp["atom_type_map"], create_graph=True)
             loss = (E ** 2).sum() + (F ** 2).sum() + (V ** 2).sum()
             return torch.autograd.grad(loss, params)
-        step()
+        grads[name] = [g.detach().double() for g in step()]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -193,7 +247,10 @@ def nep_train_leg(steps=3):
         out[name] = {"value": round(natoms * steps / dt_s, 1), "unit": "atom-steps/s",
                      "ms_per_step": round(dt_s * 1e3 / steps, 2)}
     out["speedup_fp32_vs_fp64"] = round(out["fp32"]["value"] / out["fp64"]["value"], 2)
-    log(f"NEP train: fp32 {out['fp32']['ms_per_step']} ms/step, fp64 {out['fp64']['ms_per_step']} ms/step")
+    worst = max(float((a - b).norm() / b.norm()) for a, b in zip(grads["fp32"], grads["fp64"]) if b.norm() > 0)
+    out["grad_rel_err_fp32_vs_fp64"] = float(f"{worst:.3g}")
+    log(f"NEP train: fp32 {out['fp32']['ms_per_step']} ms/step, fp64 {out['fp64']['ms_per_step']} ms/step, "
+        f"gradient rel. error fp32 vs fp64 {out['grad_rel_err_fp32_vs_fp64']}")
     return out
 
 
@@ -412,6 +469,21 @@ def main():
             md._step()
         torch.cuda.synchronize()
     md.model.events = None
+    md.check()
+
+    # the accuracy gate at the end of the trajectory as well: the FP64 path at the final positions
+    end_gate = None
+    if not args.no_fp64:
+        from matpl_amd.structures import Frame
+        fe = Frame(frame.lattice, md.pos.cpu().numpy(), frame.types, frame.elements)
+        ref_end = DPMDFP64(fe, w, md.vel.cpu().numpy(), dt=1.0, device=dev)
+        end_gate = {"steps": md.nstep,
+                    "force_rmse_vs_fp64_eV_per_A": float(torch.sqrt(((md.F.double() - ref_end.F64) ** 2).mean()).item()),
+                    "energy_rel_err_vs_fp64": abs(md.energy() - float(ref_end.E.sum().item())) /
+                    abs(float(ref_end.E.sum().item()))}
+        log(f"after {md.nstep} steps: force RMSE vs FP64 = {end_gate['force_rmse_vs_fp64_eV_per_A']:.3e} eV/A")
+        del ref_end
+        torch.cuda.empty_cache()
 
     # roofline of the dominant (backward) kernel from the recorded events
     bwd_ms = [a.elapsed_time(b) for a, b in ev]
@@ -499,6 +571,7 @@ def main():
         "fp64_dropin": fp64,
         "speedup_vs_fp64_dropin": round(value / fp64["value"], 2) if fp64 else None,
         "force_rmse_vs_fp64_eV_per_A": force_rmse,
+        "end_of_trajectory_gate": end_gate,
         "roofline": roofline,
         "launch": launch,
         "cpu_baseline": cpu,

@@ -65,20 +65,22 @@ def build(jobs: int = 8, verbose: bool = True) -> str:
     return LIB
 
 
-TORCH_OPS_SRC = os.path.join(CSRC, "torch_ops", "calcops_torch.cpp")
-TORCH_OPS_LIB = os.path.join(HERE, "libcalcops_torch.so")
+TORCH_OPS_SRC = [os.path.join(CSRC, "torch_ops", "calcops_torch.cpp"),
+                 os.path.join(CSRC, "torch_ops", "nep_ops_torch.cpp")]
+TORCH_OPS_LIB = os.path.join(HERE, "libCalcOps_bind.so")
 
 
 def build_torch_ops(verbose: bool = True) -> str:
-    """libcalcops_torch.so: the TORCH_LIBRARY registration of the DP CalcOps_cuda operators for
-    TorchScript / LAMMPS hosts (host C++ against libtorch, linked to libmatpl_hip.so)."""
+    """libCalcOps_bind.so: the TORCH_LIBRARY registration of the ten CalcOps_cuda operators (the
+    reference's library name) for torch.ops.load_library callers, TorchScript and LAMMPS hosts
+    (host C++ against libtorch, linked to libmatpl_hip.so)."""
     import torch
     from torch.utils import cpp_extension as ce
     tdir = os.path.dirname(torch.__file__)
     inc = sum((["-I", p] for p in ce.include_paths()), [])
     cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__",
            "-DUSE_ROCM", f"-D_GLIBCXX_USE_CXX11_ABI={int(torch.compiled_with_cxx11_abi())}", *inc,
-           "-I", "/opt/rocm/include", TORCH_OPS_SRC, "-o", TORCH_OPS_LIB + ".tmp",
+           "-I", "/opt/rocm/include", *TORCH_OPS_SRC, "-o", TORCH_OPS_LIB + ".tmp",
            "-L", os.path.join(tdir, "lib"), "-ltorch", "-ltorch_cpu", "-lc10", "-lc10_hip", "-ltorch_hip",
            "-L", HERE, "-lmatpl_hip", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{os.path.join(tdir, 'lib')}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -93,7 +95,7 @@ def build_torch_ops(verbose: bool = True) -> str:
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
-    ap.add_argument("--torch-ops", action="store_true", help="also build libcalcops_torch.so")
+    ap.add_argument("--torch-ops", action="store_true", help="also build libCalcOps_bind.so")
     a = ap.parse_args()
     build(a.jobs)
     if a.torch_ops:

@@ -1,17 +1,19 @@
-// TORCH_LIBRARY registration of the DP operators of CalcOps_cuda on top of
-// libmatpl_hip.so, for hosts without Python: TorchScript models and the
-// LAMMPS pair style (src/lmps/lammps-libtorch/MATPL/pair_matpl.cpp) load a
-// scripted model that calls torch.ops.CalcOps_cuda.*, with this library
-// named by OP_LIB_PATH as the reference's libCalcOps_bind.so is.
+// TORCH_LIBRARY registration of the ten CalcOps_cuda operators on top of
+// libmatpl_hip.so, built as libCalcOps_bind.so: the name and namespace the
+// reference's models load (torch.ops.load_library(<src>/op/build/lib/
+// libCalcOps_bind.so), src/model/dp_dp.py:13-20, src/model/nep_net.py:17-24)
+// and that TorchScript models and the LAMMPS pair style
+// (src/lmps/lammps-libtorch/MATPL/pair_matpl.cpp) find through OP_LIB_PATH.
 //
-// Same names, schemas and autograd edges as the reference's bindings
-// (src/op/src/CalcOps_bind.cpp:6-17, src/op/include/CalcOps.h:35-359):
+// Same names, schemas (inferred from the C++ signatures, as the reference's
+// m.def(name, fn) does) and autograd edges as the reference's bindings
+// (src/op/src/CalcOps_bind.cpp:6-17, src/op/include/CalcOps.h:35-318):
 //   calculateForce   -> [force [B][N+nghost][3]];   backward to dE only
 //   calculateVirial  -> [virial [B][9], atom_virial [B][N+nghost][9]]; backward to dE
 //   calculateCompress-> [G [n][L]];                   backward to f2 only
+//   the seven NEP operators: nep_ops_torch.cpp.
 // Work is enqueued on torch's current HIP stream; errors raise (TORCH_CHECK)
-// with the library's message. The NEP operators are registered from Python
-// (matpl_amd.calcops); MD with a NEP model goes through that path.
+// with the library's message.
 #include <torch/library.h>
 #include <torch/torch.h>
 #include <c10/hip/HIPStream.h>
@@ -221,8 +223,37 @@ TORCH_LIBRARY(matpl_amd, m) {
           "-> Tensor[]", env_mat);
 }
 
+namespace matpl_torch {
+std::vector<Tensor> calculateNepFeat(Tensor coeff2, Tensor d12_radial, Tensor NL_radial, Tensor atom_map,
+                                     Tensor feats, double rcut_radial, int64_t feat_multi_nums);
+std::vector<Tensor> calculateNepMbFeat(Tensor coeff3, Tensor d12_angluar, Tensor NL_angluar, Tensor atom_map,
+                                       Tensor feats, int64_t feat_2b_num, int64_t lmax_3, int64_t lmax_4,
+                                       int64_t lmax_5, double rcut_angluar);
+std::vector<Tensor> calculateNepForce(Tensor list_neigh, Tensor dE, Tensor Ri_d, Tensor F);
+std::vector<Tensor> calculateNepVirial(Tensor list_neigh, Tensor dE, Tensor Rij, Tensor Ri_d, Tensor num_atom);
+std::vector<Tensor> calculate_maxneigh(const Tensor &num_atoms, const Tensor &box, const Tensor &box_orig,
+                                       const Tensor &num_cell, const Tensor &position, double cutoff_2b,
+                                       double cutoff_3b, int64_t atom_type_num, const Tensor &atom_type_map,
+                                       bool with_type);
+std::vector<Tensor> calculate_neighbor(const Tensor &num_atoms, const Tensor &atom_type_map, const Tensor &atom_types,
+                                       const Tensor &box, const Tensor &box_orig, const Tensor &num_cell,
+                                       const Tensor &position, double cutoff_2b, double cutoff_3b,
+                                       int64_t max_NN_radial, int64_t max_NN_angular, bool with_rij);
+std::vector<Tensor> calculate_descriptor(const Tensor &weight_radial, const Tensor &weight_angular,
+                                         const Tensor &Ri_radial, const Tensor &NL_radial,
+                                         const Tensor &atom_type_map, double cutoff_radial, double cutoff_angular,
+                                         int64_t max_NN_radial, int64_t lmax_3, int64_t lmax_4, int64_t lmax_5);
+}  // namespace matpl_torch
+
 TORCH_LIBRARY(CalcOps_cuda, m) {
     m.def("calculateForce", calculateForce);
     m.def("calculateVirial", calculateVirial);
     m.def("calculateCompress", calculateCompress);
+    m.def("calculateNepFeat", matpl_torch::calculateNepFeat);
+    m.def("calculateNepMbFeat", matpl_torch::calculateNepMbFeat);
+    m.def("calculateNepForce", matpl_torch::calculateNepForce);
+    m.def("calculateNepVirial", matpl_torch::calculateNepVirial);
+    m.def("calculate_maxneigh", matpl_torch::calculate_maxneigh);
+    m.def("calculate_neighbor", matpl_torch::calculate_neighbor);
+    m.def("calculate_descriptor", matpl_torch::calculate_descriptor);
 }

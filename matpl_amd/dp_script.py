@@ -9,7 +9,7 @@ uses for a scripted model (pair_matpl.cpp:925-931):
 
 Operators: torch.ops.matpl_amd.env_mat (Ri, Ri_d on the HIP kernel) and
 torch.ops.CalcOps_cuda.calculateForce / calculateVirial, both registered in C++ by
-libcalcops_torch.so (``load_ops``); nets in torch. Every model type counts in the 1/(m T)
+libCalcOps_bind.so (``load_ops``); nets in torch. Every model type counts in the 1/(m T)
 normalisation of A (dp_dp.py:197, 413 with type_nums from atom_type), as in the pair style's call,
 which passes all of the model's types in atom_type. Script it with ``torch.jit.script`` after
 ``load_ops()`` in a process that has not registered the Python CalcOps_cuda ops.
@@ -24,7 +24,7 @@ import torch
 
 from .dpnet import DPWeights
 
-OPS_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcalcops_torch.so")
+OPS_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libCalcOps_bind.so")
 
 
 def load_ops(path: str = OPS_LIB) -> None:

@@ -61,6 +61,8 @@ def lib():
         L.oracle_dp_force_grad_op.argtypes = [_i32p, _f64p, _f64p, c_int, c_int, c_int, c_int, _f64p]
         L.oracle_dp_virial_grad_op.argtypes = [_i32p, _f64p, _f64p, _f64p, c_int, c_int, c_int, _f64p]
         L.oracle_num_threads.restype = c_int
+        L.oracle_set_num_threads.argtypes = [c_int]
+        L.oracle_set_num_threads.restype = None
         _LIB = L
     return _LIB
 

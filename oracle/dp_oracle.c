@@ -619,3 +619,13 @@ int oracle_num_threads(void)
     return 1;
 #endif
 }
+
+/* Threads of later oracle calls (the CPU baseline times one core and all cores). */
+void oracle_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

@@ -249,3 +249,46 @@ def nep_evaluate(frames, w, maxr=None, maxa=None):
     E = np.array([Ei[starts[k]:starts[k + 1]].sum().item() for k in range(nf)])
     return dict(E=E, Ei=Ei.detach().numpy(), F=F, V=V, q=q.detach().numpy(), packed=packed, nb=nb,
                 dE_r=gr.numpy(), dE_a=ga.numpy())
+
+
+def nep_efv_graph(frames, cfg, c2, c3, fit, q_scaler, maxr=None, maxa=None):
+    """E [F], F [N,3], V [F,9] (FP64 torch on the CPU) as differentiable functions of the parameter
+    leaves c2, c3 and fit[t] = [(W, b, r)], for force/virial-loss gradients by the double backward
+    (the reference's training path, nep_net.py:503-648 with create_graph). Same conventions as
+    nep_evaluate, whose numpy forces and virial the strain and position differences pin."""
+    packed = pack_frames(frames, cfg.rcut_radial)
+    NNr, NNa = maxneigh(packed, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes, False)
+    nb = neighbor(packed, cfg.rcut_radial, cfg.rcut_angular, maxr or int(NNr.max()), maxa or int(NNa.max()))
+    am = packed["atom_type_map"]
+    Rr = torch.as_tensor(nb["Ri_r"]).clone().requires_grad_()
+    Ra = torch.as_tensor(nb["Ri_a"]).clone().requires_grad_()
+    l3, l4, l5 = cfg.l_max
+    q = radial_feats(c2, Rr, nb["NL_r"], am, cfg.rcut_radial)
+    if l3 > 0:
+        q = torch.cat([q, angular_feats(c3, Ra, nb["NL_a"], am, cfg.rcut_angular, l3, l4, l5)], -1)
+    x = q * torch.as_tensor(q_scaler)
+    amt = torch.as_tensor(am)
+    Ei = torch.zeros(q.shape[0], dtype=torch.float64)
+    for t in range(cfg.ntypes):
+        m = amt == t
+        if m.any():
+            Ei = Ei.index_put((m,), fit_forward(fit[t], x[m]).reshape(-1))
+    nf = len(frames)
+    frame = torch.repeat_interleave(torch.arange(nf), torch.as_tensor(packed["num_atoms"]))
+    E = torch.zeros(nf, dtype=torch.float64).index_add(0, frame, Ei)
+    gr, ga = torch.autograd.grad(E.sum(), [Rr, Ra], create_graph=True, allow_unused=True)
+    F = torch.zeros((q.shape[0], 3), dtype=torch.float64)
+    V = torch.zeros((nf, 9), dtype=torch.float64)
+    for R, g, NL in ((Rr, gr, nb["NL_r"]), (Ra, ga, nb["NL_a"])):
+        if g is None:
+            continue
+        NLt = torch.as_tensor(NL)
+        valid = (NLt >= 0).unsqueeze(-1)
+        Rd = R.detach()
+        r = torch.where(valid[..., 0], Rd[..., 0], torch.ones_like(Rd[..., 0]))
+        pair = (g[..., 0:1] * Rd[..., 1:4] / r.unsqueeze(-1) + g[..., 1:4]) * valid       # dE/d(dR)
+        F = F + pair.sum(1)
+        F = F.index_add(0, NLt.clamp(min=0).reshape(-1), -pair.reshape(-1, 3))
+        outer = -(Rd[..., 1:4].unsqueeze(-1) * pair.unsqueeze(-2)).sum(1).reshape(-1, 9)   # per centre atom
+        V = V.index_add(0, frame, outer)
+    return E, F, V

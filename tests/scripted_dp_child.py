@@ -20,7 +20,8 @@ from matpl_amd.neighbor import calc_stat, find_neighbore_gpu, generate_neighdata
 def main(path):
     dev = torch.device("cuda:0")
     dp_script.load_ops()
-    assert not hasattr(torch.ops.CalcOps_cuda, "calculateNepFeat"), "Python ops registered"
+    co = sys.modules.get("matpl_amd.calcops")
+    assert co is None or not co._REGISTERED, "Python ops registered"
     RC, SKIN = 6.0, 1.0
     f, T, m, t2m = st.fluorite_hfo2((2, 2, 2), seed=7), 2, 120, [1, 0]
     x, owner = ol.ghost_system(f.lattice, f.pos, RC + SKIN)

@@ -130,3 +130,18 @@ def test_forces_and_virial_finite_difference(idx):
             E.append(on.nep_evaluate([st.Frame(f.lattice @ M, f.pos @ M, f.types, f.elements)], w)["E"][0])
         fd = -(E[0] - E[1]) / (2 * eps)
         assert abs(fd - r["V"][0][3 * a + b]) < 2e-5 * max(1.0, abs(fd))
+
+
+def test_differentiable_efv_equals_pinned_evaluate():
+    """oracle.nep.nep_efv_graph (the torch form used for loss gradients) gives the energies, forces and
+    virials of nep_evaluate, which the position and strain differences above pin."""
+    frames = [st.zincblende_sic((2, 2, 2), seed=4), st.agausic_frames(nframes=2, nmin=30, nmax=50, seed=5)[0]]
+    cfg = NEPConfig(ntypes=4, l_max=[4, 2, 1])
+    w = NEPWeights.random(cfg, seed=3)
+    want = on.nep_evaluate(frames, w)
+    fit = [[(torch.tensor(W), torch.tensor(b), None if r is None else torch.tensor(r)) for W, b, r in L]
+           for L in w.fit]
+    E, F, V = on.nep_efv_graph(frames, cfg, torch.tensor(w.c2), torch.tensor(w.c3), fit, w.q_scaler)
+    np.testing.assert_allclose(E.detach().numpy(), want["E"], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(F.detach().numpy(), want["F"], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(V.detach().numpy(), want["V"], rtol=0, atol=1e-12 * np.abs(want["V"]).max())
