@@ -319,14 +319,41 @@ def dp_train_leg(steps=5):
             q.grad = gq
         ropt.step()
     timed("fp64_plain", rstep, 1)
+    # the same step on the hand-written kernels (fused forward and force kernels, force-loss double
+    # backward of the embedding nets on the matrix cores; fitting nets in torch)
+    from matpl_amd.dp_train_fused import DPTrainFused, pack_fused_batch
+    fb = pack_fused_batch(frames, cfg, dev, seed=7)
+    nin = int(fb.nl_counts.sum().item())         # list entries at rc (= in-cutoff neighbors)
     for prec in ("bf16", "fp32"):
+        fm = DPTrainFused(w, prec, dev)
+        fparams = list(fm.parameters())
+        fopt = torch.optim.Adam(fparams, lr=1e-4)
+        grads[prec + "_fused"] = [None if x is None else x.detach().clone() for x in fm.loss_and_grads(fb)[1]]
+
+        def fstep(fm=fm, fparams=fparams, fopt=fopt):
+            loss, g = fm.loss_and_grads(fb)
+            for q, gq in zip(fparams, g):
+                q.grad = gq
+            fopt.step()
+        timed(prec + "_fused", fstep, steps)
+        del fm, fopt
+    for prec in ("bf16", "fp32", "bf16_fused", "fp32_fused"):
         worst = 0.0
         for a, b in zip(grads[prec], grads["fp64"]):
             if a is not None and b is not None and b.norm() > 0:
                 worst = max(worst, float((a.double() - b).norm() / b.norm()))
         out[f"grad_rel_err_{prec}_vs_fp64"] = float(f"{worst:.3g}")
         out[f"speedup_{prec}_vs_fp64_plain"] = round(out[prec]["value"] / out["fp64_plain"]["value"], 1)
+    # path-level work of a training step: 3 F_DP per atom (forward, dE/dRi, double backward),
+    # F_DP = 4 (1375 n + 26,650) flop with n the atom's in-cutoff neighbors
+    flops = 3 * 4 * (1375 * nin + 26650 * N)
+    for prec in ("bf16_fused", "fp32_fused"):
+        tf = flops / (out[prec]["ms_per_step"] * 1e-3) / 1e12
+        out[prec]["path_tflops"] = round(tf, 3)
+        out[prec]["path_frac_of_bf16_peak"] = round(tf / BF16_PEAK_TFLOPS, 5)
+    out["in_cutoff_neighbors"] = nin
     log(f"DP train: bf16 {out['bf16']['ms_per_step']} ms, fp32 {out['fp32']['ms_per_step']} ms, "
+        f"fused bf16 {out['bf16_fused']['ms_per_step']} ms, fused fp32 {out['fp32_fused']['ms_per_step']} ms, "
         f"fp64 plain {out['fp64_plain']['ms_per_step']} ms per step (mc={batch.mc})")
     return out
 
@@ -401,6 +428,9 @@ def main():
     ap.add_argument("--compress", action="store_true",
                     help="also time the MD workload with the compressed embedding (the reference's DP compress, "
                          "tables read by the fused kernels)")
+    ap.add_argument("--store", action="store_true",
+                    help="MD with the fused store mode: the forward keeps G and dG/ds per neighbor and the backward "
+                         "reads them instead of evaluating the embedding net again")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MiB",
                     help="after the run, copy a buffer of this size 3 times (a known-traffic kernel for "
                          "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -420,14 +450,16 @@ def main():
     N = frame.natoms
     log(f"workload: HfO2 {reps} = {N} atoms")
 
-    md = DPMD(frame, w, vel, precision=args.precision, skin=0.5, mnl=160, dt=1.0, rebuild_every=10, device=dev)
+    md = DPMD(frame, w, vel, precision=args.precision, skin=0.5, mnl=160, dt=1.0, rebuild_every=10, device=dev,
+              store=args.store)
     md.check()
     launch = None
     from matpl_amd import autotune
     if args.launch:
         launch = autotune.parse_launch(args.launch)
     elif not args.no_tune:
-        key = autotune.workload_key(w.cfg.emb_sizes[0], w.cfg.ntypes, md.mnl, w.cfg.m, N, args.precision)
+        key = autotune.workload_key(w.cfg.emb_sizes[0], w.cfg.ntypes, md.mnl, w.cfg.m, N,
+                                    args.precision + ("-store" if args.store else ""))
         launch = autotune.tune_fused(md.compute, key, log=log)
     for _ in range(args.warmup):
         md.step()
@@ -478,7 +510,8 @@ def main():
         fe = Frame(frame.lattice, md.pos.cpu().numpy(), frame.types, frame.elements)
         ref_end = DPMDFP64(fe, w, md.vel.cpu().numpy(), dt=1.0, device=dev)
         end_gate = {"steps": md.nstep,
-                    "force_rmse_vs_fp64_eV_per_A": float(torch.sqrt(((md.F.double() - ref_end.F64) ** 2).mean()).item()),
+                    "force_rmse_vs_fp64_eV_per_A":
+                        float(torch.sqrt(((md.F.double() - ref_end.F64) ** 2).mean()).item()),
                     "energy_rel_err_vs_fp64": abs(md.energy() - float(ref_end.E.sum().item())) /
                     abs(float(ref_end.E.sum().item()))}
         log(f"after {md.nstep} steps: force RMSE vs FP64 = {end_gate['force_rmse_vs_fp64_eV_per_A']:.3e} eV/A")
@@ -518,7 +551,7 @@ def main():
     force_rmse = None
     if not args.no_fp64:
         ref = DPMDFP64(frame, w, vel, dt=1.0, device=dev)
-        chk = DPMD(frame, w, vel, precision=args.precision, skin=0.5, mnl=160, device=dev)
+        chk = DPMD(frame, w, vel, precision=args.precision, skin=0.5, mnl=160, device=dev, store=args.store)
         F64 = ref.F64
         force_rmse = float(torch.sqrt(((chk.F.double() - F64) ** 2).mean()).item())
         e_rel = abs(chk.energy() - float(ref.E.sum().item())) / abs(float(ref.E.sum().item()))
@@ -567,7 +600,7 @@ def main():
                                f"({N} atoms), rcut 6.0 A, m 120/type, NL skin 0.5 A rebuilt every 10 steps, "
                                "E+F+V every step",
                    "natoms": N, "precision": args.precision + " nets / fp32 descriptors / fp64 min-image+reductions",
-                   "hip_graph": use_graph},
+                   "hip_graph": use_graph, "store": args.store},
         "fp64_dropin": fp64,
         "speedup_vs_fp64_dropin": round(value / fp64["value"], 2) if fp64 else None,
         "force_rmse_vs_fp64_eV_per_A": force_rmse,

@@ -92,7 +92,8 @@ int matpl_dp_virial_grad_f64(const int32_t *nlist, const double *Rij, const doub
  * (r_j - r_i; may be NULL), counts [Ntot][T] (may be NULL), shift
  * [Ntot][T][m] packed image (may be NULL), frac_wrapped [Ntot][3] out.
  * status (device int32, zero before the call) receives 1 + the first atom
- * whose bucket overflowed m. Replaces neighconst.find_neighbore
+ * whose bucket overflowed m, or -4 if a coordinate is NaN or infinite (the
+ * search then runs with that coordinate at 0). Replaces neighconst.find_neighbore
  * (src/pre_data/gen_feature/NeighConst.f90:14-149) as called by
  * find_neighbore (src/pre_data/dpuni_data_loader.py:595-650).
  *
@@ -234,6 +235,32 @@ int matpl_dp_fused_backward(const double *pos, const double *lattice, const int3
                             float *fslot, float *fcenter, float *force, double *virial, int32_t *status,
                             void *stream);
 /*
+ * Store mode of the fused pair: matpl_dp_fused_forward_store computes A as
+ * matpl_dp_fused_forward and also writes, for every in-cutoff neighbor slot,
+ * the embedding output G(s) and its derivative dG/ds into gbuf
+ * (matpl_dp_fused_gstore_words(natoms, ntypes, mnl, m, E, bf16) uint32 words,
+ * 16-byte aligned; bf16 != 0: fp16 pairs, else float); the backward
+ * matpl_dp_fused_backward_stored reads them instead of evaluating the net
+ * again. Same results as the recomputing pair up to the fp16 rounding of G
+ * and dG/ds in BF16 mode (FP32 mode: identical arithmetic). No counterpart in
+ * the reference, whose autograd keeps the embedding activations for its
+ * backward (dp_dp.py:429) the way this buffer does.
+ */
+int64_t matpl_dp_fused_gstore_words(int64_t natoms, int ntypes, int mnl, int m, int E, int bf16);
+int matpl_dp_fused_forward_store(const double *pos, const double *lattice, const int32_t *atom_frame,
+                                 const int64_t *frame_start, const uint32_t *frame_types, const int32_t *list,
+                                 const int32_t *shift, const int32_t *counts, const int32_t *type_map,
+                                 const float *vec, const void *mat, const float *tinfo, int64_t natoms, int ntypes,
+                                 int mnl, int m, int E, int bf16, double rcut, double rcut_smooth, double scale,
+                                 float *A, uint32_t *gbuf, int32_t *status, void *stream);
+int matpl_dp_fused_backward_stored(const double *pos, const double *lattice, const int32_t *atom_frame,
+                                   const int64_t *frame_start, const uint32_t *frame_types, const int32_t *list,
+                                   const int32_t *shift, const int32_t *counts, const int32_t *type_map,
+                                   const float *vec, const void *mat, const float *tinfo, int64_t natoms, int ntypes,
+                                   int mnl, int m, int E, int bf16, double rcut, double rcut_smooth, double scale,
+                                   const float *dA, const uint32_t *gbuf, const int32_t *rev, float *fslot,
+                                   float *fcenter, float *force, double *virial, int32_t *status, void *stream);
+/*
  * The fused forward / backward with a compressed embedding: G(s) and dG/ds
  * from piecewise polynomials instead of the net (the reference's DP compress,
  * dp_dp.py:350-355, 572-601 with the table of src/user/ckpt_compress.py).
@@ -262,6 +289,38 @@ int matpl_dp_fused_backward_compressed(const double *pos, const double *lattice,
                                        int bf16, double rcut, double rcut_smooth, double scale, const float *dA,
                                        const int32_t *rev, float *fslot, float *fcenter, float *force,
                                        double *virial, int32_t *status, void *stream);
+/*
+ * Force-loss double backward of the embedding nets for training (the
+ * reference's dE = autograd.grad(Etot, Ri, create_graph=True) then
+ * loss.backward(), src/model/dp_dp.py:429 and
+ * src/PWMLFF/dp_mods/dp_trainer.py:274, through the embedding nets).
+ * Geometry, weights and frame arguments as matpl_dp_fused_forward; the
+ * embedding must have no resnet_dt and E in {16, 25}.
+ * matpl_dp_train_tangent: dA [N][4][E] = c sum_j (v_j (x) G_j + Ri_j (x) v_j0
+ * G'_j), the tangent of A along the force residual field resid [N][3]
+ * (v_j = dRi_j/d(dR_j) (resid_j - resid_i), G' = dG/ds), and the in-cutoff
+ * neighbor counts nin [N][T] (int32, may be NULL).
+ * matpl_dp_train_embgrad: adds to grad [T*T][matpl_dp_train_grad_size(E)]
+ * (float, zero it first; per pair W3 [E][E], b3, W2 [E][E], b2, w1, b1) the
+ * gradient of sum_ij a_ij . G_ij + b_ij . G'_ij over in-cutoff neighbors,
+ * a = c (ahat^T Ri - abar^T v), b = -c v_0 abar^T Ri, from abar = dE/dA and
+ * ahat = dL/dA of the fitting side ([N][4][E], not scaled by c). chunk =
+ * atoms per wavefront. The empty-slot term is the caller's.
+ */
+int matpl_dp_train_tangent(const double *pos, const double *lattice, const int32_t *atom_frame,
+                           const int64_t *frame_start, const uint32_t *frame_types, const int32_t *list,
+                           const int32_t *shift, const int32_t *counts, const int32_t *type_map, const float *vec,
+                           const void *mat, const float *tinfo, int64_t natoms, int ntypes, int mnl, int m, int E,
+                           int bf16, double rcut, double rcut_smooth, double scale, const float *resid, float *dA,
+                           int32_t *nin, int32_t *status, void *stream);
+int matpl_dp_train_embgrad(const double *pos, const double *lattice, const int32_t *atom_frame,
+                           const int64_t *frame_start, const uint32_t *frame_types, const int32_t *list,
+                           const int32_t *shift, const int32_t *counts, const int32_t *type_map, const float *vec,
+                           const void *mat, const float *tinfo, int64_t natoms, int ntypes, int mnl, int m, int E,
+                           int bf16, double rcut, double rcut_smooth, double scale, const float *resid,
+                           const float *abar, const float *ahat, float *grad, int chunk, int32_t *status,
+                           void *stream);
+int matpl_dp_train_grad_size(int E);
 /*
  * Launch shape of the fused forward (kernel 0) or backward (kernel 1) kernel
  * for later calls in this process: the persistent grid's block cap
@@ -315,6 +374,7 @@ int matpl_dp_compress_grad_f32(const float *f2, const float *coef, const float *
                                float *gf2, void *stream);
 int matpl_dp_compress_grad_f64(const double *f2, const double *coef, const double *gG, int64_t n, int L,
                                int coe_num, double *gf2, void *stream);
+
 
 /* ------------------------------------------------------------ NEP operators */
 /*

@@ -44,10 +44,16 @@ SIGNATURES = {
     "matpl_dp_fused_forward": (I32, [P] * 12 + [I64, I32, I32, I32, I32, I32, F64, F64, F64, P, P, P]),
     "matpl_dp_fused_backward": (I32, [P] * 12 + [I64, I32, I32, I32, I32, I32, F64, F64, F64] + [P] * 8),
     "matpl_bsmm": (I32, [I32, I32, P, P, P, I64, I32, I32, I32, P]),
+    "matpl_dp_train_tangent": (I32, [P] * 12 + [I64, I32, I32, I32, I32, I32, F64, F64, F64, P, P, P, P, P]),
+    "matpl_dp_train_embgrad": (I32, [P] * 12 + [I64, I32, I32, I32, I32, I32, F64, F64, F64, P, P, P, P, I32, P, P]),
+    "matpl_dp_train_grad_size": (I32, [I32]),
     "matpl_dp_fused_forward_compressed": (I32, [P] * 11 + [I32, I32, I32, F64, F64, P, I64, I32, I32, I32, I32, I32,
                                                           F64, F64, F64, P, P, P]),
     "matpl_dp_fused_backward_compressed": (I32, [P] * 11 + [I32, I32, I32, F64, F64, P, I64, I32, I32, I32, I32, I32,
                                                            F64, F64, F64] + [P] * 8),
+    "matpl_dp_fused_gstore_words": (I64, [I64, I32, I32, I32, I32, I32]),
+    "matpl_dp_fused_forward_store": (I32, [P] * 12 + [I64, I32, I32, I32, I32, I32, F64, F64, F64, P, P, P, P]),
+    "matpl_dp_fused_backward_stored": (I32, [P] * 12 + [I64, I32, I32, I32, I32, I32, F64, F64, F64] + [P] * 9),
     "matpl_dp_fused_set_launch": (I32, [I32, I32, I32, I32]),
     "matpl_dp_fused_gather": (I32, [P, P, P, P, I32, I32, I64, P]),
     "matpl_dp_lmp_neighdata": (I32, [P, P, P, I32, P, P, P, P, I64, I64, I64, I32, I32, F64, P, P, P, P, P, P, P]),

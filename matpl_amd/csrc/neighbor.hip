@@ -65,10 +65,23 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
-__global__ void wrap_kernel(const double *__restrict__ frac, double *__restrict__ frac_w, int64_t n3) {
+// A non-finite coordinate has no cell and no image: status gets STATUS_NONFINITE (before any
+// search kernel can record an overflow) and the coordinate is searched as 0, so that every index
+// the search forms stays in range.
+constexpr int32_t STATUS_NONFINITE = -4;
+
+__global__ void wrap_kernel(const double *__restrict__ frac, double *__restrict__ frac_w, int64_t n3,
+                            int32_t *__restrict__ status) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n3;
-         i += (int64_t)gridDim.x * blockDim.x)
-        frac_w[i] = fmod(frac[i] + 10.0, 1.0);
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double x = frac[i];
+        if (!isfinite(x)) {
+            atomicCAS(status, 0, STATUS_NONFINITE);
+            frac_w[i] = 0.0;
+        } else {
+            frac_w[i] = fmod(x + 10.0, 1.0);
+        }
+    }
 }
 
 // ----------------------------------------------------------------- brute force
@@ -613,7 +626,8 @@ MATPL_EXPORT int matpl_dp_neighbor_brute(const double *lattice, const double *fr
     if (ntot == 0) return OK;
     MATPL_CHECK_ARG(lattice && frac && frame_start && atom_frame && type_map && rc_type && frac_wrapped &&
                         list && status, "dp_neighbor: null pointer");
-    hipLaunchKernelGGL(wrap_kernel, dim3(grid_cap(ntot * 3)), dim3(256), 0, st, frac, frac_wrapped, ntot * 3);
+    hipLaunchKernelGGL(wrap_kernel, dim3(grid_cap(ntot * 3)), dim3(256), 0, st, frac, frac_wrapped, ntot * 3,
+                       status);
     int rc = check_launch("wrap_kernel");
     if (rc) return rc;
     hipLaunchKernelGGL(brute_kernel, dim3(ceil_div(ntot, 4)), dim3(256), 0, st, lattice, frac_wrapped, frame_start,
@@ -652,7 +666,7 @@ MATPL_EXPORT int matpl_dp_neighbor_cell(const double *lattice_host, const double
     Lattice lat;
     for (int k = 0; k < 9; k++) lat.a[k] = lattice_host[k];
     int rc;
-    hipLaunchKernelGGL(wrap_kernel, dim3(grid_cap(n * 3)), dim3(256), 0, st, frac, frac_wrapped, n * 3);
+    hipLaunchKernelGGL(wrap_kernel, dim3(grid_cap(n * 3)), dim3(256), 0, st, frac, frac_wrapped, n * 3, status);
     if ((rc = check_launch("wrap_kernel"))) return rc;
     if ((rc = fill_async<int32_t>(cell_count, 0, ncells, st))) return rc;
     hipLaunchKernelGGL(bin_kernel, dim3(grid_cap(n)), dim3(256), 0, st, frac_wrapped, n, g, atom_cell, cell_count);

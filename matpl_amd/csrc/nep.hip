@@ -217,7 +217,35 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
-// mode 0: lists; 1: counts; 2: per-type counts. One wavefront per atom.
+// Displacement of replica (ia, ib, ic) of atom n2 from n1, minimum image in the expanded box, and
+// its square; the expression order is the reference's (calculateNepNeighbor.cu), so that the lists
+// and [r, x, y, z] are bit-identical.
+__device__ __forceinline__ double replica_disp(const double *h, const double *bo, const double *pos, int64_t n2,
+                                               int ia, int ib, int ic, double x1, double y1, double z1, double &x12,
+                                               double &y12, double &z12) {
+    const double dx = bo[0] * (double)ia + bo[1] * (double)ib + bo[2] * (double)ic;
+    const double dy = bo[3] * (double)ia + bo[4] * (double)ib + bo[5] * (double)ic;
+    const double dz = bo[6] * (double)ia + bo[7] * (double)ib + bo[8] * (double)ic;
+    x12 = pos[3 * n2] + dx - x1;
+    y12 = pos[3 * n2 + 1] + dy - y1;
+    z12 = pos[3 * n2 + 2] + dz - z1;
+    apply_mic(h, x12, y12, z12);
+    return x12 * x12 + y12 * y12 + z12 * z12;
+}
+
+__device__ __forceinline__ void put_entry(int64_t *NL, double *Ri, int64_t idx, int w, int rij4, int64_t n2, double d2,
+                                          double x, double y, double z) {
+    NL[idx] = n2;
+    double *o = Ri + idx * w;
+    if (rij4) { o[0] = sqrt(d2); o[1] = x; o[2] = y; o[3] = z; }
+    else { o[0] = x; o[1] = y; o[2] = z; }
+}
+
+// mode 0: lists; 1: counts; 2: per-type counts. One wavefront per atom n1, lanes over the frame's
+// atoms n2, each with its replicas (ia, ib, ic) in the reference's order; a list holds the hits in
+// (n2, replica) order. With one replica (the box is at least 2 rc thick) a lane has at most one hit
+// per list and one pass both counts and writes; several replicas take a counting pass, a scan and a
+// writing pass.
 __global__ void __launch_bounds__(256)
 nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int32_t *__restrict__ atom_frame,
                     const int64_t *__restrict__ type_map, const double *__restrict__ pos, double rc_r, double rc_a,
@@ -231,10 +259,12 @@ nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int
     const int64_t N1 = frame_start[f], N2 = frame_start[f + 1];
     const double *h = g.box + 18 * (int64_t)f, *bo = g.box_orig + 9 * (int64_t)f;
     const int64_t *nc = g.num_cell + 3 * (int64_t)f;
-    const int64_t nrep = nc[0] * nc[1] * nc[2];
+    const int na = (int)nc[0], nb = (int)nc[1], ncc = (int)nc[2];
+    const bool one = na == 1 && nb == 1 && ncc == 1;
     const double x1 = pos[3 * n1], y1 = pos[3 * n1 + 1], z1 = pos[3 * n1 + 2];
     const double rr2 = rc_r * rc_r, ra2 = rc_a * rc_a;
     const int w = rij4 ? 4 : 3;
+    const uint64_t below = (1ull << lane) - 1ull;
     int64_t baser = 0, basea = 0;
     int64_t tr[16], ta[16];
     for (int t = 0; t < 16; t++) tr[t] = ta[t] = 0;
@@ -245,20 +275,42 @@ nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int
         int cr = 0, ca = 0;
         // type_map is only given (and only read) for the per-type counts of mode 2
         const int t2 = (valid && mode == 2) ? (int)type_map[n2] : 0;
+        if (one) {
+            // one replica: count and, in mode 0, write in the same pass
+            double x12 = 0.0, y12 = 0.0, z12 = 0.0, d2 = 0.0;
+            if (valid && n1 != n2) {
+                d2 = replica_disp(h, bo, pos, n2, 0, 0, 0, x1, y1, z1, x12, y12, z12);
+                cr = d2 < rr2 ? 1 : 0;
+                ca = d2 < ra2 ? 1 : 0;
+            }
+            if (mode == 2) {
+                for (int t = 0; t < ntypes; t++) {
+                    tr[t] += __popcll(__ballot(cr && t2 == t));
+                    ta[t] += __popcll(__ballot(ca && t2 == t));
+                }
+                continue;
+            }
+            const uint64_t br = __ballot(cr), ba = __ballot(ca);
+            const int64_t kr = baser + __popcll(br & below), ka = basea + __popcll(ba & below);
+            baser += __popcll(br);
+            basea += __popcll(ba);
+            if (mode == 1) continue;
+            if (baser > maxr || basea > maxa) overflow = true;
+            if (cr && kr < maxr) put_entry(NL_r, Ri_r, n1 * maxr + kr, w, rij4, n2, d2, x12, y12, z12);
+            if (ca && ka < maxa) put_entry(NL_a, Ri_a, n1 * maxa + ka, w, rij4, n2, d2, x12, y12, z12);
+            continue;
+        }
         // pass 1: counts
         if (valid) {
-            for (int64_t rep = 0; rep < nrep; rep++) {
-                const int64_t ia = rep / (nc[1] * nc[2]), ib = (rep / nc[2]) % nc[1], ic = rep % nc[2];
-                if (ia == 0 && ib == 0 && ic == 0 && n1 == n2) continue;
-                const double dx = bo[0] * (double)ia + bo[1] * (double)ib + bo[2] * (double)ic;
-                const double dy = bo[3] * (double)ia + bo[4] * (double)ib + bo[5] * (double)ic;
-                const double dz = bo[6] * (double)ia + bo[7] * (double)ib + bo[8] * (double)ic;
-                double x12 = pos[3 * n2] + dx - x1, y12 = pos[3 * n2 + 1] + dy - y1, z12 = pos[3 * n2 + 2] + dz - z1;
-                apply_mic(h, x12, y12, z12);
-                const double d2 = x12 * x12 + y12 * y12 + z12 * z12;
-                if (d2 < rr2) cr++;
-                if (d2 < ra2) ca++;
-            }
+            for (int ia = 0; ia < na; ia++)
+                for (int ib = 0; ib < nb; ib++)
+                    for (int ic = 0; ic < ncc; ic++) {
+                        if (ia == 0 && ib == 0 && ic == 0 && n1 == n2) continue;
+                        double x12, y12, z12;
+                        const double d2 = replica_disp(h, bo, pos, n2, ia, ib, ic, x1, y1, z1, x12, y12, z12);
+                        if (d2 < rr2) cr++;
+                        if (d2 < ra2) ca++;
+                    }
         }
         if (mode == 2) {
             for (int t = 0; t < ntypes; t++) {
@@ -269,22 +321,8 @@ nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int
             }
             continue;
         }
-        // list offsets: with one replica (the box is large enough) each lane has at most one hit per
-        // list, and a ballot gives the prefix; several replicas need the scan
-        int ir, ia_, sumr, suma;
-        if (nrep == 1) {
-            const uint64_t br = __ballot(cr > 0), ba = __ballot(ca > 0);
-            const uint64_t below = (1ull << lane) - 1ull;
-            ir = __popcll(br & below) + cr;
-            ia_ = __popcll(ba & below) + ca;
-            sumr = __popcll(br);
-            suma = __popcll(ba);
-        } else {
-            ir = wave_incl_scan(cr);
-            ia_ = wave_incl_scan(ca);
-            sumr = __shfl(ir, WAVE - 1, WAVE);
-            suma = __shfl(ia_, WAVE - 1, WAVE);
-        }
+        const int ir = wave_incl_scan(cr), ia_ = wave_incl_scan(ca);
+        const int sumr = __shfl(ir, WAVE - 1, WAVE), suma = __shfl(ia_, WAVE - 1, WAVE);
         const int64_t offr = baser + ir - cr, offa = basea + ia_ - ca;
         baser += sumr;
         basea += suma;
@@ -293,30 +331,15 @@ nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int
         // pass 2: write in (n2, replica) order
         if (valid && (cr > 0 || ca > 0)) {
             int64_t kr = offr, ka = offa;
-            for (int64_t rep = 0; rep < nrep; rep++) {
-                const int64_t ia = rep / (nc[1] * nc[2]), ib = (rep / nc[2]) % nc[1], ic = rep % nc[2];
-                if (ia == 0 && ib == 0 && ic == 0 && n1 == n2) continue;
-                const double dx = bo[0] * (double)ia + bo[1] * (double)ib + bo[2] * (double)ic;
-                const double dy = bo[3] * (double)ia + bo[4] * (double)ib + bo[5] * (double)ic;
-                const double dz = bo[6] * (double)ia + bo[7] * (double)ib + bo[8] * (double)ic;
-                double x12 = pos[3 * n2] + dx - x1, y12 = pos[3 * n2 + 1] + dy - y1, z12 = pos[3 * n2 + 2] + dz - z1;
-                apply_mic(h, x12, y12, z12);
-                const double d2 = x12 * x12 + y12 * y12 + z12 * z12;
-                if (d2 < rr2) {
-                    NL_r[n1 * maxr + kr] = n2;
-                    double *o = Ri_r + (n1 * maxr + kr) * w;
-                    if (rij4) { o[0] = sqrt(d2); o[1] = x12; o[2] = y12; o[3] = z12; }
-                    else { o[0] = x12; o[1] = y12; o[2] = z12; }
-                    kr++;
-                }
-                if (d2 < ra2) {
-                    NL_a[n1 * maxa + ka] = n2;
-                    double *o = Ri_a + (n1 * maxa + ka) * w;
-                    if (rij4) { o[0] = sqrt(d2); o[1] = x12; o[2] = y12; o[3] = z12; }
-                    else { o[0] = x12; o[1] = y12; o[2] = z12; }
-                    ka++;
-                }
-            }
+            for (int ia = 0; ia < na; ia++)
+                for (int ib = 0; ib < nb; ib++)
+                    for (int ic = 0; ic < ncc; ic++) {
+                        if (ia == 0 && ib == 0 && ic == 0 && n1 == n2) continue;
+                        double x12, y12, z12;
+                        const double d2 = replica_disp(h, bo, pos, n2, ia, ib, ic, x1, y1, z1, x12, y12, z12);
+                        if (d2 < rr2) put_entry(NL_r, Ri_r, n1 * maxr + kr++, w, rij4, n2, d2, x12, y12, z12);
+                        if (d2 < ra2) put_entry(NL_a, Ri_a, n1 * maxa + ka++, w, rij4, n2, d2, x12, y12, z12);
+                    }
         }
     }
     if (lane == 0) {

@@ -31,6 +31,7 @@ import numpy as np
 import torch
 
 from .dpnet import DPConfig, DPWeights, Layer
+from .typedp import DEFAULT_PROPERTIES, TypeDPWeights, fold, type_vectors
 
 
 def _numpy_globals() -> list:
@@ -75,7 +76,13 @@ def _net(sd: Dict[str, torch.Tensor], prefix: str, kind: int) -> List[Layer]:
 
 
 def read_dp_checkpoint(path: str) -> Tuple[DPWeights, List[int], Dict[str, Any]]:
-    """(weights, atomic numbers in type order, {epoch, energy_shift, sij_max, compressed})."""
+    """(weights, atomic numbers in type order, {epoch, energy_shift, sij_max, compressed, typedp}).
+
+    A type-embedding (TypeDP) checkpoint is read too: its shared embedding net on [s, v(t_j)]
+    (src/model/dp_dp_typ_emb.py:66-97, 361-403) with the type vectors of its physical properties
+    (json_file model.descriptor.type_embedding.physical_property, default nn_feature_type.py:35)
+    comes back exactly folded into per-pair DPWeights (matpl_amd.typedp.fold), and unfolded as
+    info["typedp"] (TypeDPWeights)."""
     ck = load_checkpoint(path)
     for key in ("json_file", "state_dict", "davg", "dstd", "atom_type_order"):
         if key not in ck:
@@ -87,13 +94,26 @@ def read_dp_checkpoint(path: str) -> Tuple[DPWeights, List[int], Dict[str, Any]]
     order = np.asarray(ck["atom_type_order"]).reshape(-1).astype(np.int64).tolist()
     T = len(order)
     desc = js.get("model", {}).get("descriptor", {})
-    if desc.get("type_embedding") or any(k.startswith(f"embedding_net.{T * T}.") for k in sd):
-        raise NotImplementedError(f"{path}: type-embedding DP checkpoints are not read by this reader")
-    emb = [_net(sd, f"embedding_net.{p}", 0) for p in range(T * T)]
+    nemb = 0
+    while f"embedding_net.{nemb}.layers.0.weight" in sd:
+        nemb += 1
+    typedp = bool(desc.get("type_embedding")) or str(js.get("model_type", "")).upper() == "TYPEDP" or (
+        nemb and _np(sd["embedding_net.0.layers.0.weight"]).shape[0] > 1)
     fit = [_net(sd, f"fitting_net.{t}", 1) for t in range(T)]
-    if emb[0][0][0].shape[0] != 1:
-        raise NotImplementedError(f"{path}: embedding input width {emb[0][0][0].shape[0]} (type-embedding DP)")
-    E = emb[0][-1][0].shape[1]
+    if typedp:
+        # TypeDP (dp_dp_typ_emb.py:66-97): the last embedding net is the shared one, on [s, v(t_j)]
+        shared = _net(sd, f"embedding_net.{nemb - 1}", 0)
+        props = list((desc.get("type_embedding") or {}).get("physical_property", DEFAULT_PROPERTIES))
+        if shared[0][0].shape[0] != 1 + len(props):
+            raise ValueError(f"{path}: shared embedding input width {shared[0][0].shape[0]}, expected "
+                             f"1 + {len(props)} physical properties")
+        emb = None
+    else:
+        emb = [_net(sd, f"embedding_net.{p}", 0) for p in range(T * T)]
+        if emb[0][0][0].shape[0] != 1:
+            raise ValueError(f"{path}: embedding input width {emb[0][0][0].shape[0]} without a type embedding")
+    first = shared if typedp else emb[0]
+    E = first[-1][0].shape[1]
     M2 = fit[0][0][0].shape[0] // E
     m = int(js.get("max_neigh_num", 0))
     davg_full, dstd_full = _np(ck["davg"]), _np(ck["dstd"])
@@ -109,14 +129,19 @@ def read_dp_checkpoint(path: str) -> Tuple[DPWeights, List[int], Dict[str, Any]]
     if not (np.all(davg == davg[:, :1]) and np.all(dstd == dstd[:, :1])):
         raise ValueError(f"{path}: davg/dstd vary over neighbor slots; this build keeps one [4] vector per type")
     cfg = DPConfig(ntypes=T, rcut=float(desc.get("Rmax", 6.0)), rcut_smooth=float(desc.get("Rmin", 0.5)), m=m,
-                   M2=M2, emb_sizes=[L[0].shape[1] for L in emb[0]], fit_sizes=[L[0].shape[1] for L in fit[0]],
-                   emb_bias=emb[0][0][1] is not None,
-                   emb_resnet_dt=any(L[2] is not None for L in emb[0]),
+                   M2=M2, emb_sizes=[L[0].shape[1] for L in first], fit_sizes=[L[0].shape[1] for L in fit[0]],
+                   emb_bias=first[0][1] is not None,
+                   emb_resnet_dt=any(L[2] is not None for L in first),
                    fit_bias=fit[0][0][1] is not None,
                    fit_resnet_dt=any(L[2] is not None for L in fit[0]))
-    w = DPWeights(cfg, emb, fit, davg[:, 0].copy(), dstd[:, 0].copy())
     info = {"epoch": ck.get("epoch"), "energy_shift": ck.get("energy_shift"), "sij_max": ck.get("sij_max"),
-            "compressed": "compress" in ck}
+            "compressed": "compress" in ck, "typedp": None}
+    if typedp:
+        tw = TypeDPWeights(cfg, type_vectors(order, props), shared, fit, davg[:, 0].copy(), dstd[:, 0].copy())
+        info["typedp"] = tw
+        info["physical_property"] = props
+        return fold(tw), order, info
+    w = DPWeights(cfg, emb, fit, davg[:, 0].copy(), dstd[:, 0].copy())
     return w, order, info
 
 

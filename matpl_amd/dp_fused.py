@@ -7,6 +7,9 @@ Three stages per evaluation, all on the current stream:
   2. fitting: DR = A^T A[:, :M2] -> fitting net -> Ei, and dEtot/dA;
   3. ``matpl_dp_fused_backward``: recomputed embedding forward + backward,
      dE/dRi -> force scatter and virial (dp_dp.py:417-469 semantics).
+     With ``store=True`` the forward also keeps G(s) and dG/ds of every
+     neighbor (``matpl_dp_fused_forward_store``) and the backward reads them
+     (``matpl_dp_fused_backward_stored``) instead of running the net again.
 
 Precision modes: "bf16" runs the embedding GEMMs on BF16 matrix cores with FP32
 accumulation (and a fast exp2/rcp tanh); "fp32" runs them on FP32 matrix
@@ -225,10 +228,18 @@ class FusedDP:
     its tiles full.
     """
 
-    def __init__(self, w: DPWeights, precision: str = "bf16", device="cuda", fitting: str = "mfma", compress=None):
+    def __init__(self, w: DPWeights, precision: str = "bf16", device="cuda", fitting: str = "mfma", compress=None,
+                 store: bool = False):
         """fitting: "mfma" (hand-written matrix-core kernel) or "torch" (FP32 torch + autograd).
         compress: a matpl_amd.compress.CompressTable; the fused kernels then evaluate the embedding
-        from the table (G and dG/ds by Horner) instead of the net, as DPTorch(compress=...) does."""
+        from the table (G and dG/ds by Horner) instead of the net, as DPTorch(compress=...) does.
+        store: the forward keeps G(s) and dG/ds of every in-cutoff neighbor in a device buffer
+        (fp16 pairs in BF16 mode, floats in FP32 mode) and the backward reads them instead of
+        evaluating the embedding net a second time (net mode only)."""
+        if store and compress is not None:
+            raise ValueError("FusedDP: store mode applies to the embedding net, not to a compressed table")
+        self.store = store
+        self._gbuf = None
         self.w = w
         self.cfg = w.cfg
         self.device = torch.device(device)
@@ -286,7 +297,17 @@ class FusedDP:
         else:
             tb = self.table
             args, sfx = head + (_ptr(tb.tab), tb.order, tb.rows, tb.n_fine, tb.x0, tb.dx) + tail, "_compressed"
-        _lib.call("matpl_dp_fused_forward" + sfx, *args, _ptr(A), _ptr(status), _stream())
+        gbuf = None
+        if self.store:
+            words = _lib.query("matpl_dp_fused_gstore_words", N, T, mnl, cfg.m, self.fw.E, int(self.fw.bf16))
+            if words < 0:
+                raise ValueError("FusedDP: no store layout for this embedding width")
+            if self._gbuf is None or self._gbuf.numel() < words or self._gbuf.device != dev:
+                self._gbuf = torch.empty(max(words, 4), dtype=torch.int32, device=dev)
+            gbuf = self._gbuf
+            _lib.call("matpl_dp_fused_forward_store", *args, _ptr(A), _ptr(gbuf), _ptr(status), _stream())
+        else:
+            _lib.call("matpl_dp_fused_forward" + sfx, *args, _ptr(A), _ptr(status), _stream())
         if self.fitting == "mfma":
             key = tuple(segments(type_ranges))
             if key not in self._mfma_fit:
@@ -307,8 +328,12 @@ class FusedDP:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
-        _lib.call("matpl_dp_fused_backward" + sfx, *args, _ptr(dA), _ptr(rev), _ptr(fslot), _ptr(fcenter), _ptr(F),
-                  _ptr(V), _ptr(status), _stream())
+        if gbuf is not None:
+            _lib.call("matpl_dp_fused_backward_stored", *args, _ptr(dA), _ptr(gbuf), _ptr(rev), _ptr(fslot),
+                      _ptr(fcenter), _ptr(F), _ptr(V), _ptr(status), _stream())
+        else:
+            _lib.call("matpl_dp_fused_backward" + sfx, *args, _ptr(dA), _ptr(rev), _ptr(fslot), _ptr(fcenter),
+                      _ptr(F), _ptr(V), _ptr(status), _stream())
         if self.events is not None:
             e1.record()
             self.events.append((e0, e1))

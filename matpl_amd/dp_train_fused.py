@@ -1,0 +1,307 @@
+"""Batched DP training step on hand-written kernels: forward, forces and the force-loss double
+backward of the embedding nets on the matrix cores (csrc/dp_fused.hip, csrc/dp_train.hip).
+
+The reference trains DP (src/PWMLFF/dp_mods/dp_trainer.py:163-275) by
+dE = autograd.grad(Etot, Ri, create_graph=True) (src/model/dp_dp.py:429), forces from
+calculateForce and loss.backward() through all of it (its force-loss edge is
+calculateForceGrad.cu:9-90). Here, for a packed batch of frames (each with its own lattice, type
+list and neighbor list with image shifts):
+
+  1. matpl_dp_fused_forward: A_i = c_i (sum_j Ri_ij (x) G(s_ij) + empty slots), BF16 or FP32
+     embedding on the matrix cores, c_i = 1 / (m * type_nums of i's frame) (dp_dp.py:413);
+  2. the fitting nets in torch: E_i, and Abar = dE/dA with create_graph (dp_dp.py:304-342);
+  3. matpl_dp_fused_backward + gather: forces F (atomic-free);
+  4. the loss L = pe mean_f ((E_f - E^_f) / N_f)^2 + pf mean (F - F^)^2 and the force residual
+     r = dL/dF;
+  5. matpl_dp_train_tangent: dA = the tangent of A along r (forward mode through the nets);
+  6. torch, through the fitting nets only: Ahat = d(L_E - <Abar, dA>)/dA and the fitting-net
+     gradients (the force loss enters the fitting side as -<Abar, dA>);
+  7. matpl_dp_train_embgrad: the embedding-net gradients from (Ahat, Abar, r), recomputing the
+     nets' activations (nothing per neighbor is stored between the kernels);
+  8. torch: the gradient of the empty-slot term sum (m - n_ij) Ri_pad (x) G(s_pad) (one net
+     evaluation per type pair).
+Loss and gradients equal the plain FP64 path's (DPTorch with create_graph, matpl_amd.dp_train.
+plain_grads) within each precision mode's rounding; tests/test_gpu_dp_train_fused.py checks them,
+and the plain path against central differences of the oracle's loss.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List
+
+import numpy as np
+import torch
+
+from . import _lib
+from .calcops import _ptr, _stream
+from .dp import _Net
+from .dp_fused import M_W2B, M_W2F, M_W3B, M_W3F, NMAT, NTI, NVEC, TANH_K, V_B1, V_B2, V_B3, V_GPAD, V_R2, V_R3, \
+    V_W1, feat, type_masks
+from .dpnet import DPWeights
+from .neighbor import find_neighbore_batch, reverse_map
+
+
+@dataclass
+class FusedBatch:
+    """Frames packed along the atom axis in the fused kernels' layout, with labels."""
+    nframes: int
+    natoms: int
+    pos: torch.Tensor            # [N, 3] float64, wrapped into each frame's cell
+    lattice: torch.Tensor        # [F, 9] float64
+    type_map: torch.Tensor       # [N] int32
+    atom_frame: torch.Tensor     # [N] int32
+    frame_start: torch.Tensor    # [F + 1] int64
+    frame_natoms: torch.Tensor   # [F] float64
+    masks: torch.Tensor          # [F] int32 bit masks of each frame's type list
+    type_present: torch.Tensor   # [F, T] bool
+    by_type: List[torch.Tensor]  # atoms of each type (int64)
+    nl_list: torch.Tensor        # [N, T*m] int32, 1-based within the frame
+    nl_shift: torch.Tensor       # [N, T*m] int32 packed image shifts
+    nl_counts: torch.Tensor      # [N, T] int32
+    rev: torch.Tensor            # [N, T*m] int32 partner slot
+    E_label: torch.Tensor        # [F] float64
+    F_label: torch.Tensor        # [N, 3] float64
+
+
+def pack_fused_batch(frames, cfg, device="cuda", seed=0, type_present=None) -> FusedBatch:
+    """The neighbor lists (cutoff rcut, m per type, image shifts) of all frames in one launch, the
+    reverse slot map, and seeded synthetic labels drawn as matpl_amd.dp_train.pack_batch draws them
+    (same seed, same labels)."""
+    T, m = cfg.ntypes, cfg.m
+    nl = find_neighbore_batch([f.lattice for f in frames], [f.frac() for f in frames], [f.types for f in frames],
+                              T, m, cfg.rcut, want_dR=False, want_shift=True, device=device)
+    sizes = [f.natoms for f in frames]
+    N = sum(sizes)
+    fs = torch.as_tensor(np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64), device=device)
+    af = torch.as_tensor(np.repeat(np.arange(len(frames), dtype=np.int32), sizes), device=device)
+    lat = torch.as_tensor(np.stack([np.asarray(f.lattice, np.float64).reshape(9) for f in frames]), device=device)
+    pos = torch.cat([nl.frac_wrapped[int(fs[k]):int(fs[k + 1])] @ lat[k].reshape(3, 3)
+                     for k in range(len(frames))]).contiguous()
+    tm = torch.as_tensor(np.concatenate([f.types for f in frames]).astype(np.int32), device=device)
+    rev, rst = reverse_map(nl, tm, m, frame_start=fs, atom_frame=af)
+    if int(rst.item()) != 0:
+        raise RuntimeError("pack_fused_batch: neighbor list is not symmetric")
+    if type_present is None:
+        tp = np.zeros((len(frames), T), bool)
+        for k, f in enumerate(frames):
+            tp[k, np.unique(f.types)] = True
+    else:
+        tp = np.asarray(type_present, bool).reshape(len(frames), T)
+        for k, f in enumerate(frames):
+            if not tp[k, np.unique(f.types)].all():
+                raise ValueError(f"type_present: frame {k} has atoms of a type outside its list")
+    rng = np.random.default_rng(seed)
+    E_label = torch.as_tensor(rng.normal(0.0, 0.1, len(frames)) * np.asarray(sizes), device=device)
+    F_label = torch.as_tensor(rng.normal(0.0, 0.5, (N, 3)), device=device)
+    tm64 = tm.to(torch.int64)
+    return FusedBatch(len(frames), N, pos, lat, tm, af, fs, torch.as_tensor(sizes, dtype=torch.float64, device=device),
+                      torch.as_tensor(type_masks(tp, len(frames), T).view(np.int32), device=device),
+                      torch.as_tensor(tp, device=device), [torch.nonzero(tm64 == t).reshape(-1) for t in range(T)],
+                      nl.list, nl.shift, nl.counts, rev, E_label, F_label)
+
+
+class _EmbPack:
+    """Gathers the embedding parameters (torch, on the device) into the fused kernels' per-pair
+    operand layouts (matpl_amd.dp_fused.pack_fused's layout, built on the device every step)."""
+
+    def __init__(self, E: int, device):
+        h = np.repeat([0, 1], 16)
+        r = np.tile(np.arange(16), 2)
+        f = np.array([feat(rr, hh) for hh, rr in zip(h, r)]).reshape(2, 16)
+        self.fvec = torch.as_tensor(np.where(f < E, f, 0), device=device)
+        self.mvec = torch.as_tensor(f < E, device=device)
+        lanes = np.arange(64)
+        hl, il = lanes // 32, lanes % 32
+        fm = np.array([[feat(rr, hh) for rr in range(16)] for hh in hl])          # [64, 16]
+        im = np.repeat(il[:, None], 16, 1)
+        ok = (fm < E) & (im < E)
+        self.fm = torch.as_tensor(np.where(ok, fm, 0), device=device)
+        self.im = torch.as_tensor(np.where(ok, im, 0), device=device)
+        self.ok = torch.as_tensor(ok, device=device)
+
+    def vec(self, v: torch.Tensor) -> torch.Tensor:
+        return torch.where(self.mvec, v[self.fvec], torch.zeros((), dtype=v.dtype, device=v.device))
+
+    def fwd(self, W):   # [lane][r] = W[f(r,h)][lane % 32]
+        return torch.where(self.ok, W[self.fm, self.im], torch.zeros((), dtype=W.dtype, device=W.device))
+
+    def bwd(self, W):   # [lane][r] = W[lane % 32][f(r,h)]
+        return torch.where(self.ok, W[self.im, self.fm], torch.zeros((), dtype=W.dtype, device=W.device))
+
+
+class DPTrainFused(torch.nn.Module):
+    """The DP model's parameters (FP32, DPTorch's module layout) and its training step on the
+    hand-written kernels. precision: "bf16" (BF16 matrix-core products, FP32 accumulation) or
+    "fp32"."""
+
+    def __init__(self, w: DPWeights, precision: str = "bf16", device="cuda", pe: float = 1.0, pf: float = 1.0):
+        super().__init__()
+        cfg = w.cfg
+        if precision not in ("bf16", "fp32"):
+            raise ValueError("precision must be 'bf16' or 'fp32'")
+        es = cfg.emb_sizes
+        if len(es) != 3 or es[1] != es[0] or es[2] != es[0] or es[0] not in (16, 25):
+            raise ValueError("the training kernels need embedding sizes [E, E, E] with E in {16, 25}")
+        if cfg.emb_resnet_dt:
+            raise ValueError("the training kernels need an embedding without resnet_dt (the reference default)")
+        self.cfg = cfg
+        self.E = es[0]
+        self.bf16 = precision == "bf16"
+        self.pe, self.pf = pe, pf
+        T = cfg.ntypes
+        f32 = torch.float32
+        self.emb = torch.nn.ModuleList([_Net(w.emb[p], 0, f32, device) for p in range(T * T)])
+        self.fit = torch.nn.ModuleList([_Net(w.fit[t], 1, f32, device) for t in range(T)])
+        dev = torch.device(device)
+        tinfo = np.zeros((T, NTI), np.float32)
+        ri_pad = -w.davg / w.dstd
+        for ti in range(T):
+            tinfo[ti, 0:4] = w.davg[ti]
+            tinfo[ti, 4:8] = 1.0 / w.dstd[ti]
+            tinfo[ti, 8:12] = w.dstd[ti]
+            tinfo[ti, 12:16] = ri_pad[ti]
+        self.register_buffer("tinfo", torch.as_tensor(tinfo, device=dev))
+        self.register_buffer("ri_pad", torch.as_tensor(ri_pad, dtype=f32, device=dev))
+        self._pk = _EmbPack(self.E, dev)
+        self.gsize = _lib.query("matpl_dp_train_grad_size", self.E)
+        self.chunk = 0          # atoms per wavefront of the gradient kernel (0: chosen from the batch)
+
+    # ---------------------------------------------------------------- weights
+    def _pack(self):
+        """(vec, mat) from the current parameters, and G at each pair's empty-slot input."""
+        cfg, T, pk = self.cfg, self.cfg.ntypes, self._pk
+        K = TANH_K if self.bf16 else 1.0
+        vec = torch.zeros((T * T, NVEC, 2, 16), dtype=torch.float32, device=self.tinfo.device)
+        mat = torch.zeros((T * T, NMAT, 64, 16), dtype=torch.float32, device=self.tinfo.device)
+        with torch.no_grad():
+            for p in range(T * T):
+                net = self.emb[p]
+                W1, W2, W3 = net.W[0], net.W[1], net.W[2]
+                b1, b2, b3 = net.b[0], net.b[1], net.b[2]
+                gpad = net(self.ri_pad[p // T, 0].view(1, 1)).reshape(-1)
+                vec[p, V_W1] = pk.vec(W1.reshape(-1)) * K
+                vec[p, V_B1] = pk.vec(b1) * K
+                vec[p, V_B2] = pk.vec(b2) * K
+                vec[p, V_B3] = pk.vec(b3) * K
+                vec[p, V_R2] = pk.vec(torch.ones_like(b2))
+                vec[p, V_R3] = pk.vec(torch.ones_like(b3))
+                vec[p, V_GPAD] = pk.vec(gpad)
+                mat[p, M_W2F] = pk.fwd(W2) * K
+                mat[p, M_W3F] = pk.fwd(W3) * K
+                mat[p, M_W2B] = pk.bwd(W2)
+                mat[p, M_W3B] = pk.bwd(W3)
+        return vec.contiguous(), (mat.to(torch.bfloat16) if self.bf16 else mat).contiguous()
+
+    # ---------------------------------------------------------------- fitting
+    def _energies(self, A: torch.Tensor, b: FusedBatch):
+        M2 = self.cfg.M2
+        Ei = torch.zeros(b.natoms, dtype=A.dtype, device=A.device)
+        for t, idx in enumerate(b.by_type):
+            if idx.numel() == 0:
+                continue
+            At = A[idx]
+            DR = (At.transpose(1, 2) @ At[:, :, :M2]).reshape(idx.numel(), -1)
+            Ei = Ei.index_copy(0, idx, self.fit[t](DR).reshape(-1))
+        return Ei
+
+    # ---------------------------------------------------------------- one step
+    def _head(self, b: FusedBatch):
+        cfg = self.cfg
+        m = cfg.m
+        return (_ptr(b.pos), _ptr(b.lattice), _ptr(b.atom_frame), _ptr(b.frame_start), _ptr(b.masks),
+                _ptr(b.nl_list), _ptr(b.nl_shift), _ptr(b.nl_counts), _ptr(b.type_map))
+
+    def _tail(self, b: FusedBatch, mnl: int):
+        cfg = self.cfg
+        return (_ptr(self.tinfo), b.natoms, cfg.ntypes, mnl, cfg.m, self.E, int(self.bf16), float(cfg.rcut),
+                float(cfg.rcut_smooth), 1.0 / (cfg.m * cfg.ntypes))
+
+    def loss_and_grads(self, b: FusedBatch):
+        """(loss, gradients aligned with self.parameters(); None for parameters the loss does not reach)."""
+        cfg = self.cfg
+        T, m, N, E = cfg.ntypes, cfg.m, b.natoms, self.E
+        dev = b.pos.device
+        mnl = b.nl_list.shape[1] // T
+        vec, mat = self._pack()
+        head, tail = self._head(b), self._tail(b, mnl)
+        status = torch.zeros(1, dtype=torch.int32, device=dev)
+        # 1. descriptor A
+        A = torch.empty((N, 4, E), dtype=torch.float32, device=dev)
+        _lib.call("matpl_dp_fused_forward", *head, _ptr(vec), _ptr(mat), *tail, _ptr(A), _ptr(status), _stream())
+        # 2. fitting, Abar = dE/dA kept differentiable
+        A.requires_grad_()
+        Ei = self._energies(A, b)
+        Ef = torch.zeros(b.nframes, dtype=Ei.dtype, device=dev).index_add(0, b.atom_frame.to(torch.int64), Ei)
+        (Abar,) = torch.autograd.grad(Ei.sum(), A, create_graph=True)
+        abar = Abar.detach().contiguous()
+        # 3. forces
+        F = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        V = torch.zeros((b.nframes, 9), dtype=torch.float64, device=dev)
+        fslot = torch.empty((N * T * mnl, 4), dtype=torch.float32, device=dev)
+        fcenter = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        _lib.call("matpl_dp_fused_backward", *head, _ptr(vec), _ptr(mat), *tail, _ptr(abar), _ptr(b.rev), _ptr(fslot),
+                  _ptr(fcenter), _ptr(F), _ptr(V), _ptr(status), _stream())
+        _lib.call("matpl_dp_fused_gather", _ptr(b.nl_counts), _ptr(fslot), _ptr(fcenter), _ptr(F), T, mnl, N, _stream())
+        # 4. loss and the force residual r = dL/dF
+        le = (((Ef.double() - b.E_label) / b.frame_natoms) ** 2).mean()
+        lf = ((F.double() - b.F_label) ** 2).mean()
+        resid = (2.0 * self.pf / (3 * N) * (F.double() - b.F_label)).to(torch.float32).contiguous()
+        # 5. tangent of A along r
+        dA = torch.empty((N, 4, E), dtype=torch.float32, device=dev)
+        nin = torch.empty((N, T), dtype=torch.int32, device=dev)
+        _lib.call("matpl_dp_train_tangent", *head, _ptr(vec), _ptr(mat), *tail, _ptr(resid), _ptr(dA), _ptr(nin),
+                  _ptr(status), _stream())
+        # 6. fitting side: the force loss enters as -<Abar, dA>
+        fparams = [q for q in self.fit.parameters()]
+        ltot = self.pe * le - (Abar * dA).sum()
+        g = torch.autograd.grad(ltot, fparams + [A], allow_unused=True)
+        gfit, ahat = list(g[:-1]), g[-1].detach().contiguous()
+        # 7. embedding gradients
+        grad = torch.zeros((T * T, self.gsize), dtype=torch.float32, device=dev)
+        chunk = self.chunk or max(8, -(-N * T // 8192))
+        _lib.call("matpl_dp_train_embgrad", *head, _ptr(vec), _ptr(mat), *tail, _ptr(resid), _ptr(abar), _ptr(ahat),
+                  _ptr(grad), chunk, _ptr(status), _stream())
+        if int(status.item()) != 0:
+            raise OverflowError("more than maxNeighborNum neighbors of one type within rcut")
+        gemb = self._emb_grads(grad, b, ahat, nin)
+        loss = (self.pe * le + self.pf * lf).detach()
+        return loss, gemb + gfit
+
+    def _emb_grads(self, grad, b: FusedBatch, ahat, nin):
+        """Per-parameter embedding gradients: the kernel's sums plus the empty-slot term."""
+        cfg, T, E, m = self.cfg, self.cfg.ntypes, self.E, self.cfg.m
+        # empty slots: u[p] = sum_i c_i listed(i, tj) (m - n_ij) ahat_i^T Ri_pad(ti)
+        listed = b.type_present[b.atom_frame.to(torch.int64)].to(torch.float32)         # [N, T]
+        c = 1.0 / (m * listed.sum(1))                                                   # [N]
+        npad = (m - nin.to(torch.float32)) * listed                                     # [N, T]
+        ti = b.type_map.to(torch.int64)
+        proj = torch.einsum("nwe,nw->ne", ahat, self.ri_pad[ti])                        # [N, E]
+        wsum = (c[:, None] * npad)[:, :, None] * proj[:, None, :]                        # [N, T, E]
+        u = torch.zeros((T, T, E), dtype=torch.float32, device=grad.device).index_add_(0, ti, wsum)
+        has_atoms = [bool(idx.numel()) for idx in b.by_type]
+        listed_any = [bool(b.type_present[:, t].any()) for t in range(T)]
+        out = []
+        o = self._offsets()
+        for p in range(T * T):
+            a, t = divmod(p, T)
+            net = self.emb[p]
+            if not (has_atoms[a] and listed_any[t]):
+                out += [None] * (3 * len(net.W))
+                continue
+            gpad = net(self.ri_pad[a, 0].view(1, 1)).reshape(-1)
+            pg = torch.autograd.grad((u[a, t] * gpad).sum(), [net.W[0], net.W[1], net.W[2], net.b[0], net.b[1],
+                                                               net.b[2]])
+            gk = grad[p]
+            out += [gk[o["W1"]:o["W1"] + E].view(1, E) + pg[0], gk[o["W2"]:o["W2"] + E * E].view(E, E) + pg[1],
+                    gk[o["W3"]:o["W3"] + E * E].view(E, E) + pg[2], gk[o["B1"]:o["B1"] + E] + pg[3],
+                    gk[o["B2"]:o["B2"] + E] + pg[4], gk[o["B3"]:o["B3"] + E] + pg[5]]
+            out += [None] * len(net.r)
+        return out
+
+    def _offsets(self):
+        E = self.E
+        W3, B3 = 0, E * E
+        W2 = B3 + E
+        B2 = W2 + E * E
+        W1 = B2 + E
+        return {"W3": W3, "B3": B3, "W2": W2, "B2": B2, "W1": W1, "B1": W1 + E}

@@ -33,8 +33,9 @@ def _affine(x: torch.Tensor, M: np.ndarray, out=None) -> torch.Tensor:
 
 class DPMD:
     def __init__(self, frame: Frame, w: DPWeights, velocities: np.ndarray, precision="bf16", skin=0.5,
-                 mnl=None, dt=1.0, rebuild_every=10, device="cuda", compress=None):
-        """compress: a matpl_amd.compress.CompressTable for the tabulated embedding."""
+                 mnl=None, dt=1.0, rebuild_every=10, device="cuda", compress=None, store=False):
+        """compress: a matpl_amd.compress.CompressTable for the tabulated embedding.
+        store: FusedDP store mode (G and dG/ds kept from the forward for the backward)."""
         self.dev = torch.device(device)
         self.cfg = w.cfg
         self.N = frame.natoms
@@ -50,7 +51,7 @@ class DPMD:
         self.mnl = mnl or self.cfg.m
         self.dt = dt
         self.rebuild_every = rebuild_every
-        self.model = FusedDP(w, precision, self.dev, compress=compress)
+        self.model = FusedDP(w, precision, self.dev, compress=compress, store=store)
         tm = frame.types
         self.type_ranges = [(int(np.searchsorted(tm, t, "left")), int(np.searchsorted(tm, t, "right")))
                             for t in range(self.cfg.ntypes)]

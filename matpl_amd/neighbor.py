@@ -32,13 +32,18 @@ class NeighborList:
     counts: torch.Tensor        # [Ntot, T] int32
     shift: Optional[torch.Tensor]  # [Ntot, T*m] int32 packed image shift
     frac_wrapped: torch.Tensor  # [Ntot, 3] float64
-    status: torch.Tensor        # [1] int32, 0 or 1 + first overflowing atom
+    status: torch.Tensor        # [1] int32: 0, 1 + first overflowing atom, or STATUS_NONFINITE
 
     def check(self):
         s = int(self.status.item())
+        if s == STATUS_NONFINITE:
+            raise ValueError("neighbor search: a position is not finite (NaN or inf)")
         if s != 0:
             raise OverflowError(f"maxNeighborNum too small (atom {s - 1})")
         return self
+
+
+STATUS_NONFINITE = -4           # csrc/neighbor.hip: a coordinate is NaN or infinite
 
 
 # small device-side constants (cutoffs per type, one-frame index arrays) are made once and

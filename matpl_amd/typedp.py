@@ -14,9 +14,9 @@ layer's bias, which becomes b1 + v(t_j) W1[1:]. ``fold`` builds those DPWeights;
 ``TypeDPTorch`` keeps the reference's concatenated-input form on the HIP ops.
 
 Type vectors: ``type_vectors`` min-max normalizes per-element property values,
-(value - min) / (max - min) over the property's whole table (atom_type_emb_dict.py:130-153).
-Only "atomic_number" is built in (range 1..109, the table's keys); other properties take the
-caller's values and ranges.
+(value - min) / (max - min) over the property's whole table (atom_type_emb_dict.py:130-153). The
+five default properties (nn_feature_type.py:35) come from the element table in
+matpl_amd.elements; other properties take the caller's values and ranges.
 """
 from __future__ import annotations
 
@@ -29,23 +29,47 @@ import torch
 from .dp import DPTorch, _Net
 from .dpnet import DPConfig, DPWeights, Layer
 
+from . import elements
+
 ATOMIC_NUMBER_RANGE = (1.0, 109.0)
+DEFAULT_PROPERTIES = ("atomic_number", "atom_radius", "atom_mass", "electron_affin", "pauling")
 
 
-def type_vectors(atomic_numbers: Sequence[int], properties: Sequence[str] = ("atomic_number",),
+def property_range(name: str) -> Tuple[float, float]:
+    """(min, max) of a built-in property over the elements the table gives a number for."""
+    if name == "atomic_number":
+        return ATOMIC_NUMBER_RANGE
+    vals = [v for v in elements.PROPERTIES[name] if v is not None and not np.isnan(v)]
+    return float(min(vals)), float(max(vals))
+
+
+def type_vectors(atomic_numbers: Sequence[int], properties: Sequence[str] = DEFAULT_PROPERTIES,
                  values: Optional[Dict[str, Dict[int, float]]] = None,
                  ranges: Optional[Dict[str, Tuple[float, float]]] = None) -> np.ndarray:
-    """[T, P] normalized property vectors, one row per element (by atomic number)."""
+    """[T, P] normalized property vectors, one row per element (by atomic number). Built-in
+    properties (DEFAULT_PROPERTIES) are read from the element table unless ``values`` gives them."""
     out = np.zeros((len(atomic_numbers), len(properties)))
     for p, name in enumerate(properties):
-        if name == "atomic_number":
-            lo, hi = ATOMIC_NUMBER_RANGE
-            vals = [float(z) for z in atomic_numbers]
-        else:
-            if not values or name not in values or not ranges or name not in ranges:
-                raise ValueError(f"type_vectors: give the values and (min, max) range of property '{name}'")
+        if values and name in values:
+            if not ranges or name not in ranges:
+                raise ValueError(f"type_vectors: give the (min, max) range of property '{name}'")
             lo, hi = ranges[name]
             vals = [float(values[name][int(z)]) for z in atomic_numbers]
+        elif name == "atomic_number":
+            lo, hi = ATOMIC_NUMBER_RANGE
+            vals = [float(z) for z in atomic_numbers]
+        elif name in elements.PROPERTIES:
+            lo, hi = property_range(name)
+            vals = []
+            for z in atomic_numbers:
+                if not 1 <= int(z) <= len(elements.SYMBOLS):
+                    raise ValueError(f"type_vectors: no element with atomic number {z}")
+                v = elements.PROPERTIES[name][int(z) - 1]
+                if v is None:
+                    raise ValueError(f"type_vectors: the table has no '{name}' for {elements.SYMBOLS[int(z) - 1]}")
+                vals.append(float(v))
+        else:
+            raise ValueError(f"type_vectors: give the values and (min, max) range of property '{name}'")
         for t, v in enumerate(vals):
             if np.isnan(v):
                 raise ValueError(f"type_vectors: property '{name}' of element {atomic_numbers[t]} is NaN")

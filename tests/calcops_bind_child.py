@@ -46,6 +46,7 @@ def load(tmp):
 
 
 def rel(a, b):
+    a, b = (x.detach().cpu() if isinstance(x, torch.Tensor) else x for x in (a, b))
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.abs(a - b).max(initial=0.0) / max(np.abs(b).max(initial=0.0), 1e-30))
@@ -159,7 +160,7 @@ def main(tmp):
     r = nep_step(ops, w, dev, torch.float64, frames, create_graph=True)
     want = on.nep_evaluate(frames, w, r["maxr"], r["maxa"])
     nb = want["nb"]
-    out["nl_exact"] = bool(all(np.array_equal(r[k].cpu().numpy(), nb[k])
+    out["nl_exact"] = bool(all(np.array_equal(r[k].detach().cpu().numpy(), nb[k])
                                for k in ("NN_r", "NN_a", "NL_r", "NL_a", "Ri_r", "Ri_a")))
     mr, ma = on.maxneigh(r["pk"], cfg.rcut_radial, cfg.rcut_angular, 4, True)
     pk = r["pk"]

@@ -146,3 +146,42 @@ def test_rejects_other_layouts(tmp_path):
     torch.save(ck, path)
     with pytest.raises(ValueError, match="not DP"):
         read_dp_checkpoint(path)
+
+
+def _typedp_checkpoint(path, props=None, T=2, atom_types=(72, 8), seed=3):
+    """A checkpoint saved from modules shaped like MatPL's TypeDP (dp_dp_typ_emb.py:66-97: one shared
+    embedding net whose first layer takes [s, v(t_j)], per-type fitting nets)."""
+    from matpl_amd.typedp import DEFAULT_PROPERTIES, TypeDPWeights, type_vectors
+    props = list(props or DEFAULT_PROPERTIES)
+    cfg = DPConfig(ntypes=T, m=100)
+    rng = np.random.default_rng(seed)
+    tv = type_vectors(list(atom_types), props)
+    tw = TypeDPWeights.random(cfg, tv, seed=seed, davg=rng.normal(0.1, 0.02, (T, 4)), dstd=rng.uniform(0.5, 1.0, (T, 4)))
+    model = _DP([tw.emb], tw.fit).double()
+    desc = {"Rmax": cfg.rcut, "Rmin": cfg.rcut_smooth, "M2": cfg.M2, "network_size": cfg.emb_sizes}
+    desc["type_embedding"] = {"physical_property": props}
+    torch.save({"json_file": {"model_type": "DP", "atom_type": list(atom_types), "max_neigh_num": cfg.m,
+                              "model": {"descriptor": desc, "fitting_net": {"network_size": cfg.fit_sizes}}},
+                "epoch": 2, "state_dict": model.state_dict(), "davg": DPWeights(cfg, [], [], tw.davg, tw.dstd).davg_full(),
+                "dstd": DPWeights(cfg, [], [], tw.davg, tw.dstd).dstd_full(), "energy_shift": [0.0] * T,
+                "atom_type_order": np.array(atom_types), "sij_max": 1.0}, path)
+    return tw
+
+
+def test_reads_typedp_checkpoint(tmp_path):
+    """A default-config TypeDP checkpoint (five physical properties, nn_feature_type.py:35) is read: the
+    shared net and the fitting nets as saved, type vectors from the element table, and the folded
+    per-pair model gives the same energy on the oracle as folding the saved weights directly."""
+    from matpl_amd.typedp import DEFAULT_PROPERTIES, fold
+    path = str(tmp_path / "typedp.ckpt")
+    tw = _typedp_checkpoint(path)
+    w, order, info = read_dp_checkpoint(path)
+    rt = info["typedp"]
+    assert order == [72, 8] and info["physical_property"] == list(DEFAULT_PROPERTIES)
+    assert rt.emb[0][0].shape == (1 + len(DEFAULT_PROPERTIES), 25)
+    for (Wa, ba, _), (Wb, bb, _) in zip(rt.emb, tw.emb):
+        assert np.array_equal(Wa, Wb) and np.array_equal(ba, bb)
+    assert np.array_equal(rt.type_vec, tw.type_vec)
+    f = st.fluorite_hfo2((1, 1, 1), seed=3)
+    a, b = oracle.dp_evaluate(f, fold(tw)), oracle.dp_evaluate(f, w)
+    assert a["E"] == b["E"] and np.array_equal(a["F"], b["F"])

@@ -24,7 +24,7 @@ def _setup(frame, T, m, seed, E=25):
     return DPWeights.random(cfg, seed=seed, davg=davg, dstd=dstd)
 
 
-def _fused(frame, w, precision, dev, rc_nl=None, mnl=None, gather=True):
+def _fused(frame, w, precision, dev, rc_nl=None, mnl=None, gather=True, store=False):
     cfg = w.cfg
     rc_nl = rc_nl or cfg.rcut
     mnl = mnl or cfg.m
@@ -32,7 +32,7 @@ def _fused(frame, w, precision, dev, rc_nl=None, mnl=None, gather=True):
                             want_dR=False, want_shift=True)
     L = torch.as_tensor(frame.lattice, device=dev)
     pos = (nl.frac_wrapped @ L).contiguous()
-    model = FusedDP(w, precision, dev)
+    model = FusedDP(w, precision, dev, store=store)
     tm = torch.as_tensor(frame.types, device=dev)
     rev = None
     if gather:
@@ -212,3 +212,78 @@ def test_fused_several_frames(dev, precision):
         else:
             assert np.sqrt(((F[a:b] - ref["F"]) ** 2).mean()) <= 1e-3, k
             assert abs(Ei[a:b].sum() - ref["E"]) <= 1e-3 * abs(ref["E"]), k
+
+
+STORE_CASES = {
+    "cu108": (lambda: st.fcc_cu(), 1, 100, 25),
+    "hfo2_324": (lambda: st.fluorite_hfo2((3, 3, 3), seed=21), 2, 120, 25),
+    "hfo2_324_E32": (lambda: st.fluorite_hfo2((3, 3, 3), seed=21), 2, 120, 32),
+    "sic_512_E16": (lambda: st.zincblende_sic((4, 4, 4), seed=4), 2, 100, 16),
+    "lgps_300_T4": (lambda: st.lgps_frames(nframes=1, nmin=300, nmax=300, seed=3)[0], 4, 150, 25),
+}
+
+
+@pytest.mark.parametrize("name", list(STORE_CASES))
+def test_store_mode_matches_oracle_and_recompute(dev, name):
+    """Store mode (the forward keeps G and dG/ds per neighbor, the backward reads them) against the
+    FP64 oracle at the recomputing pair's tolerances (FP32: 1e-6 |E|, 1e-5 eV/A per component, virial
+    1e-5 relative; BF16: force RMSE 1e-3 eV/A, 1e-3 |E|), and against the recomputing pair itself:
+    identical energies (the forward arithmetic is the same), FP32 forces within 2e-6 eV/A, BF16 forces
+    within RMSE 2e-4 eV/A (G and dG/ds are kept as fp16)."""
+    maker, T, m, E = STORE_CASES[name]
+    f = maker()
+    w = _setup(f, T, m, seed=7, E=E)
+    ref = oracle.dp_evaluate(f, w)
+    for precision in ("fp32", "bf16"):
+        print(f"case fused store {name} {precision}", flush=True)
+        Ei, F, V = _fused(f, w, precision, dev, store=True)
+        Ei0, F0, V0 = _fused(f, w, precision, dev, store=False)
+        np.testing.assert_array_equal(Ei, Ei0)
+        if precision == "fp32":
+            assert abs(Ei.sum() - ref["E"]) <= 1e-6 * abs(ref["E"])
+            assert np.abs(F - ref["F"]).max() <= 1e-5, np.abs(F - ref["F"]).max()
+            np.testing.assert_allclose(V.T, ref["V"].reshape(3, 3), atol=1e-5 * np.abs(ref["V"]).max())
+            assert np.abs(F - F0).max() <= 2e-6, np.abs(F - F0).max()
+        else:
+            rmse = np.sqrt(((F - ref["F"]) ** 2).mean())
+            d0 = np.sqrt(((F - F0) ** 2).mean())
+            print(f"{name}: store bf16 force RMSE {rmse:.2e} vs oracle, {d0:.2e} vs recompute", flush=True)
+            assert rmse <= 1e-3
+            assert abs(Ei.sum() - ref["E"]) <= 1e-3 * abs(ref["E"])
+            assert d0 <= 2e-4
+
+
+def test_store_mode_absent_types_and_frames(dev):
+    """Store mode over packed frames whose atom_type lists differ (a two-type model, a frame with
+    one type only): per-frame energies and forces against the FP64 oracle of each frame evaluated
+    with that frame's list, FP32 within 1e-6 |E| and 1e-5 eV/A."""
+    from matpl_amd.neighbor import find_neighbore_batch
+    frames = [st.fluorite_hfo2((2, 2, 2), seed=3), st.fcc_cu((3, 3, 3), seed=1)]
+    T, m = 2, 120
+    w = _setup(frames[0], T, m, seed=11)
+    types = [f.types for f in frames]
+    types[1] = np.zeros_like(types[1])
+    present = np.array([[True, True], [True, False]])
+    nl = find_neighbore_batch([f.lattice for f in frames], [f.frac() for f in frames], types, T, m, w.cfg.rcut,
+                              want_dR=False, want_shift=True, device=dev)
+    sizes = [f.natoms for f in frames]
+    fs = torch.as_tensor(np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64), device=dev)
+    af = torch.as_tensor(np.repeat(np.arange(len(frames), dtype=np.int32), sizes), device=dev)
+    lat = torch.as_tensor(np.stack([np.asarray(f.lattice).reshape(9) for f in frames]), device=dev)
+    pos = torch.cat([nl.frac_wrapped[int(fs[k]):int(fs[k + 1])] @ lat[k].reshape(3, 3)
+                     for k in range(len(frames))]).contiguous()
+    tm = torch.as_tensor(np.concatenate(types).astype(np.int32), device=dev)
+    rev, rst = reverse_map(nl, tm, m, frame_start=fs, atom_frame=af)
+    assert int(rst.item()) == 0
+    model = FusedDP(w, "fp32", dev, store=True)
+    Ei, F, V, status = model(pos, lat, tm, nl.list, nl.shift, nl.counts, m, atom_frame=af, frame_start=fs,
+                             nframes=len(frames), rev=rev, type_present=present)
+    torch.cuda.synchronize()
+    assert int(status.item()) == 0
+    Ei, F = Ei.double().cpu().numpy(), F.double().cpu().numpy()
+    for k, f in enumerate(frames):
+        a, b = int(fs[k]), int(fs[k + 1])
+        g = st.Frame(f.lattice, f.pos, types[k]) if k == 1 else f
+        ref = oracle.dp_evaluate(g, w, type_present=present[k])
+        assert abs(Ei[a:b].sum() - ref["E"]) <= 1e-6 * abs(ref["E"]), k
+        assert np.abs(F[a:b] - ref["F"]).max() <= 1e-5, (k, np.abs(F[a:b] - ref["F"]).max())

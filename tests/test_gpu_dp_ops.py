@@ -26,6 +26,19 @@ def _rand_op_inputs(B, N, S, ng, seed, dtype):
 TOL = {torch.float64: 1e-11, torch.float32: 2e-5}
 
 
+def test_neighbor_nonfinite_position_raises(dev):
+    """A NaN or infinite coordinate is an error of the search (status -4, the coordinate searched as 0
+    so that every index stays in range), for the brute-force (small box) and cell-list searches."""
+    from matpl_amd.neighbor import find_neighbore_gpu
+    for f, T, m in ((st.fcc_cu(), 1, 100), (st.fluorite_hfo2((6, 6, 6)), 2, 120)):
+        for bad in (float("nan"), float("inf")):
+            frac = f.frac().copy()
+            frac[3, 1] = bad
+            print(f"case non-finite position {f.natoms} atoms {bad}", flush=True)
+            with pytest.raises(ValueError, match="not finite"):
+                find_neighbore_gpu(f.lattice, frac, f.types, T, m, 6.0, device=dev)
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("B,N,S,ng", [(1, 7, 5, 0), (2, 33, 24, 3), (1, 0, 4, 0), (3, 17, 1, 0)])
 def test_force_virial_ops(dev, dtype, B, N, S, ng):

@@ -86,3 +86,34 @@ def test_type_weight_gradient_vs_finite_difference(dev):
             vals.append(oracle_loss(tw2))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(g[idx].item() - fd) <= 1e-6 * max(1.0, abs(fd)), (idx, g[idx].item(), fd)
+
+
+def test_default_typedp_checkpoint_vs_oracle(dev, tmp_path):
+    """A default-config TypeDP model (five physical properties from the element table,
+    nn_feature_type.py:35) saved in MatPL's checkpoint layout and read back: the reference form on the
+    HIP ops (TypeDPTorch) and the fused BF16 kernels on the folded model against the FP64 oracle."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_dp_ckpt import _typedp_checkpoint
+    from matpl_amd.dp_ckpt import read_dp_checkpoint
+    path = str(tmp_path / "typedp.ckpt")
+    _typedp_checkpoint(path)
+    w, order, info = read_dp_checkpoint(path)
+    f = st.fluorite_hfo2((2, 2, 2), seed=9)
+    print(f"case default TypeDP checkpoint N={f.natoms}", flush=True)
+    ref = oracle.dp_evaluate(f, w)
+    E1, Ei1, F1, V1 = _eval(TypeDPTorch(info["typedp"], torch.float64, dev), f, dev)
+    assert abs(E1.item() - ref["E"]) < 1e-10 * abs(ref["E"])
+    assert np.abs(F1[0].detach().cpu().numpy() - ref["F"]).max() < 1e-9
+    nl = find_neighbore_gpu(f.lattice, f.frac(), f.types, 2, w.cfg.m, w.cfg.rcut, device=dev, want_dR=False,
+                            want_shift=True)
+    L = torch.as_tensor(f.lattice, device=dev)
+    tm = torch.as_tensor(f.types, device=dev)
+    rev, rst = reverse_map(nl, tm, w.cfg.m)
+    Ei, F, V, status = FusedDP(w, "bf16", dev)((nl.frac_wrapped @ L).contiguous(), L.reshape(1, 9).contiguous(), tm,
+                                               nl.list, nl.shift, nl.counts, w.cfg.m, rev=rev)
+    torch.cuda.synchronize()
+    assert int(status.item()) == 0 and int(rst.item()) == 0
+    rmse = float(np.sqrt(((F.double().cpu().numpy() - ref["F"]) ** 2).mean()))
+    assert rmse <= 1e-3, rmse
