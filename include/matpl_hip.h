@@ -375,6 +375,21 @@ int matpl_dp_compress_grad_f32(const float *f2, const float *coef, const float *
 int matpl_dp_compress_grad_f64(const double *f2, const double *coef, const double *gG, int64_t n, int L,
                                int coe_num, double *gf2, void *stream);
 
+/*
+ * calculateVirial without a workspace and without allocating device memory:
+ * the same work, kernels and results as matpl_dp_virial_f32 (f64 = 0) /
+ * matpl_dp_virial_f64 (f64 != 0), with the reduction's partial sums in a
+ * static device array of the library (1.18 MB per GPU, held until the process
+ * ends). B <= 64 frames, else MATPL_ERR_ARGUMENT. The array is looked up on the
+ * current device, which must be the one the pointers and stream belong to.
+ * Calls that overlap on one GPU share the array and race; serialize them (one
+ * stream, or events). No counterpart in the reference: for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).
+ */
+int matpl_dp_virial_noalloc(int f64, const int32_t *nlist, const void *dE, const void *Rij, const void *Ri_d,
+                            int64_t B, int64_t N, int64_t S, int64_t nghost, void *virial, void *atom_virial,
+                            void *stream);
+
 
 /* ------------------------------------------------------------ NEP operators */
 /*

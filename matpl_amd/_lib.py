@@ -26,6 +26,7 @@ SIGNATURES = {
     "matpl_dp_virial_workspace_elems": (I32, [I64, I64, I64]),
     "matpl_dp_virial_f32": (I32, [P, P, P, P, I64, I64, I64, I64, P, P, P, I64, P]),
     "matpl_dp_virial_f64": (I32, [P, P, P, P, I64, I64, I64, I64, P, P, P, I64, P]),
+    "matpl_dp_virial_noalloc": (I32, [I32, P, P, P, P, I64, I64, I64, I64, P, P, P]),
     "matpl_dp_force_grad_f32": (I32, [P, P, P, I64, I64, I64, I64, P, P]),
     "matpl_dp_force_grad_f64": (I32, [P, P, P, I64, I64, I64, I64, P, P]),
     "matpl_dp_virial_grad_f32": (I32, [P, P, P, P, I64, I64, I64, P, P]),

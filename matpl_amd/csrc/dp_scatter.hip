@@ -13,6 +13,8 @@
 // contributions from any slot. Everything is enqueued on the caller's stream.
 #include "common.h"
 
+#include <mutex>
+
 namespace matpl {
 namespace {
 
@@ -280,6 +282,44 @@ int virial_grad_impl(const int32_t *nlist, const T *Rij, const T *Ri_d, const T 
 }  // namespace
 }  // namespace matpl
 
+// Scratch of matpl_dp_virial_noalloc: the partial sums of the virial's two-stage row reduction
+// (at most 256 chunks x 9 per frame) for up to NOALLOC_MAX_FRAMES frames, 1.18 MB. Each GPU the
+// library runs on holds its own copy until the process ends. Host code never passes the array's
+// name to a kernel: it uses the device address hipGetSymbolAddress returns for the current device.
+constexpr int64_t NOALLOC_MAX_FRAMES = 64;
+constexpr int64_t NOALLOC_SCRATCH_ELEMS = NOALLOC_MAX_FRAMES * 256 * 9;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_ELEMS];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+int scratch_address(void **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        matpl::set_error("dp_virial_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+        return matpl::ERR_HIP;
+    }
+    MATPL_CHECK_ARG(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "dp_virial_noalloc: device %d beyond %d", dev,
+                    NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            matpl::set_error("dp_virial_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+            return matpl::ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return matpl::OK;
+}
+}  // namespace
+
 using matpl::force_grad_impl;
 using matpl::force_impl;
 using matpl::virial_grad_impl;
@@ -333,4 +373,26 @@ MATPL_EXPORT int matpl_dp_virial_grad_f64(const int32_t *nlist, const double *Ri
                                           const double *gV, int64_t B, int64_t N, int64_t S, double *g,
                                           void *stream) {
     return virial_grad_impl<double>(nlist, Rij, Ri_d, gV, B, N, S, g, (hipStream_t)stream);
+}
+
+// calculateVirial without a workspace argument and without allocating: the reduction's partial sums
+// go to this GPU's `scratch` (B <= NOALLOC_MAX_FRAMES). Same kernels, launch order and results as
+// matpl_dp_virial_f32/_f64. Calls that overlap on one GPU share the scratch and race.
+MATPL_EXPORT int matpl_dp_virial_noalloc(int f64, const int32_t *nlist, const void *dE, const void *Rij,
+                                         const void *Ri_d, int64_t B, int64_t N, int64_t S, int64_t nghost,
+                                         void *virial, void *atom_virial, void *stream) {
+    MATPL_CHECK_ARG(B >= 0 && B <= NOALLOC_MAX_FRAMES, "dp_virial_noalloc: %lld frames, at most %lld",
+                    (long long)B, (long long)NOALLOC_MAX_FRAMES);
+    void *ws = nullptr;
+    int rc = scratch_address(&ws);
+    if (rc) return rc;
+    if (f64)
+        return virial_impl<double>(nlist, static_cast<const double *>(dE), static_cast<const double *>(Rij),
+                                   static_cast<const double *>(Ri_d), B, N, S, nghost, static_cast<double *>(virial),
+                                   static_cast<double *>(atom_virial), static_cast<double *>(ws),
+                                   NOALLOC_SCRATCH_ELEMS, (hipStream_t)stream);
+    return virial_impl<float>(nlist, static_cast<const float *>(dE), static_cast<const float *>(Rij),
+                              static_cast<const float *>(Ri_d), B, N, S, nghost, static_cast<float *>(virial),
+                              static_cast<float *>(atom_virial), static_cast<float *>(ws),
+                              2 * NOALLOC_SCRATCH_ELEMS, (hipStream_t)stream);
 }
