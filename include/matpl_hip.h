@@ -390,7 +390,6 @@ int matpl_dp_virial_noalloc(int f64, const int32_t *nlist, const void *dE, const
                             int64_t B, int64_t N, int64_t S, int64_t nghost, void *virial, void *atom_virial,
                             void *stream);
 
-
 /* ------------------------------------------------------------ NEP operators */
 /*
  * Neighbor search over replicated boxes (num_cell[f][3] replicas of the
@@ -488,6 +487,37 @@ int matpl_nep_force_grad(int f64, const int64_t *NL, const void *Ri_d, const voi
  */
 int matpl_nep_virial_grad(int f64, const int64_t *NL, const void *Rij, const void *Ri_d, const void *gV,
                           const int32_t *atom_frame, void *g, int64_t N, int S, void *stream);
+
+/*
+ * NEP fitting nets (one hidden layer of H tanh units, linear output; the
+ * reference's FittingNet with network_size [H, 1], src/model/nep_fitting.py:
+ * 40-163, run per atom type at src/model/nep_net.py:470-501) for all types in
+ * one launch. f64: double, else float, for every floating argument.
+ * q [N][D] features, qs [D] q_scaler; perm [N] the atoms grouped by type and
+ * tiles [ntiles][3] (type, first position in perm, count <= tile_atoms), the
+ * tiles of one type consecutive and the types in increasing order; tile_atoms
+ * must be matpl_nep_fit_tile_atoms(f64, D, H) (-1: D, H too large).
+ * W0 [T][D][H], b0 [T][H], w1 [T][H] (the output layer's weight column),
+ * b1 [T].
+ * Forward: Ei [N] and, when g is not NULL, g [N][D] = dEi/dq.
+ * Backward: given gE [N] = dL/dEi and gg [N][D] = dL/dg (either may be NULL
+ * for zero), dq [N][D] = dL/dq (may be NULL) and the parameter gradients
+ * dW0/db0/dw1/db1 (written, not accumulated) of L through Ei and g, i.e. the
+ * net's first and second derivatives (the force loss reaches the weights
+ * through g). part: workspace of ntiles * matpl_nep_fit_part_width(D, H)
+ * elements; tstart [T+1]: type t's tiles are [tstart[t], tstart[t+1]).
+ * Results do not depend on scheduling (per-tile sums added in tile order).
+ */
+int matpl_nep_fit_tile_atoms(int f64, int D, int H);
+int matpl_nep_fit_part_width(int D, int H);
+int matpl_nep_fit_forward(int f64, const void *q, const void *qs, const int32_t *perm, const int32_t *tiles,
+                          int ntiles, int tile_atoms, const void *W0, const void *b0, const void *w1, const void *b1,
+                          int D, int H, void *Ei, void *g, void *stream);
+int matpl_nep_fit_backward(int f64, const void *q, const void *qs, const int32_t *perm, const int32_t *tiles,
+                           int ntiles, int tile_atoms, const int32_t *tstart, int ntypes, const void *W0,
+                           const void *b0, const void *w1, const void *b1, int D, int H, const void *gE,
+                           const void *gg, void *dq, void *part, void *dW0, void *db0, void *dw1, void *db1,
+                           void *stream);
 
 /* ------------------------------------------------------------ MD helpers */
 /*

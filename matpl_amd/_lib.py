@@ -74,6 +74,11 @@ SIGNATURES = {
     "matpl_nep_frame_virial": (I32, [I32, P, P, P, P, P, I64, I64, I32, P, P, P]),
     "matpl_nep_force_grad": (I32, [I32, P, P, P, P, I64, I32, P]),
     "matpl_nep_virial_grad": (I32, [I32, P, P, P, P, P, P, I64, I32, P]),
+    "matpl_nep_fit_tile_atoms": (I32, [I32, I32, I32]),
+    "matpl_nep_fit_part_width": (I32, [I32, I32]),
+    "matpl_nep_fit_forward": (I32, [I32, P, P, P, P, I32, I32, P, P, P, P, I32, I32, P, P, P]),
+    "matpl_nep_fit_backward": (I32, [I32, P, P, P, P, I32, I32, P, I32, P, P, P, P, I32, I32, P, P, P, P, P, P, P, P,
+                                     P]),
     # MD helpers
     "matpl_md_kick": (I32, [P, P, P, P, F64, I64, I32, P]),
     "matpl_md_affine": (I32, [P, P, P, I64, P]),

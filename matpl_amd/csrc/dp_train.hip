@@ -429,14 +429,15 @@ dp_train_embgrad_kernel(TrainParams tp) {
                                     zb, a);
             mfma_layer<BF16, false>(mat_ptr<BF16>(reinterpret_cast<const unsigned char *>(p.mat), pr, M_W2B), lane,
                                     zbd, b);
-            // layer 1 (h1 = tanh(s w1 + b1), ḣ1 = (1 - h1^2) w1): w̄1 += s z̄1 + ż̄1, b̄1 += z̄1
+            // layer 1 (h1 = tanh(x w1 + b1) with x = Ri[0] the normalized input, ḣ1 = (1 - h1^2) w1):
+            // w̄1 += x z̄1 + ż̄1, b̄1 += z̄1
             const float *w1 = vec_ptr(p.vec, pr, V_W1, h);
 #pragma unroll
             for (int rr = 0; rr < NR; rr++) {
                 const float q = 1.0f - et.h1[rr] * et.h1[rr];
                 const float zd1 = q * b[rr];
                 const float z1 = q * (a[rr] - 2.0f * et.h1[rr] * (w1[rr] * IK) * b[rr]);
-                w1s[rr] += s * z1 + zd1;
+                w1s[rr] += Ri[0] * z1 + zd1;
                 b1s[rr] += z1;
             }
         }

@@ -24,7 +24,8 @@ from typing import List, Optional
 import numpy as np
 import torch
 
-from . import calcops, nep_ops
+from . import _lib, calcops, nep_ops
+from .calcops import _ptr, _stream
 from .tallmm import _SPLIT_ROWS, _TallMM, _tall_tn  # noqa: F401
 from .nepnet import NEPWeights
 
@@ -137,6 +138,68 @@ class _FitNet(torch.nn.Module):
         return x
 
 
+class FitPlan:
+    """Atoms grouped by type and cut into tiles for the fitting kernels (nep_fit.hip): perm [N]
+    (int32, the atoms of type 0, then type 1, ...), tiles [ntiles, 3] (type, first position in
+    perm, count), tstart [T + 1] (type t's tiles)."""
+
+    def __init__(self, type_index, f64: bool, D: int, H: int, device):
+        T = len(type_index)
+        ta = _lib.query("matpl_nep_fit_tile_atoms", int(f64), D, H)
+        if ta < 0:
+            raise ValueError(f"NEP fitting kernel: no tile fits D={D}, H={H}")
+        counts = [int(ix.numel()) for ix in type_index]
+        rows, tstart, off = [], [0], 0
+        for t, c in enumerate(counts):
+            for s0 in range(0, c, ta):
+                rows.append((t, off + s0, min(ta, c - s0)))
+            off += c
+            tstart.append(len(rows))
+        self.tile_atoms, self.ntiles, self.ntypes = ta, len(rows), T
+        self.perm = (torch.cat(list(type_index)).to(torch.int32) if T else
+                     torch.zeros(0, dtype=torch.int32, device=device)).contiguous()
+        self.tiles = torch.as_tensor(np.asarray(rows, np.int32).reshape(-1, 3), device=device)
+        self.tstart = torch.as_tensor(np.asarray(tstart, np.int32), device=device)
+
+
+class _NepFitFn(torch.autograd.Function):
+    """(Ei [N], g = dEi/dq [N, D]) of the per-type one-hidden-layer nets; the backward takes
+    dL/dEi and dL/dg and returns dL/dq and the parameter gradients (nep_fit.hip)."""
+
+    @staticmethod
+    def forward(ctx, q, qs, W0, b0, w1, b1, plan):
+        q = q.contiguous()
+        N, D = q.shape
+        H = W0.shape[2]
+        Ei = torch.empty(N, dtype=q.dtype, device=q.device)
+        g = torch.empty_like(q)
+        _lib.call("matpl_nep_fit_forward", int(q.dtype == torch.float64), _ptr(q), _ptr(qs), _ptr(plan.perm),
+                  _ptr(plan.tiles), plan.ntiles, plan.tile_atoms, _ptr(W0), _ptr(b0), _ptr(w1), _ptr(b1), D, H,
+                  _ptr(Ei), _ptr(g), _stream())
+        ctx.save_for_backward(q, qs, W0, b0, w1, b1)
+        ctx.plan = plan
+        return Ei, g
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, gE, gg):
+        q, qs, W0, b0, w1, b1 = ctx.saved_tensors
+        plan = ctx.plan
+        N, D = q.shape
+        H = W0.shape[2]
+        f64 = int(q.dtype == torch.float64)
+        gE = None if gE is None else gE.contiguous()
+        gg = None if gg is None else gg.contiguous()
+        dq = torch.empty_like(q) if ctx.needs_input_grad[0] else None
+        part = torch.empty(max(plan.ntiles, 1) * _lib.query("matpl_nep_fit_part_width", D, H), dtype=q.dtype,
+                           device=q.device)
+        dW0, db0, dw1, db1 = (torch.empty_like(x) for x in (W0, b0, w1, b1))
+        _lib.call("matpl_nep_fit_backward", f64, _ptr(q), _ptr(qs), _ptr(plan.perm), _ptr(plan.tiles), plan.ntiles,
+                  plan.tile_atoms, _ptr(plan.tstart), plan.ntypes, _ptr(W0), _ptr(b0), _ptr(w1), _ptr(b1), D, H,
+                  _ptr(gE), _ptr(gg), _ptr(dq), _ptr(part), _ptr(dW0), _ptr(db0), _ptr(dw1), _ptr(db1), _stream())
+        return dq, None, dW0, db0, dw1, db1, None
+
+
 class NEPTorch(torch.nn.Module):
     """NEP.forward on the HIP operators: (Etot [F,1], Ei [N], Force [N,3], Virial [F,9])."""
 
@@ -150,6 +213,11 @@ class NEPTorch(torch.nn.Module):
         self.register_buffer("q_scaler", torch.as_tensor(w.q_scaler, dtype=dtype, device=device))
         self.fit = torch.nn.ModuleList([_FitNet(layers, dtype, device) for layers in w.fit])
         self.dtype = dtype
+        # one hidden layer with biases in every net (the NEP form, network_size [H, 1]): the fitting
+        # kernels; otherwise the per-type torch layers
+        self.fit_kernel = all(len(n.W) == 2 and n.W[1].shape[1] == 1 and all(r is None for r in n.rdt)
+                              for n in self.fit) and len({tuple(n.W[0].shape) for n in self.fit}) == 1
+        self._plans = {}
 
     def features(self, Ri_r, NL_r, Ri_a, NL_a, atom_type_map):
         cfg = self.cfg
@@ -163,8 +231,35 @@ class NEPTorch(torch.nn.Module):
             q = torch.cat([q, qa], -1)
         return q
 
+    def fit_plan(self, atom_type_map, type_index=None) -> FitPlan:
+        """The fitting kernels' tiles; cached for a given ``type_index`` list (graph replays)."""
+        D, H = self.fit[0].W[0].shape
+        f64 = self.dtype == torch.float64
+        if type_index is not None:
+            key = tuple(ix.data_ptr() for ix in type_index) + tuple(ix.numel() for ix in type_index)
+            plan = self._plans.get(key)
+            if plan is None:
+                plan = FitPlan(type_index, f64, D, H, atom_type_map.device)
+                self._plans = {key: plan}
+            return plan
+        tm = atom_type_map.reshape(-1)
+        perm = torch.argsort(tm, stable=True)
+        counts = torch.bincount(tm, minlength=self.cfg.ntypes).cpu().tolist()
+        return FitPlan(list(torch.split(perm, counts)), f64, D, H, tm.device)
+
+    def energies_and_grad(self, feats, atom_type_map, type_index=None):
+        """(Ei [N], dEi/dfeats [N, D]) from the fitting kernels (fit_kernel nets only)."""
+        plan = self.fit_plan(atom_type_map, type_index)
+        W0 = torch.stack([n.W[0] for n in self.fit])
+        b0 = torch.stack([n.b[0] for n in self.fit])
+        w1 = torch.stack([n.W[1][:, 0] for n in self.fit])
+        b1 = torch.stack([n.b[1].reshape(()) for n in self.fit])
+        return _NepFitFn.apply(feats, self.q_scaler, W0, b0, w1, b1, plan)
+
     def energies(self, feats, atom_type_map, type_index=None):
         """type_index: the atoms of each type (precomputed, e.g. for graph capture); else found here."""
+        if self.fit_kernel:
+            return self.energies_and_grad(feats, atom_type_map, type_index)[0]
         x = feats * self.q_scaler
         Ei = torch.zeros(feats.shape[0], dtype=feats.dtype, device=feats.device)
         for t, net in enumerate(self.fit):
@@ -179,7 +274,11 @@ class NEPTorch(torch.nn.Module):
         Ri_a = Ri_a.to(self.dtype).detach().requires_grad_()
         l3 = self.cfg.l_max[0]
         feats = self.features(Ri_r, NL_r, Ri_a, NL_a, atom_type_map)
-        Ei = self.energies(feats, atom_type_map, type_index)
+        gq = None
+        if self.fit_kernel:
+            Ei, gq = self.energies_and_grad(feats, atom_type_map, type_index)
+        else:
+            Ei = self.energies(feats, atom_type_map, type_index)
         nf = num_atom.numel()
         frame = nep_ops._frame_index(num_atom, Ei.device)[1]
         Etot = torch.zeros(nf, dtype=Ei.dtype, device=Ei.device).index_add(0, frame, Ei).unsqueeze(-1)
@@ -188,7 +287,11 @@ class NEPTorch(torch.nn.Module):
         if create_graph is None:
             create_graph = torch.is_grad_enabled() and any(p.requires_grad for p in self.parameters())
         ins = [Ri_r, Ri_a] if l3 > 0 else [Ri_r]
-        g = torch.autograd.grad([Etot.sum()], ins, create_graph=create_graph, allow_unused=True)
+        if gq is not None:
+            # dEtot/dRi = (dEi/dq) dq/dRi: the fitting kernel's g through the feature ops only
+            g = torch.autograd.grad([feats], ins, grad_outputs=[gq], create_graph=create_graph, allow_unused=True)
+        else:
+            g = torch.autograd.grad([Etot.sum()], ins, create_graph=create_graph, allow_unused=True)
         ops = torch.ops.CalcOps_cuda
         Force = Virial = None
         for Ri, NL, dE in zip(ins, (NL_r, NL_a), g):
