@@ -254,6 +254,53 @@ def nep_train_leg(steps=3):
     return out
 
 
+def _dp_train_workload(dev):
+    """256 LGPS frames, 4 types, m = 150; weights with davg/dstd from the batch's own statistics."""
+    from matpl_amd import structures as st
+    from matpl_amd.dp_train import pack_batch
+    from matpl_amd.dpnet import DPConfig, DPWeights
+    from matpl_amd.neighbor import calc_stat
+    frames = st.lgps_frames(nframes=256, nmin=64, nmax=512, seed=1002)
+    cfg = DPConfig(ntypes=4, m=150)
+    batch = pack_batch(frames, cfg, dev, seed=7, keep_full=True)
+    davg, dstd = calc_stat(batch.list_full[0], batch.image_dr_full[0, ..., 1:], batch.type_map, 4, cfg.rcut_smooth,
+                           cfg.rcut)
+    w = DPWeights.random(cfg, seed=1002, davg=davg.cpu().numpy(), dstd=dstd.cpu().numpy())
+    return frames, cfg, w, batch
+
+
+def dp_train_fused_leg(steps=5, precisions=("bf16", "fp32")):
+    """The DP training workload of dp_train_leg on the hand-written kernels only (DPTrainFused: fused
+    forward and force kernels, force-loss double backward of the embedding nets; fitting nets in
+    torch), gradient step + Adam, timed per precision; no FP64 path (for profiling the step alone)."""
+    import torch
+    from matpl_amd.dp_train_fused import DPTrainFused, pack_fused_batch
+    dev = torch.device("cuda:0")
+    frames, cfg, w, batch = _dp_train_workload(dev)
+    fb = pack_fused_batch(frames, cfg, dev, seed=7)
+    out = {"natoms": fb.natoms}
+    for prec in precisions:
+        fm = DPTrainFused(w, prec, dev)
+        fparams = list(fm.parameters())
+        fopt = torch.optim.Adam(fparams, lr=1e-4)
+
+        def fstep():
+            loss, g = fm.loss_and_grads(fb)
+            for q, gq in zip(fparams, g):
+                q.grad = gq
+            fopt.step()
+        for _ in range(2):
+            fstep()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fstep()
+        torch.cuda.synchronize()
+        out[prec] = {"ms_per_step": round((time.perf_counter() - t0) * 1e3 / steps, 2)}
+        log(f"DP train fused {prec}: {out[prec]['ms_per_step']} ms/step")
+    return out
+
+
 def dp_train_leg(steps=5):
     """DP training step on 256 Li10GeP2S12-stoichiometry frames (N_f uniform in [64, 512], 2.0 g/cm^3,
     73.7k atoms in all), rc 6 A, m = 150 per type (S = 600 slots), nets as the MD workload: one step =
@@ -262,18 +309,10 @@ def dp_train_leg(steps=5):
     frames packed, compact buckets, FP32 descriptors, BF16 or FP32 products) against the plain FP64
     path (DPTorch over all 600 slots, frames in chunks of 32 with gradients accumulated)."""
     import torch
-    from matpl_amd import structures as st
     from matpl_amd.dp import DPTorch
-    from matpl_amd.dp_train import DPTrain, pack_batch, plain_grads, train_grads
-    from matpl_amd.dpnet import DPConfig, DPWeights
-    from matpl_amd.neighbor import calc_stat
+    from matpl_amd.dp_train import DPTrain, plain_grads, train_grads
     dev = torch.device("cuda:0")
-    frames = st.lgps_frames(nframes=256, nmin=64, nmax=512, seed=1002)
-    cfg = DPConfig(ntypes=4, m=150)
-    batch = pack_batch(frames, cfg, dev, seed=7, keep_full=True)
-    davg, dstd = calc_stat(batch.list_full[0], batch.image_dr_full[0, ..., 1:], batch.type_map, 4, cfg.rcut_smooth,
-                           cfg.rcut)
-    w = DPWeights.random(cfg, seed=1002, davg=davg.cpu().numpy(), dstd=dstd.cpu().numpy())
+    frames, cfg, w, batch = _dp_train_workload(dev)
     N = batch.natoms
     out = {"workload": f"DP training step, 256 Li-Ge-P-S frames ({N} atoms), rc 6 A, m=150 per type (S=600), "
                        "E+F loss backward to all net parameters + Adam", "natoms": N, "mc": batch.mc}
@@ -408,6 +447,17 @@ def compressed_md_leg(frame, w, vel, dev, ct, precision, steps, warmup, tune=Tru
             "order": ct.order, "dx": ct.dx, "table_rows": ct.n_fine + ct.n_coarse, "launch": launch}
 
 
+def copy_calibration(mib):
+    """A known-traffic kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE: copy a buffer of `mib` MiB 3 times."""
+    import torch
+    a = torch.empty(mib << 18, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 3 x {mib} MiB read + written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -431,6 +481,8 @@ def main():
     ap.add_argument("--store", action="store_true",
                     help="MD with the fused store mode: the forward keeps G and dG/ds per neighbor and the backward "
                          "reads them instead of evaluating the embedding net again")
+    ap.add_argument("--only", default=None, choices=["nep", "nep_train", "dp_train", "dp_train_fused"],
+                    help="run one leg alone and print its JSON (for profiling that leg); no MD line")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MiB",
                     help="after the run, copy a buffer of this size 3 times (a known-traffic kernel for "
                          "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -441,6 +493,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.only:
+        leg = {"nep": nep_leg, "nep_train": nep_train_leg, "dp_train": dp_train_leg,
+               "dp_train_fused": dp_train_fused_leg}[args.only]
+        given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
+        res = leg(steps=args.steps) if given else leg()      # each leg has its own default step count
+        if args.copy_calib:
+            copy_calibration(args.copy_calib)
+        print(json.dumps(res), flush=True)
+        return
     from matpl_amd import calcops
     from matpl_amd.md import DPMD, DPMDFP64
     calcops.register()
@@ -580,13 +641,7 @@ def main():
         torch.cuda.empty_cache()
 
     if args.copy_calib:
-        a = torch.empty(args.copy_calib << 18, dtype=torch.float32, device=dev).fill_(1.0)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log(f"copy calibration: 3 x {args.copy_calib} MiB read + written")
-        del a, b
+        copy_calibration(args.copy_calib)
 
     nep = None if args.no_nep else nep_leg()
     nep_train = None if args.no_nep else nep_train_leg()

@@ -331,6 +331,11 @@ int matpl_dp_train_grad_size(int E);
  * the knob the launch autotuner (matpl_amd/autotune.py) turns.
  */
 int matpl_dp_fused_set_launch(int kernel, int max_blocks, int lds_weights, int xcd);
+/* Wavefronts per block (2, 4 = default, or 8) of the net-mode fused forward
+ * (kernel 0) / backward (kernel 1) for later calls; one atom per wavefront, so
+ * this sets the atoms that share a block's staged weights. The autotuner's
+ * second knob; no reference counterpart. */
+int matpl_dp_fused_set_waves(int kernel, int waves);
 
 /* force[i] = fcenter[i] + sum of fslot over the used entries of atom i's list
  * (counts [N][T]), in a fixed order (deterministic, no atomics). No reference

@@ -56,6 +56,7 @@ SIGNATURES = {
     "matpl_dp_fused_forward_store": (I32, [P] * 12 + [I64, I32, I32, I32, I32, I32, F64, F64, F64, P, P, P, P]),
     "matpl_dp_fused_backward_stored": (I32, [P] * 12 + [I64, I32, I32, I32, I32, I32, F64, F64, F64] + [P] * 9),
     "matpl_dp_fused_set_launch": (I32, [I32, I32, I32, I32]),
+    "matpl_dp_fused_set_waves": (I32, [I32, I32]),
     "matpl_dp_fused_gather": (I32, [P, P, P, P, I32, I32, I64, P]),
     "matpl_dp_lmp_neighdata": (I32, [P, P, P, I32, P, P, P, P, I64, I64, I64, I32, I32, F64, P, P, P, P, P, P, P]),
     "matpl_dp_neighbor_reverse": (I32, [P, P, P, P, P, P, I32, I32, I64, P, P, P]),

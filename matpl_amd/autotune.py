@@ -1,8 +1,10 @@
 """Launch-shape autotuner for the fused DP kernels.
 
 The fused forward and backward kernels (csrc/dp_fused.hip) run a persistent
-grid, one atom per wavefront, four wavefronts per block. Three launch choices
-change their speed but not their results:
+grid, one atom per wavefront. Four launch choices change their speed but not
+their results:
+  * the wavefronts per block (2, 4 or 8 atoms sharing one block's staged
+    weights and LDS; each is a separate instantiation of the kernels);
   * the grid's block cap (how many blocks stride over the atoms);
   * whether the weights of every type pair are staged in LDS per block or read
     through the caches from global memory (LDS costs occupancy as T grows);
@@ -28,11 +30,13 @@ import torch
 from . import _lib
 
 BLOCK_CAPS = (512, 768, 1024, 1536, 2048, 3072, 4096, 8192, 16384)
-DEFAULT = {"max_blocks": 4096, "lds_weights": -1, "xcd": 0}
+WAVES = (2, 4, 8)
+DEFAULT = {"max_blocks": 4096, "lds_weights": -1, "xcd": 0, "waves": 4}
 
 
-def set_launch(kernel: int, max_blocks: int, lds_weights: int, xcd: int) -> None:
+def set_launch(kernel: int, max_blocks: int, lds_weights: int, xcd: int, waves: int = 4) -> None:
     _lib.call("matpl_dp_fused_set_launch", kernel, max_blocks, lds_weights, xcd)
+    _lib.call("matpl_dp_fused_set_waves", kernel, waves)
 
 
 def reset() -> None:
@@ -55,12 +59,15 @@ def device_id() -> str:
 
 
 def parse_launch(spec: str) -> Dict:
-    """'fb,fl,fx,bb,bl,bx' (blocks, LDS weights, XCD order of the forward, then the backward kernel)."""
+    """'fb,fl,fx,bb,bl,bx' (blocks, LDS weights, XCD order of the forward, then the backward kernel;
+    4 wavefronts per block) or 'fb,fl,fx,fw,bb,bl,bx,bw' (with the wavefronts per block)."""
     v = [int(x) for x in spec.split(",")]
-    if len(v) != 6:
-        raise ValueError("launch spec needs 6 integers: fwd blocks,lds,xcd,bwd blocks,lds,xcd")
-    f = {"max_blocks": v[0], "lds_weights": v[1], "xcd": v[2]}
-    b = {"max_blocks": v[3], "lds_weights": v[4], "xcd": v[5]}
+    if len(v) == 6:
+        v = v[:3] + [4] + v[3:] + [4]
+    if len(v) != 8:
+        raise ValueError("launch spec needs 6 or 8 integers: fwd blocks,lds,xcd[,waves],bwd blocks,lds,xcd[,waves]")
+    f = {"max_blocks": v[0], "lds_weights": v[1], "xcd": v[2], "waves": v[3]}
+    b = {"max_blocks": v[4], "lds_weights": v[5], "xcd": v[6], "waves": v[7]}
     set_launch(0, **f)
     set_launch(1, **b)
     return {"forward": f, "backward": b, "fixed": True}
@@ -138,19 +145,20 @@ def tune_fused(run: Callable[[], None], key: str, reps: int = 10, use_cache: boo
     best = {0: dict(DEFAULT), 1: dict(DEFAULT)}
     best_ms = ms_default
     for k in (0, 1):
-        for lw in (-1, 0):
-            for xcd in (0, 1):
-                for cap in BLOCK_CAPS:
-                    cand = {"max_blocks": cap, "lds_weights": lw, "xcd": xcd}
-                    if cand == best[k]:
-                        continue
-                    set_launch(k, **cand)
-                    try:
-                        ms = _time(run, reps)
-                    except _lib.MatplError:
-                        continue
-                    if ms < best_ms:
-                        best_ms, best[k] = ms, cand
+        for waves in WAVES:
+            for lw in (-1, 0):
+                for xcd in (0, 1):
+                    for cap in BLOCK_CAPS:
+                        cand = {"max_blocks": cap, "lds_weights": lw, "xcd": xcd, "waves": waves}
+                        if cand == best[k]:
+                            continue
+                        set_launch(k, **cand)
+                        try:
+                            ms = _time(run, reps)
+                        except _lib.MatplError:
+                            continue      # e.g. 8 waves' neighbor tiles exceed the LDS budget
+                        if ms < best_ms:
+                            best_ms, best[k] = ms, cand
         set_launch(k, **best[k])
     # a fresh measurement of the chosen pair (the search's own minimum is biased low)
     best_ms = _time(run, reps)

@@ -57,23 +57,24 @@ struct LaunchCfg {
     int max_blocks;
     int lds_weights;
     int xcd;
+    int waves;          // wavefronts per block: 2, 4 or 8 (net mode; table mode always 4)
 };
-LaunchCfg g_launch[2] = {{4096, -1, 0}, {4096, -1, 0}};
+LaunchCfg g_launch[2] = {{4096, -1, 0, FUSED_WAVES}, {4096, -1, 0, FUSED_WAVES}};
 
-template <int E, bool BF16, bool LW, int CK, bool GST>
-__global__ void __launch_bounds__(FUSED_WAVES * WAVE)
+template <int E, bool BF16, bool LW, int CK, bool GST, int W>
+__global__ void __launch_bounds__(W * WAVE)
 dp_fused_fwd_kernel(FusedParams p) {
     constexpr int NR = NReg<E>::value;
     extern __shared__ float4 smem[];
     // wid is the same on every lane: readfirstlane lets the compiler keep the atom index and the
     // per-atom bucket counts in scalar registers
     const int lane = threadIdx.x & (WAVE - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-    const BlockLds L = block_lds<BF16, LW, false>(p, smem, wid);
+    const BlockLds L = block_lds<BF16, LW, false, W>(p, smem, wid);
     float4 *nb = L.nb;
     int *nvs = L.nvs;
     const int h = lane >> 5, n = lane & 31;
 
-    for (int64_t i = logical_block(p.xcd) * FUSED_WAVES + wid; i < p.natoms; i += (int64_t)gridDim.x * FUSED_WAVES) {
+    for (int64_t i = logical_block(p.xcd) * W + wid; i < p.natoms; i += (int64_t)gridDim.x * W) {
         const int ti = p.type_map[i];
         const float *tinfo = p.tinfo + ti * NTI;
         const AtomGeo g = atom_geo(p, i);
@@ -219,15 +220,15 @@ dp_fused_fwd_kernel(FusedParams p) {
     }
 }
 
-template <int E, bool BF16, bool LW, int CK, bool GST>
-__global__ void __launch_bounds__(FUSED_WAVES * WAVE)
+template <int E, bool BF16, bool LW, int CK, bool GST, int W>
+__global__ void __launch_bounds__(W * WAVE)
 dp_fused_bwd_kernel(FusedParams p) {
     constexpr int NR = NReg<E>::value;
     extern __shared__ float4 smem[];
     // wid is the same on every lane: readfirstlane lets the compiler keep the atom index and the
     // per-atom bucket counts in scalar registers
     const int lane = threadIdx.x & (WAVE - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-    const BlockLds L = block_lds<BF16, LW, true>(p, smem, wid);
+    const BlockLds L = block_lds<BF16, LW, true, W>(p, smem, wid);
     float4 *nb = L.nb;
     int *nvs = L.nvs;
     float4 *dAs = L.dAs;        // [32 features] x (dA_0..dA_3) of the current atom, scale folded in
@@ -244,7 +245,7 @@ dp_fused_bwd_kernel(FusedParams p) {
 #pragma unroll
     for (int v = 0; v < 9; v++) { vf[v] = 0.0f; vd[v] = 0.0; }
 
-    for (int64_t i = logical_block(p.xcd) * FUSED_WAVES + wid; i < p.natoms; i += (int64_t)gridDim.x * FUSED_WAVES) {
+    for (int64_t i = logical_block(p.xcd) * W + wid; i < p.natoms; i += (int64_t)gridDim.x * W) {
         const int ti = p.type_map[i];
         const float *tinfo = p.tinfo + ti * NTI;
         const AtomGeo g = atom_geo(p, i);
@@ -432,7 +433,7 @@ dp_fused_bwd_kernel(FusedParams p) {
     }
     __builtin_amdgcn_wave_barrier();
     // virial: wave sums -> one set of atomics per block and frame
-    __shared__ int vfr[FUSED_WAVES];
+    __shared__ int vfr[W];
     if (lane == 0) vfr[wid] = vframe;
     __syncthreads();
     if (threadIdx.x < 9) {
@@ -440,7 +441,7 @@ dp_fused_bwd_kernel(FusedParams p) {
         const double *vall = vacc - wid * 9;     // wave 0's row
         double acc = 0.0;
         int fr = -1;
-        for (int w = 0; w < FUSED_WAVES; w++) {
+        for (int w = 0; w < W; w++) {
             if (vfr[w] < 0) continue;
             if (vfr[w] != fr) {
                 if (fr >= 0 && acc != 0.0) atomic_add(p.virial + 9 * fr + v, acc);
@@ -494,13 +495,14 @@ gather_kernel(const int32_t *__restrict__ counts, const float4 *__restrict__ fsl
     }
 }
 
-template <int E, bool BF16, bool LW, int CK = 0, bool GST = false>
+template <int E, bool BF16, bool LW, int CK = 0, bool GST = false, int W = FUSED_WAVES>
 int launch_lw(FusedParams &p, bool fwd, hipStream_t st) {
-    const size_t lds = fwd ? fused_lds_bytes<BF16, LW, false>(p.T, p.cap) : fused_lds_bytes<BF16, LW, true>(p.T, p.cap);
-    MATPL_CHECK_ARG(lds <= 64 * 1024, "dp_fused: T*mnl too large for LDS (%zu bytes)", lds);
+    const size_t lds =
+        fwd ? fused_lds_bytes<BF16, LW, false>(p.T, p.cap, W) : fused_lds_bytes<BF16, LW, true>(p.T, p.cap, W);
+    MATPL_CHECK_ARG(lds <= 64 * 1024, "dp_fused: T*mnl too large for LDS (%zu bytes, %d waves per block)", lds, W);
     // persistent-style grid: a few blocks per CU, each strides over atoms
     const LaunchCfg &cfg = g_launch[fwd ? 0 : 1];
-    int64_t blocks = (p.natoms + FUSED_WAVES - 1) / FUSED_WAVES;
+    int64_t blocks = (p.natoms + W - 1) / W;
     if (blocks > cfg.max_blocks) blocks = cfg.max_blocks;
     p.xcd = 0;
     if (cfg.xcd && blocks >= NXCD) {
@@ -509,13 +511,26 @@ int launch_lw(FusedParams &p, bool fwd, hipStream_t st) {
     }
     if (blocks < 1) blocks = 1;
     if (fwd) {
-        hipLaunchKernelGGL((dp_fused_fwd_kernel<E, BF16, LW, CK, GST>), dim3((unsigned)blocks),
-                           dim3(FUSED_WAVES * WAVE), lds, st, p);
+        hipLaunchKernelGGL((dp_fused_fwd_kernel<E, BF16, LW, CK, GST, W>), dim3((unsigned)blocks), dim3(W * WAVE), lds,
+                           st, p);
         return check_launch("dp_fused_fwd_kernel");
     }
-    hipLaunchKernelGGL((dp_fused_bwd_kernel<E, BF16, LW, CK, GST>), dim3((unsigned)blocks), dim3(FUSED_WAVES * WAVE),
-                       lds, st, p);
+    hipLaunchKernelGGL((dp_fused_bwd_kernel<E, BF16, LW, CK, GST, W>), dim3((unsigned)blocks), dim3(W * WAVE), lds, st,
+                       p);
     return check_launch("dp_fused_bwd_kernel");
+}
+
+// Net mode: weights in LDS or global memory, 2, 4 or 8 wavefronts per block.
+template <int E, bool BF16, bool GST>
+int launch_net(FusedParams &p, bool fwd, hipStream_t st, bool lw, int w) {
+    if (lw) {
+        if (w == 2) return launch_lw<E, BF16, true, 0, GST, 2>(p, fwd, st);
+        if (w == 8) return launch_lw<E, BF16, true, 0, GST, 8>(p, fwd, st);
+        return launch_lw<E, BF16, true, 0, GST, 4>(p, fwd, st);
+    }
+    if (w == 2) return launch_lw<E, BF16, false, 0, GST, 2>(p, fwd, st);
+    if (w == 8) return launch_lw<E, BF16, false, 0, GST, 8>(p, fwd, st);
+    return launch_lw<E, BF16, false, 0, GST, 4>(p, fwd, st);
 }
 
 // Weights go to LDS when they and the neighbor tiles fit in 64 KB per block
@@ -526,15 +541,13 @@ int launch(FusedParams &p, bool fwd, hipStream_t st, int ck) {
     if (ck == 4) return launch_lw<E, BF16, false, 4>(p, fwd, st);
     if (ck == 6) return launch_lw<E, BF16, false, 6>(p, fwd, st);
     const int mode = g_launch[fwd ? 0 : 1].lds_weights;
-    const bool fits = (fwd ? fused_lds_bytes<BF16, true, false>(p.T, p.cap)
-                           : fused_lds_bytes<BF16, true, true>(p.T, p.cap)) <= 64 * 1024;
+    const int w = g_launch[fwd ? 0 : 1].waves;
+    const bool fits = (fwd ? fused_lds_bytes<BF16, true, false>(p.T, p.cap, w)
+                           : fused_lds_bytes<BF16, true, true>(p.T, p.cap, w)) <= 64 * 1024;
     MATPL_CHECK_ARG(mode != 1 || fits, "dp_fused: weights do not fit in LDS for this T and list width");
-    if (p.gbuf) {
-        if (mode == 1 || (mode < 0 && fits)) return launch_lw<E, BF16, true, 0, true>(p, fwd, st);
-        return launch_lw<E, BF16, false, 0, true>(p, fwd, st);
-    }
-    if (mode == 1 || (mode < 0 && fits)) return launch_lw<E, BF16, true>(p, fwd, st);
-    return launch_lw<E, BF16, false>(p, fwd, st);
+    const bool lw = mode == 1 || (mode < 0 && fits);
+    if (p.gbuf) return launch_net<E, BF16, true>(p, fwd, st, lw, w);
+    return launch_net<E, BF16, false>(p, fwd, st, lw, w);
 }
 
 int dispatch(FusedParams &p, int E, int bf16, bool fwd, hipStream_t st, int ck = 0) {
@@ -748,7 +761,16 @@ MATPL_EXPORT int matpl_dp_fused_set_launch(int kernel, int max_blocks, int lds_w
     MATPL_CHECK_ARG(max_blocks >= 1 && max_blocks <= (1 << 20), "dp_fused_set_launch: max_blocks out of range");
     MATPL_CHECK_ARG(lds_weights >= -1 && lds_weights <= 1 && (xcd == 0 || xcd == 1),
                     "dp_fused_set_launch: lds_weights in {-1, 0, 1}, xcd in {0, 1}");
-    g_launch[kernel] = LaunchCfg{max_blocks, lds_weights, xcd};
+    g_launch[kernel] = LaunchCfg{max_blocks, lds_weights, xcd, g_launch[kernel].waves};
+    return OK;
+}
+
+// Wavefronts per block (2, 4 or 8) of the net-mode fused forward (kernel 0) or backward (1) for
+// later calls in this process; results do not depend on it beyond the virial's FP64 summation order.
+MATPL_EXPORT int matpl_dp_fused_set_waves(int kernel, int waves) {
+    MATPL_CHECK_ARG(kernel == 0 || kernel == 1, "dp_fused_set_waves: kernel is 0 (forward) or 1 (backward)");
+    MATPL_CHECK_ARG(waves == 2 || waves == 4 || waves == 8, "dp_fused_set_waves: waves in {2, 4, 8}");
+    g_launch[kernel].waves = waves;
     return OK;
 }
 

@@ -10,7 +10,7 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int TILE = 32;
-constexpr int FUSED_WAVES = 4;     // wavefronts per block
+constexpr int FUSED_WAVES = 4;     // wavefronts per block (default; the net-mode kernels also build 2 and 8)
 constexpr int MAXT = 8;
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
@@ -607,7 +607,7 @@ struct BlockLds {
     __bf16 *xt;
 };
 
-template <bool BF16, bool LW, bool BWD>
+template <bool BF16, bool LW, bool BWD, int W = FUSED_WAVES>
 __device__ __forceinline__ BlockLds block_lds(const FusedParams &p, float4 *smem, int wid) {
     unsigned char *base = reinterpret_cast<unsigned char *>(smem);
     BlockLds b;
@@ -625,11 +625,11 @@ __device__ __forceinline__ BlockLds block_lds(const FusedParams &p, float4 *smem
     }
     float4 *nball = reinterpret_cast<float4 *>(base + off);
     b.nb = nball + (size_t)wid * p.T * p.cap;
-    int *nvall = reinterpret_cast<int *>(nball + (size_t)FUSED_WAVES * p.T * p.cap);
+    int *nvall = reinterpret_cast<int *>(nball + (size_t)W * p.T * p.cap);
     b.nvs = nvall + wid * MAXT;
-    float4 *dAall = reinterpret_cast<float4 *>(nvall + FUSED_WAVES * MAXT);
+    float4 *dAall = reinterpret_cast<float4 *>(nvall + W * MAXT);
     b.dAs = dAall + wid * 32;
-    b.vacc = reinterpret_cast<double *>(dAall + FUSED_WAVES * 32) + wid * 9;
+    b.vacc = reinterpret_cast<double *>(dAall + W * 32) + wid * 9;
     // forward: the transposed tile sits where the backward keeps dA (16-byte aligned)
     b.xt = reinterpret_cast<__bf16 *>(dAall) + (size_t)wid * XT_ELEMS;
     (void)BWD;
@@ -637,11 +637,10 @@ __device__ __forceinline__ BlockLds block_lds(const FusedParams &p, float4 *smem
 }
 
 template <bool BF16, bool LW, bool BWD>
-inline size_t fused_lds_bytes(int T, int cap) {
-    size_t b = (LW ? weight_lds_bytes<BF16>(T) : 0) + sizeof(float4) * FUSED_WAVES * (size_t)T * cap +
-               sizeof(int) * FUSED_WAVES * MAXT;
-    if (BWD) b += sizeof(float4) * FUSED_WAVES * 32 + sizeof(double) * FUSED_WAVES * 9;
-    if (!BWD && BF16) b += sizeof(__bf16) * FUSED_WAVES * XT_ELEMS;
+inline size_t fused_lds_bytes(int T, int cap, int W = FUSED_WAVES) {
+    size_t b = (LW ? weight_lds_bytes<BF16>(T) : 0) + sizeof(float4) * W * (size_t)T * cap + sizeof(int) * W * MAXT;
+    if (BWD) b += sizeof(float4) * W * 32 + sizeof(double) * W * 9;
+    if (!BWD && BF16) b += sizeof(__bf16) * W * XT_ELEMS;
     return b;
 }
 

@@ -119,14 +119,14 @@ class _EmbPack:
         self.im = torch.as_tensor(np.where(ok, im, 0), device=device)
         self.ok = torch.as_tensor(ok, device=device)
 
-    def vec(self, v: torch.Tensor) -> torch.Tensor:
-        return torch.where(self.mvec, v[self.fvec], torch.zeros((), dtype=v.dtype, device=v.device))
+    def vec(self, v: torch.Tensor) -> torch.Tensor:      # [P, E] -> [P, 2, 16]
+        return torch.where(self.mvec, v[:, self.fvec], torch.zeros((), dtype=v.dtype, device=v.device))
 
-    def fwd(self, W):   # [lane][r] = W[f(r,h)][lane % 32]
-        return torch.where(self.ok, W[self.fm, self.im], torch.zeros((), dtype=W.dtype, device=W.device))
+    def fwd(self, W):   # [P, E, E] -> [P, 64, 16]: [lane][r] = W[f(r,h)][lane % 32]
+        return torch.where(self.ok, W[:, self.fm, self.im], torch.zeros((), dtype=W.dtype, device=W.device))
 
     def bwd(self, W):   # [lane][r] = W[lane % 32][f(r,h)]
-        return torch.where(self.ok, W[self.im, self.fm], torch.zeros((), dtype=W.dtype, device=W.device))
+        return torch.where(self.ok, W[:, self.im, self.fm], torch.zeros((), dtype=W.dtype, device=W.device))
 
 
 class DPTrainFused(torch.nn.Module):
@@ -167,29 +167,46 @@ class DPTrainFused(torch.nn.Module):
         self.chunk = 0          # atoms per wavefront of the gradient kernel (0: chosen from the batch)
 
     # ---------------------------------------------------------------- weights
+    def _stacked(self):
+        """The embedding parameters of all type pairs stacked: W1, b1, b2, b3 [P, E], W2, W3 [P, E, E]
+        (returned as W1, b1, W2, b2, W3, b3)."""
+        W1 = torch.stack([n.W[0].reshape(-1) for n in self.emb])
+        b1 = torch.stack([n.b[0] for n in self.emb])
+        W2 = torch.stack([n.W[1] for n in self.emb])
+        b2 = torch.stack([n.b[1] for n in self.emb])
+        W3 = torch.stack([n.W[2] for n in self.emb])
+        b3 = torch.stack([n.b[2] for n in self.emb])
+        return W1, b1, W2, b2, W3, b3
+
+    def _gpad(self, W1, b1, W2, b2, W3, b3):
+        """G(s_pad) of every pair [P, E] (the embedding layer rules: plain first layer, residual
+        E -> E layers), batched over pairs."""
+        T = self.cfg.ntypes
+        x0 = self.ri_pad[torch.arange(T * T, device=W1.device) // T, 0]
+        h = torch.tanh(x0[:, None] * W1 + b1)
+        h = h + torch.tanh(torch.bmm(h[:, None, :], W2)[:, 0] + b2)
+        return h + torch.tanh(torch.bmm(h[:, None, :], W3)[:, 0] + b3)
+
     def _pack(self):
         """(vec, mat) from the current parameters, and G at each pair's empty-slot input."""
         cfg, T, pk = self.cfg, self.cfg.ntypes, self._pk
         K = TANH_K if self.bf16 else 1.0
-        vec = torch.zeros((T * T, NVEC, 2, 16), dtype=torch.float32, device=self.tinfo.device)
-        mat = torch.zeros((T * T, NMAT, 64, 16), dtype=torch.float32, device=self.tinfo.device)
+        P = T * T
+        vec = torch.zeros((P, NVEC, 2, 16), dtype=torch.float32, device=self.tinfo.device)
+        mat = torch.zeros((P, NMAT, 64, 16), dtype=torch.float32, device=self.tinfo.device)
         with torch.no_grad():
-            for p in range(T * T):
-                net = self.emb[p]
-                W1, W2, W3 = net.W[0], net.W[1], net.W[2]
-                b1, b2, b3 = net.b[0], net.b[1], net.b[2]
-                gpad = net(self.ri_pad[p // T, 0].view(1, 1)).reshape(-1)
-                vec[p, V_W1] = pk.vec(W1.reshape(-1)) * K
-                vec[p, V_B1] = pk.vec(b1) * K
-                vec[p, V_B2] = pk.vec(b2) * K
-                vec[p, V_B3] = pk.vec(b3) * K
-                vec[p, V_R2] = pk.vec(torch.ones_like(b2))
-                vec[p, V_R3] = pk.vec(torch.ones_like(b3))
-                vec[p, V_GPAD] = pk.vec(gpad)
-                mat[p, M_W2F] = pk.fwd(W2) * K
-                mat[p, M_W3F] = pk.fwd(W3) * K
-                mat[p, M_W2B] = pk.bwd(W2)
-                mat[p, M_W3B] = pk.bwd(W3)
+            W1, b1, W2, b2, W3, b3 = self._stacked()
+            vec[:, V_W1] = pk.vec(W1) * K
+            vec[:, V_B1] = pk.vec(b1) * K
+            vec[:, V_B2] = pk.vec(b2) * K
+            vec[:, V_B3] = pk.vec(b3) * K
+            vec[:, V_R2] = pk.vec(torch.ones_like(b2))
+            vec[:, V_R3] = pk.vec(torch.ones_like(b3))
+            vec[:, V_GPAD] = pk.vec(self._gpad(W1, b1, W2, b2, W3, b3))
+            mat[:, M_W2F] = pk.fwd(W2) * K
+            mat[:, M_W3F] = pk.fwd(W3) * K
+            mat[:, M_W2B] = pk.bwd(W2)
+            mat[:, M_W3B] = pk.bwd(W3)
         return vec.contiguous(), (mat.to(torch.bfloat16) if self.bf16 else mat).contiguous()
 
     # ---------------------------------------------------------------- fitting
@@ -279,22 +296,28 @@ class DPTrainFused(torch.nn.Module):
         wsum = (c[:, None] * npad)[:, :, None] * proj[:, None, :]                        # [N, T, E]
         u = torch.zeros((T, T, E), dtype=torch.float32, device=grad.device).index_add_(0, ti, wsum)
         has_atoms = [bool(idx.numel()) for idx in b.by_type]
-        listed_any = [bool(b.type_present[:, t].any()) for t in range(T)]
-        out = []
+        listed_any = b.type_present.any(0).cpu().tolist()
+        # the empty-slot term's gradient, all pairs at once
+        with torch.enable_grad():
+            leaves = [x.detach().requires_grad_() for x in self._stacked()]
+            gpad = self._gpad(*leaves)
+            pg = torch.autograd.grad((u.reshape(T * T, E) * gpad).sum(), leaves)
         o = self._offsets()
-        for p in range(T * T):
+        P = T * T
+        gW1 = grad[:, o["W1"]:o["W1"] + E] + pg[0]
+        gb1 = grad[:, o["B1"]:o["B1"] + E] + pg[1]
+        gW2 = grad[:, o["W2"]:o["W2"] + E * E].reshape(P, E, E) + pg[2]
+        gb2 = grad[:, o["B2"]:o["B2"] + E] + pg[3]
+        gW3 = grad[:, o["W3"]:o["W3"] + E * E].reshape(P, E, E) + pg[4]
+        gb3 = grad[:, o["B3"]:o["B3"] + E] + pg[5]
+        out = []
+        for p in range(P):
             a, t = divmod(p, T)
             net = self.emb[p]
             if not (has_atoms[a] and listed_any[t]):
                 out += [None] * (3 * len(net.W))
                 continue
-            gpad = net(self.ri_pad[a, 0].view(1, 1)).reshape(-1)
-            pg = torch.autograd.grad((u[a, t] * gpad).sum(), [net.W[0], net.W[1], net.W[2], net.b[0], net.b[1],
-                                                               net.b[2]])
-            gk = grad[p]
-            out += [gk[o["W1"]:o["W1"] + E].view(1, E) + pg[0], gk[o["W2"]:o["W2"] + E * E].view(E, E) + pg[1],
-                    gk[o["W3"]:o["W3"] + E * E].view(E, E) + pg[2], gk[o["B1"]:o["B1"] + E] + pg[3],
-                    gk[o["B2"]:o["B2"] + E] + pg[4], gk[o["B3"]:o["B3"] + E] + pg[5]]
+            out += [gW1[p].view(1, E), gW2[p], gW3[p], gb1[p], gb2[p], gb3[p]]
             out += [None] * len(net.r)
         return out
 

@@ -1,6 +1,11 @@
-"""Launch shapes of the fused DP kernels (matpl_dp_fused_set_launch) change speed, not results: every
-shape the autotuner may pick gives the same energies and forces bit for bit (gather mode sums in a
-fixed order) and the virial to FP64 summation order; the autotuner picks one, sets it and caches it."""
+"""Launch shapes of the fused DP kernels (matpl_dp_fused_set_launch, matpl_dp_fused_set_waves) change
+speed, not results. Within one instantiation (wavefronts per block) every grid cap, weight placement
+(LDS or global) and block order gives the same energies and forces bit for bit (gather mode sums in a
+fixed order) and the virial to FP64 summation order. The 2, 4 and 8 wavefront instantiations are
+compiled separately and may round differently: their forces agree within 1e-6 of the largest force
+component (FP32 rounding: measured 3.7e-9 against a largest component of 0.0315, 1.2e-7, at 8 waves),
+energies bit for bit. The autotuner picks
+a shape, sets it and caches it."""
 import numpy as np
 import pytest
 import torch
@@ -43,12 +48,19 @@ def test_results_independent_of_launch_shape(dev):
         for k in (0, 1):
             for cand in ({"max_blocks": 8, "lds_weights": 0, "xcd": 1}, {"max_blocks": 3, "lds_weights": -1, "xcd": 1},
                          {"max_blocks": 64, "lds_weights": 1, "xcd": 0}, {"max_blocks": 16384, "lds_weights": 0,
-                                                                          "xcd": 1}):
+                                                                          "xcd": 1},
+                         {"max_blocks": 5, "lds_weights": 1, "xcd": 0, "waves": 2},
+                         {"max_blocks": 4096, "lds_weights": -1, "xcd": 1, "waves": 8},
+                         {"max_blocks": 7, "lds_weights": 0, "xcd": 0, "waves": 8}):
                 autotune.set_launch(k, **cand)
                 Ei, F, V, status = run()
                 torch.cuda.synchronize()
                 assert int(status.item()) == 0
-                assert torch.equal(Ei, Ei0) and torch.equal(F, F0), (k, cand)
+                assert torch.equal(Ei, Ei0), (k, cand)
+                if cand.get("waves", 4) == 4:
+                    assert torch.equal(F, F0), (k, cand)
+                else:
+                    assert float((F - F0).abs().max()) <= 1e-6 * float(F0.abs().max()), (k, cand)
                 assert torch.allclose(V, V0, rtol=1e-12, atol=1e-12 * float(V0.abs().max())), (k, cand)
             autotune.set_launch(k, **autotune.DEFAULT)
     finally:
