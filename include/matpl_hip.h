@@ -358,6 +358,36 @@ int matpl_dp_fitting(const float *A, const int32_t *tiles, int ntiles, const voi
                      const void *whf, const void *whb, const float *vec, int64_t type_stride_w0f,
                      int64_t type_stride_w0b, int64_t type_stride_wh, int64_t type_stride_vec, int E, int nlayers,
                      int bf16, float *Ei, float *dA, void *stream);
+/* The same with tiles over positions of perm [Npos] (the atom at each
+ * position; atoms grouped by type need not be contiguous, e.g. packed frames). */
+int matpl_dp_fitting_perm(const float *A, const int32_t *tiles, const int32_t *perm, int ntiles, const void *w0f,
+                          const void *w0b, const void *whf, const void *whb, const float *vec,
+                          int64_t type_stride_w0f, int64_t type_stride_w0b, int64_t type_stride_wh,
+                          int64_t type_stride_vec, int E, int nlayers, int bf16, float *Ei, float *dA, void *stream);
+/*
+ * The fitting nets' part of the force-loss double backward for training (the
+ * reference's dE = autograd.grad(Etot, Ri, create_graph=True) and
+ * loss.backward() through FittingNet, src/model/dp_dp.py:304-429,
+ * src/PWMLFF/dp_mods/dp_trainer.py:274). Per atom, with At [N][4][E] the
+ * tangent of A along the force residual and gE [N] = dL/dEi of the energy
+ * loss (NULL: 0), the net runs forward with tangents and then the reverse
+ * pass of L_i = gE_i E_i - <dE_i/dA_i, At_i> (the force loss reaches the
+ * fitting side only through that term), giving Ahat [N][4][E] = dL/dA and,
+ * per position of perm (tiles and weights as matpl_dp_fitting_perm), the
+ * rows whose sums over each type's positions are the parameter gradients:
+ * xw0 [Npos][2][16E] (DR, its tangent) and zw0 [Npos][2][64] (the layer-0
+ * pre-activation adjoint and its tangent's adjoint): dW0 = sum xw0^T zw0,
+ * db0 = sum zw0[., 0]; xh, zh [Npos][nlayers-1][2][64] likewise for each
+ * hidden layer, rb [Npos][nlayers-1][64] the resnet_dt gradient terms,
+ * xo [Npos][64] = gE x - x_tangent the output weight's; tz [Npos][nlayers]
+ * [2][2][32] is scratch. Row buffers 16-byte aligned; hidden features past
+ * the width are zero.
+ */
+int matpl_dp_fit_train(const float *A, const float *At, const float *gE, const int32_t *perm, const int32_t *tiles,
+                       int ntiles, const void *w0f, const void *w0b, const void *whf, const void *whb,
+                       const float *vec, int64_t type_stride_w0f, int64_t type_stride_w0b, int64_t type_stride_wh,
+                       int64_t type_stride_vec, int E, int nlayers, int bf16, float *Ahat, float *xw0, float *zw0,
+                       float *xh, float *zh, float *rb, float *xo, float *tz, void *stream);
 
 /* ------------------------------------------------------------ DP compress */
 /*

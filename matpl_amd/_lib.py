@@ -61,6 +61,8 @@ SIGNATURES = {
     "matpl_dp_lmp_neighdata": (I32, [P, P, P, I32, P, P, P, P, I64, I64, I64, I32, I32, F64, P, P, P, P, P, P, P]),
     "matpl_dp_neighbor_reverse": (I32, [P, P, P, P, P, P, I32, I32, I64, P, P, P]),
     "matpl_dp_fitting": (I32, [P, P, I32, P, P, P, P, P, I64, I64, I64, I64, I32, I32, I32, P, P, P]),
+    "matpl_dp_fitting_perm": (I32, [P, P, P, I32, P, P, P, P, P, I64, I64, I64, I64, I32, I32, I32, P, P, P]),
+    "matpl_dp_fit_train": (I32, [P, P, P, P, P, I32, P, P, P, P, P, I64, I64, I64, I64, I32, I32, I32] + [P] * 9),
     # DP compress
     "matpl_dp_compress_f32": (I32, [P, P, I64, I32, I32, P, P]),
     "matpl_dp_compress_f64": (I32, [P, P, I64, I32, I32, P, P]),

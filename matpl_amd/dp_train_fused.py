@@ -129,13 +129,79 @@ class _EmbPack:
         return torch.where(self.ok, W[:, self.im, self.fm], torch.zeros((), dtype=W.dtype, device=W.device))
 
 
+class _FitPack:
+    """The fitting nets' parameters (torch, on the device) in the matrix-core fitting kernels' operand
+    layouts (matpl_amd.dp_fitting.FittingMFMA's packing, built on the device every step)."""
+
+    def __init__(self, E: int, H: int, L: int, bf16: bool, device):
+        from .dp_fitting import _lane_maps
+        self.E, self.H, self.L, self.bf16 = E, H, L, bf16
+        D = 16 * E
+        self.D, self.Dh, self.nb0 = D, D // 2, (D + 31) // 32
+        hh, ii, fin = _lane_maps()
+        kk = np.arange(self.Dh)
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.int64, device=device)
+        if bf16:
+            k = 16 * (kk[None, :] // 8) + 8 * hh[:, None] + (kk[None, :] % 8)
+        else:
+            k = 2 * kk[None, :] + hh[:, None]
+        self.k0 = t(k)                                                         # [64, Dh] DR index
+        self.out = t(np.stack([32 * ob + ii[:, None] for ob in range(2)]))     # [2, 64, 1]
+        self.rows = t(np.stack([32 * blk + ii[:, None] for blk in range(self.nb0)]))   # [nb0, 64, 1]
+        self.fin = t(fin)                                                      # [64, 32]
+        f = np.array([[[32 * blk + feat(rr, h) for rr in range(16)] for h in range(2)] for blk in range(2)])
+        self.vf = t(f)                                                         # [2 blk, 2 h, 16]
+        self.device = device
+
+    def __call__(self, nets):
+        """(w0f, w0b, whf, whb, vec, strides) for a list of per-type _Net fitting nets."""
+        E, H, L, D = self.E, self.H, self.L, self.D
+        T = len(nets)
+        dev = self.device
+        W0 = torch.zeros((T, self.nb0 * 32, 64), dtype=torch.float32, device=dev)
+        W0[:, :D, :H] = torch.stack([n.W[0] for n in nets])
+        w0f = W0[:, self.k0[None], self.out]                                   # [T, 2, 64, Dh]
+        w0b = W0[:, self.rows, self.fin[None]]                                 # [T, nb0, 64, 32]
+        if L > 1:
+            Wh = torch.zeros((T, L - 1, 64, 64), dtype=torch.float32, device=dev)
+            Wh[:, :, :H, :H] = torch.stack([torch.stack([n.W[l] for l in range(1, L)]) for n in nets])
+            whf = Wh[:, :, self.fin[None], self.out]                           # [T, L-1, 2, 64, 32]
+            whb = Wh[:, :, self.out, self.fin[None]]
+        else:
+            whf = whb = torch.zeros((T, 1, 2, 64, 32), dtype=torch.float32, device=dev)
+        b = torch.zeros((T, L, 64), dtype=torch.float32, device=dev)
+        r = torch.zeros((T, L, 64), dtype=torch.float32, device=dev)
+        b[:, :, :H] = torch.stack([torch.stack([n.b[l] for l in range(L)]) for n in nets])
+        r[:, :, :H] = torch.stack([torch.stack([n.r[l] if n.has_rdt[l] else torch.ones_like(n.b[l])
+                                                for l in range(L)]) for n in nets])
+        wo = torch.zeros((T, 64), dtype=torch.float32, device=dev)
+        wo[:, :H] = torch.stack([n.W[L][:, 0] for n in nets])
+        bo = torch.stack([n.b[L].reshape(()) for n in nets])
+        vec = torch.zeros((T, L * 3 * 2 * 2 * 16 + 2 * 2 * 16 + 16), dtype=torch.float32, device=dev)
+        v = vec[:, :L * 3 * 64].view(T, L, 3, 2, 2, 16)
+        v[:, :, 0] = b[:, :, self.vf]
+        v[:, :, 1] = r[:, :, self.vf]
+        vec[:, L * 3 * 64:L * 3 * 64 + 64] = wo[:, self.vf].reshape(T, 64)
+        vec[:, L * 3 * 64 + 64] = bo
+        dt = torch.bfloat16 if self.bf16 else torch.float32
+        w0f, w0b, whf, whb = (x.to(dt).contiguous() for x in (w0f, w0b, whf, whb))
+        strides = (w0f[0].numel(), w0b[0].numel(), whf[0].numel(), vec[0].numel())
+        return w0f, w0b, whf, whb, vec.contiguous(), strides
+
+
 class DPTrainFused(torch.nn.Module):
     """The DP model's parameters (FP32, DPTorch's module layout) and its training step on the
     hand-written kernels. precision: "bf16" (BF16 matrix-core products, FP32 accumulation) or
     "fp32"."""
 
-    def __init__(self, w: DPWeights, precision: str = "bf16", device="cuda", pe: float = 1.0, pf: float = 1.0):
+    def __init__(self, w: DPWeights, precision: str = "bf16", device="cuda", pe: float = 1.0, pf: float = 1.0,
+                 fitting: str = "kernel"):
+        """fitting: "kernel" (the fitting nets' energies, first and second derivatives on the
+        matrix-core kernels matpl_dp_fitting_perm / matpl_dp_fit_train) or "torch" (torch autograd
+        with create_graph)."""
         super().__init__()
+        if fitting not in ("kernel", "torch"):
+            raise ValueError("fitting must be 'kernel' or 'torch'")
         cfg = w.cfg
         if precision not in ("bf16", "fp32"):
             raise ValueError("precision must be 'bf16' or 'fp32'")
@@ -165,6 +231,16 @@ class DPTrainFused(torch.nn.Module):
         self._pk = _EmbPack(self.E, dev)
         self.gsize = _lib.query("matpl_dp_train_grad_size", self.E)
         self.chunk = 0          # atoms per wavefront of the gradient kernel (0: chosen from the batch)
+        fit0 = w.fit[0]
+        H, L = fit0[0][0].shape[1], len(fit0) - 1
+        ok = (cfg.M2 == 16 and H <= 64 and 1 <= L <= 4 and all(l[0].shape[1] == H for l in fit0[1:-1]) and
+              fit0[-1][0].shape[1] == 1 and all(b is not None for _, b, _ in fit0))
+        if fitting == "kernel" and not ok:
+            raise ValueError("the fitting kernels need M2 = 16, 1..4 hidden layers of one width <= 64, biases")
+        self.fitting = fitting
+        self.H, self.L = H, L
+        self._fp = _FitPack(self.E, H, L, self.bf16, dev) if fitting == "kernel" else None
+        self._tiles = {}
 
     # ---------------------------------------------------------------- weights
     def _stacked(self):
@@ -233,8 +309,73 @@ class DPTrainFused(torch.nn.Module):
         return (_ptr(self.tinfo), b.natoms, cfg.ntypes, mnl, cfg.m, self.E, int(self.bf16), float(cfg.rcut),
                 float(cfg.rcut_smooth), 1.0 / (cfg.m * cfg.ntypes))
 
+    def _fit_tiles(self, b: FusedBatch):
+        """perm (atoms grouped by type, int32), tiles of <= 32 positions of one type, and each type's
+        position range, for the fitting kernels (cached per batch)."""
+        key = id(b)
+        hit = self._tiles.get(key)
+        if hit is not None and hit[0] is b:
+            return hit[1]
+        rows, ranges, off = [], [], 0
+        for t, idx in enumerate(b.by_type):
+            c = int(idx.numel())
+            for s0 in range(0, c, 32):
+                rows.append((t, off + s0, min(32, c - s0)))
+            ranges.append((off, off + c))
+            off += c
+        perm = torch.cat(list(b.by_type)).to(torch.int32).contiguous()
+        tiles = torch.as_tensor(np.asarray(rows, np.int32).reshape(-1, 3), device=perm.device)
+        val = (perm, tiles, ranges)
+        self._tiles = {key: (b, val)}
+        return val
+
+    def _fit_kernel_grads(self, b: FusedBatch, A, dA, gE, fpk):
+        """Ahat = dL/dA and the fitting-net gradients (self.fit.parameters() order) from
+        matpl_dp_fit_train's per-atom rows: weight gradients are sums of products over each type's
+        atoms, bias / rdt / output gradients sums of rows."""
+        w0f, w0b, whf, whb, vec, strides = fpk
+        perm, tiles, ranges = self._fit_tiles(b)
+        N, E, H, L, D = b.natoms, self.E, self.H, self.L, 16 * self.E
+        dev = A.device
+        f32 = torch.float32
+        ahat = torch.empty_like(A)
+        xw0 = torch.empty((N, 2, D), dtype=f32, device=dev)
+        zw0 = torch.empty((N, 2, 64), dtype=f32, device=dev)
+        nh = max(L - 1, 1)
+        xh = torch.empty((N, nh, 2, 64), dtype=f32, device=dev)
+        zh = torch.empty((N, nh, 2, 64), dtype=f32, device=dev)
+        rb = torch.empty((N, nh, 64), dtype=f32, device=dev)
+        xo = torch.empty((N, 64), dtype=f32, device=dev)
+        tz = torch.empty((N, L, 2, 2, 32), dtype=f32, device=dev)
+        _lib.call("matpl_dp_fit_train", _ptr(A), _ptr(dA), _ptr(gE), _ptr(perm), _ptr(tiles), tiles.shape[0],
+                  _ptr(w0f), _ptr(w0b), _ptr(whf), _ptr(whb), _ptr(vec), *strides, E, L, int(self.bf16), _ptr(ahat),
+                  _ptr(xw0), _ptr(zw0), _ptr(xh), _ptr(zh), _ptr(rb), _ptr(xo), _ptr(tz), _stream())
+        gE_pos = gE[perm.to(torch.int64)]
+        out = []
+        for t, net in enumerate(self.fit):
+            a, e = ranges[t]
+            if e == a:
+                out += [None] * (3 * len(net.W))
+                continue
+            gW = [xw0[a:e].reshape(-1, D).t() @ zw0[a:e].reshape(-1, 64)[:, :H]]
+            gb = [zw0[a:e, 0, :H].sum(0)]
+            gr = [None]
+            for l in range(1, L):
+                X = xh[a:e, l - 1].reshape(-1, 64)[:, :H]
+                Z = zh[a:e, l - 1].reshape(-1, 64)[:, :H]
+                gW.append(X.t() @ Z)
+                gb.append(zh[a:e, l - 1, 0, :H].sum(0))
+                gr.append(rb[a:e, l - 1, :H].sum(0) if net.has_rdt[l] else None)
+            gW.append(xo[a:e, :H].sum(0).reshape(H, 1))
+            gb.append(gE_pos[a:e].sum().reshape(1))
+            gr.append(None)
+            out += gW + gb + gr
+        return ahat, out
+
     def loss_and_grads(self, b: FusedBatch):
         """(loss, gradients aligned with self.parameters(); None for parameters the loss does not reach)."""
+        if self.fitting == "kernel":
+            return self._loss_and_grads_kernels(b)
         cfg = self.cfg
         T, m, N, E = cfg.ntypes, cfg.m, b.natoms, self.E
         dev = b.pos.device
@@ -273,6 +414,62 @@ class DPTrainFused(torch.nn.Module):
         ltot = self.pe * le - (Abar * dA).sum()
         g = torch.autograd.grad(ltot, fparams + [A], allow_unused=True)
         gfit, ahat = list(g[:-1]), g[-1].detach().contiguous()
+        # 7. embedding gradients
+        grad = torch.zeros((T * T, self.gsize), dtype=torch.float32, device=dev)
+        chunk = self.chunk or max(8, -(-N * T // 8192))
+        _lib.call("matpl_dp_train_embgrad", *head, _ptr(vec), _ptr(mat), *tail, _ptr(resid), _ptr(abar), _ptr(ahat),
+                  _ptr(grad), chunk, _ptr(status), _stream())
+        if int(status.item()) != 0:
+            raise OverflowError("more than maxNeighborNum neighbors of one type within rcut")
+        gemb = self._emb_grads(grad, b, ahat, nin)
+        loss = (self.pe * le + self.pf * lf).detach()
+        return loss, gemb + gfit
+
+    def _loss_and_grads_kernels(self, b: FusedBatch):
+        """The step with the fitting nets on the matrix-core kernels as well: energies and Abar from
+        matpl_dp_fitting_perm, the fitting side's second derivatives from matpl_dp_fit_train."""
+        cfg = self.cfg
+        T, N, E = cfg.ntypes, b.natoms, self.E
+        dev = b.pos.device
+        mnl = b.nl_list.shape[1] // T
+        vec, mat = self._pack()
+        fpk = self._fp(list(self.fit))
+        w0f, w0b, whf, whb, fvec, strides = fpk
+        perm, tiles, _ = self._fit_tiles(b)
+        head, tail = self._head(b), self._tail(b, mnl)
+        status = torch.zeros(1, dtype=torch.int32, device=dev)
+        # 1. descriptor A
+        A = torch.empty((N, 4, E), dtype=torch.float32, device=dev)
+        _lib.call("matpl_dp_fused_forward", *head, _ptr(vec), _ptr(mat), *tail, _ptr(A), _ptr(status), _stream())
+        # 2. fitting: Ei and Abar = dEi/dA
+        Ei = torch.empty(N, dtype=torch.float32, device=dev)
+        abar = torch.empty_like(A)
+        _lib.call("matpl_dp_fitting_perm", _ptr(A), _ptr(tiles), _ptr(perm), tiles.shape[0], _ptr(w0f), _ptr(w0b),
+                  _ptr(whf), _ptr(whb), _ptr(fvec), *strides, E, self.L, int(self.bf16), _ptr(Ei), _ptr(abar),
+                  _stream())
+        af = b.atom_frame.to(torch.int64)
+        Ef = torch.zeros(b.nframes, dtype=torch.float64, device=dev).index_add(0, af, Ei.double())
+        # 3. forces
+        F = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        V = torch.zeros((b.nframes, 9), dtype=torch.float64, device=dev)
+        fslot = torch.empty((N * T * mnl, 4), dtype=torch.float32, device=dev)
+        fcenter = torch.empty((N, 3), dtype=torch.float32, device=dev)
+        _lib.call("matpl_dp_fused_backward", *head, _ptr(vec), _ptr(mat), *tail, _ptr(abar), _ptr(b.rev), _ptr(fslot),
+                  _ptr(fcenter), _ptr(F), _ptr(V), _ptr(status), _stream())
+        _lib.call("matpl_dp_fused_gather", _ptr(b.nl_counts), _ptr(fslot), _ptr(fcenter), _ptr(F), T, mnl, N, _stream())
+        # 4. loss, dL/dEi of the energy term, and the force residual r = dL/dF
+        de = (Ef - b.E_label) / b.frame_natoms
+        le = (de ** 2).mean()
+        lf = ((F.double() - b.F_label) ** 2).mean()
+        gE = (self.pe * 2.0 / b.nframes * de / b.frame_natoms)[af].to(torch.float32).contiguous()
+        resid = (2.0 * self.pf / (3 * N) * (F.double() - b.F_label)).to(torch.float32).contiguous()
+        # 5. tangent of A along r
+        dA = torch.empty((N, 4, E), dtype=torch.float32, device=dev)
+        nin = torch.empty((N, T), dtype=torch.int32, device=dev)
+        _lib.call("matpl_dp_train_tangent", *head, _ptr(vec), _ptr(mat), *tail, _ptr(resid), _ptr(dA), _ptr(nin),
+                  _ptr(status), _stream())
+        # 6. fitting side: Ahat = d(pe L_E - <Abar, dA>)/dA and the fitting-net gradients
+        ahat, gfit = self._fit_kernel_grads(b, A, dA, gE, fpk)
         # 7. embedding gradients
         grad = torch.zeros((T * T, self.gsize), dtype=torch.float32, device=dev)
         chunk = self.chunk or max(8, -(-N * T // 8192))

@@ -2,9 +2,9 @@
 speed, not results. Within one instantiation (wavefronts per block) every grid cap, weight placement
 (LDS or global) and block order gives the same energies and forces bit for bit (gather mode sums in a
 fixed order) and the virial to FP64 summation order. The 2, 4 and 8 wavefront instantiations are
-compiled separately and may round differently: their forces agree within 1e-6 of the largest force
-component (FP32 rounding: measured 3.7e-9 against a largest component of 0.0315, 1.2e-7, at 8 waves),
-energies bit for bit. The autotuner picks
+compiled separately and may round differently: their forces and virials agree within 1e-6 of the
+largest component (FP32 rounding: measured 3.7e-9 against a largest force component of 0.0315,
+1.2e-7, and 2.1e-8 on the virial, at 8 waves), energies bit for bit. The autotuner picks
 a shape, sets it and caches it."""
 import numpy as np
 import pytest
@@ -59,9 +59,10 @@ def test_results_independent_of_launch_shape(dev):
                 assert torch.equal(Ei, Ei0), (k, cand)
                 if cand.get("waves", 4) == 4:
                     assert torch.equal(F, F0), (k, cand)
+                    assert torch.allclose(V, V0, rtol=1e-12, atol=1e-12 * float(V0.abs().max())), (k, cand)
                 else:
                     assert float((F - F0).abs().max()) <= 1e-6 * float(F0.abs().max()), (k, cand)
-                assert torch.allclose(V, V0, rtol=1e-12, atol=1e-12 * float(V0.abs().max())), (k, cand)
+                    assert float((V - V0).abs().max()) <= 1e-6 * float(V0.abs().max()), (k, cand)
             autotune.set_launch(k, **autotune.DEFAULT)
     finally:
         autotune.reset()

@@ -1,8 +1,8 @@
 """The batched DP training step on the hand-written kernels (matpl_amd.dp_train_fused.DPTrainFused:
-fused forward and force kernels, the force-loss double backward of the embedding nets on the matrix
-cores) against the plain FP64 path (DPTorch over all T*m slots with create_graph,
-matpl_amd.dp_train.plain_grads), whose gradients the last test pins against central differences
-of the FP64 oracle's loss:
+fused forward and force kernels, the force-loss double backward of the embedding and fitting nets on
+the matrix cores; the fitting side also on torch autograd) against the plain FP64 path (DPTorch over
+all T*m slots with create_graph, matpl_amd.dp_train.plain_grads), whose gradients the last test pins
+against central differences of the FP64 oracle's loss:
   loss within 1e-5 (FP32) / 3e-3 (BF16) relative, every parameter gradient within 2e-4 (FP32) /
   3e-2 (BF16) relative norm; frames of Li-Ge-P-S with N in [24, 512], and frames whose type lists
   differ (one lacks a type)."""
@@ -32,12 +32,12 @@ def _weights(frames, cfg, dev, seed):
     return DPWeights.random(cfg, seed=seed, davg=davg.cpu().numpy(), dstd=dstd.cpu().numpy())
 
 
-def _compare(frames, cfg, w, dev, precision, seed=3):
+def _compare(frames, cfg, w, dev, precision, seed=3, fitting="kernel"):
     ref_batch = pack_batch(frames, cfg, dev, seed=seed, keep_full=True)
     ref_loss, ref_g = plain_grads(DPTorch(w, torch.float64, dev), ref_batch)
     fb = pack_fused_batch(frames, cfg, dev, seed=seed)
     assert torch.equal(fb.E_label, ref_batch.E_label) and torch.equal(fb.F_label, ref_batch.F_label)
-    model = DPTrainFused(w, precision, dev)
+    model = DPTrainFused(w, precision, dev, fitting=fitting)
     loss, g = model.loss_and_grads(fb)
     tl, tg = TOL[precision]
     print(f"loss {loss.item():.8e} plain FP64 {ref_loss.item():.8e}", flush=True)
@@ -62,6 +62,38 @@ def test_fused_step_matches_plain_fp64(dev, precision):
     w = _weights(frames, cfg, dev, seed=22)
     print(f"case fused training step {precision}: sizes {[f.natoms for f in frames]}", flush=True)
     _compare(frames, cfg, w, dev, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_fused_step_torch_fitting_matches_plain_fp64(dev, precision):
+    """The same step with the fitting nets' derivatives from torch autograd instead of the kernels."""
+    frames = st.lgps_frames(nframes=5, nmin=24, nmax=72, seed=21)
+    cfg = DPConfig(ntypes=4, m=150)
+    w = _weights(frames, cfg, dev, seed=22)
+    print(f"case fused training step, torch fitting {precision}", flush=True)
+    _compare(frames, cfg, w, dev, precision, fitting="torch")
+
+
+def test_fitting_kernels_match_torch_fitting(dev):
+    """FP32: the fitting kernels' loss and gradients against the torch-fitting path of the same step
+    (both FP32, same embedding kernels): loss within 1e-6, every gradient within 1e-4 relative norm."""
+    frames = st.lgps_frames(nframes=4, nmin=64, nmax=200, seed=8)
+    cfg = DPConfig(ntypes=4, m=150)
+    w = _weights(frames, cfg, dev, seed=9)
+    fb = pack_fused_batch(frames, cfg, dev, seed=3)
+    print("case fitting kernels vs torch fitting (fp32)", flush=True)
+    l1, g1 = DPTrainFused(w, "fp32", dev, fitting="kernel").loss_and_grads(fb)
+    l2, g2 = DPTrainFused(w, "fp32", dev, fitting="torch").loss_and_grads(fb)
+    assert abs(l1.item() - l2.item()) <= 1e-6 * abs(l2.item())
+    names = [n for n, _ in DPTrainFused(w, "fp32", dev).named_parameters()]
+    worst = 0.0
+    for name, a, b in zip(names, g1, g2):
+        assert (a is None) == (b is None), name
+        if a is not None and b.norm() > 0:
+            e = float((a - b).norm() / b.norm())
+            worst = max(worst, e)
+            assert e <= 1e-4, (name, e)
+    print(f"worst relative difference {worst:.2e}", flush=True)
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
