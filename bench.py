@@ -599,7 +599,15 @@ def main():
                     "traffic": traffic,
                     "traffic_GBps": round(traffic / kt / 1e9, 1) if traffic else None,
                     "kernel": "dp_fused_bwd_kernel", "avg_ms": round(kt * 1e3, 4),
-                    "in_cutoff_neighbors": nvalid}
+                    "in_cutoff_neighbors": nvalid,
+                    # the kernel issues ~69 VALU per MFMA instruction: VALU issue, not the matrix
+                    # cores or HBM, limits it (SQ counters in profiles/dp_md_hfo2_99636_instruction_mix.md)
+                    "limiter": "valu"}
+        # the whole MD step against the same peak: 4 (1375 n + 26,650) flop per atom and step
+        # (forward, fitting, backward of the DP path; n = in-cutoff neighbors of the atom)
+        path_flops = 4.0 * (1375.0 * nvalid + 26650.0 * N)
+        roofline["path_tflops"] = round(path_flops / (ms_per_step * 1e-3) / 1e12, 3)
+        roofline["path_frac"] = round(roofline["path_tflops"] / BF16_PEAK_TFLOPS, 5)
 
     compressed, ct = None, None
     if args.compress:
@@ -643,6 +651,7 @@ def main():
     if args.copy_calib:
         copy_calibration(args.copy_calib)
 
+    autotune.reset()        # the other legs run other workloads: default launch shapes
     nep = None if args.no_nep else nep_leg()
     nep_train = None if args.no_nep else nep_train_leg()
     dp_train = None if args.no_train else dp_train_leg()

@@ -541,7 +541,7 @@ int matpl_nep_virial_grad(int f64, const int64_t *NL, const void *Rij, const voi
  * net's first and second derivatives (the force loss reaches the weights
  * through g). part: workspace of ntiles * matpl_nep_fit_part_width(D, H)
  * elements; tstart [T+1]: type t's tiles are [tstart[t], tstart[t+1]).
- * Results do not depend on scheduling (per-tile sums added in tile order).
+ * Results do not depend on scheduling (per-tile sums added in a fixed order).
  */
 int matpl_nep_fit_tile_atoms(int f64, int D, int H);
 int matpl_nep_fit_part_width(int D, int H);

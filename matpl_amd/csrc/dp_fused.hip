@@ -541,7 +541,12 @@ int launch(FusedParams &p, bool fwd, hipStream_t st, int ck) {
     if (ck == 4) return launch_lw<E, BF16, false, 4>(p, fwd, st);
     if (ck == 6) return launch_lw<E, BF16, false, 6>(p, fwd, st);
     const int mode = g_launch[fwd ? 0 : 1].lds_weights;
-    const int w = g_launch[fwd ? 0 : 1].waves;
+    // a wavefront count set for another workload whose neighbor tiles would exceed the LDS budget
+    // here falls back to fewer wavefronts per block
+    int w = g_launch[fwd ? 0 : 1].waves;
+    while (w > 2 && (fwd ? fused_lds_bytes<BF16, false, false>(p.T, p.cap, w)
+                         : fused_lds_bytes<BF16, false, true>(p.T, p.cap, w)) > 64 * 1024)
+        w /= 2;
     const bool fits = (fwd ? fused_lds_bytes<BF16, true, false>(p.T, p.cap, w)
                            : fused_lds_bytes<BF16, true, true>(p.T, p.cap, w)) <= 64 * 1024;
     MATPL_CHECK_ARG(mode != 1 || fits, "dp_fused: weights do not fit in LDS for this T and list width");

@@ -19,7 +19,7 @@
 // subset of hidden units (or features) for atom a, with the weights read as the
 // same address across the atom lanes. Parameter gradients are summed per tile
 // over its atoms in LDS (threads over weight entries) and written as one partial
-// row per tile; a second kernel adds each type's rows in tile order, so results
+// row per tile; a second kernel adds each type's rows in a fixed order, so results
 // do not depend on scheduling.
 #include "common.h"
 
@@ -195,15 +195,25 @@ __global__ void __launch_bounds__(FIT_THREADS) nep_fit_bwd_kernel(FitArgs a) {
     }
 }
 
-// Parameter gradients of type t = sum of its tiles' rows [tstart[t], tstart[t+1]) in tile order.
+// Parameter gradients of type t = sum of its tiles' rows [tstart[t], tstart[t+1]): a block takes 32
+// entries, its 8 thread groups sum every 8th tile each, and the 8 partial sums are added in group
+// order (deterministic).
+constexpr int RED_ENTRIES = 32, RED_GROUPS = 8;
 template <typename T>
-__global__ void nep_fit_reduce_kernel(const T *part, const int32_t *tstart, int PW, int D, int H, T *dW0, T *db0,
-                                      T *dw1, T *db1) {
+__global__ void __launch_bounds__(RED_ENTRIES * RED_GROUPS)
+nep_fit_reduce_kernel(const T *part, const int32_t *tstart, int PW, int D, int H, T *dW0, T *db0, T *dw1, T *db1) {
+    __shared__ T red[RED_GROUPS][RED_ENTRIES];
     const int t = blockIdx.y;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= PW) return;
+    const int el = threadIdx.x % RED_ENTRIES, g = threadIdx.x / RED_ENTRIES;
+    const int e = blockIdx.x * RED_ENTRIES + el;
     T s = T(0);
-    for (int k = tstart[t]; k < tstart[t + 1]; k++) s += part[(size_t)k * PW + e];
+    if (e < PW)
+        for (int k = tstart[t] + g; k < tstart[t + 1]; k += RED_GROUPS) s += part[(size_t)k * PW + e];
+    red[g][el] = s;
+    __syncthreads();
+    if (g != 0 || e >= PW) return;
+    s = T(0);
+    for (int q = 0; q < RED_GROUPS; q++) s += red[q][el];
     if (e < D * H) dW0[(size_t)t * D * H + e] = s;
     else if (e < D * H + H) db0[(size_t)t * H + e - D * H] = s;
     else if (e < D * H + 2 * H) dw1[(size_t)t * H + e - D * H - H] = s;
@@ -281,13 +291,14 @@ MATPL_EXPORT int matpl_nep_fit_backward(int f64, const void *q, const void *qs, 
         int rc = check_launch("nep_fit_bwd_kernel");
         if (rc) return rc;
     }
-    const dim3 grid((PW + 255) / 256, ntypes);
+    const dim3 grid((PW + RED_ENTRIES - 1) / RED_ENTRIES, ntypes);
+    const dim3 block(RED_ENTRIES * RED_GROUPS);
     if (f64)
-        hipLaunchKernelGGL(nep_fit_reduce_kernel<double>, grid, dim3(256), 0, st, static_cast<const double *>(part),
+        hipLaunchKernelGGL(nep_fit_reduce_kernel<double>, grid, block, 0, st, static_cast<const double *>(part),
                            tstart, PW, D, H, static_cast<double *>(dW0), static_cast<double *>(db0),
                            static_cast<double *>(dw1), static_cast<double *>(db1));
     else
-        hipLaunchKernelGGL(nep_fit_reduce_kernel<float>, grid, dim3(256), 0, st, static_cast<const float *>(part),
+        hipLaunchKernelGGL(nep_fit_reduce_kernel<float>, grid, block, 0, st, static_cast<const float *>(part),
                            tstart, PW, D, H, static_cast<float *>(dW0), static_cast<float *>(db0),
                            static_cast<float *>(dw1), static_cast<float *>(db1));
     return check_launch("nep_fit_reduce_kernel");

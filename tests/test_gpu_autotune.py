@@ -83,3 +83,38 @@ def test_autotuner_sets_and_caches(dev, tmp_path, monkeypatch):
         assert r2["cached"] and r2["forward"] == r1["forward"] and r2["backward"] == r1["backward"]
     finally:
         autotune.reset()
+
+
+def test_waves_fall_back_when_tiles_exceed_lds(dev):
+    """A wavefront count set for one workload (8 waves, tuned on a two-type box) does not break
+    another whose neighbor tiles would not fit the LDS budget at that count (four types, m = 150):
+    the launch falls back to fewer wavefronts, and the result equals the default launch's within
+    FP32 rounding."""
+    f = st.lgps_frames(nframes=1, nmin=200, nmax=200, seed=3)[0]
+    T, m = 4, 150
+    cfg = DPConfig(ntypes=T, m=m)
+    w = DPWeights.random(cfg, seed=5, davg=np.tile([0.1, 0, 0, 0], (T, 1)), dstd=np.tile([0.2, 0.1, 0.1, 0.1], (T, 1)))
+    nl = find_neighbore_gpu(f.lattice, f.frac(), f.types, T, m, cfg.rcut, device=dev, want_dR=False, want_shift=True)
+    L = torch.as_tensor(f.lattice, device=dev)
+    pos = (nl.frac_wrapped @ L).contiguous()
+    tm = torch.as_tensor(f.types, device=dev)
+    rev, rst = reverse_map(nl, tm, m)
+    assert int(rst.item()) == 0
+    model = FusedDP(w, "bf16", dev)
+    print(f"case waves fallback: N={f.natoms} T={T} m={m}", flush=True)
+
+    def run():
+        out = model(pos, L.reshape(1, 9).contiguous(), tm, nl.list, nl.shift, nl.counts, m, rev=rev)
+        torch.cuda.synchronize()
+        return [x.clone() for x in out]
+    try:
+        autotune.reset()
+        Ei0, F0, V0, s0 = run()
+        for k in (0, 1):
+            autotune.set_launch(k, 4096, -1, 0, 8)
+        Ei, F, V, s = run()
+        assert int(s.item()) == 0
+        assert float((Ei - Ei0).abs().max()) <= 1e-6 * float(Ei0.abs().max())
+        assert float((F - F0).abs().max()) <= 1e-6 * float(F0.abs().max())
+    finally:
+        autotune.reset()
