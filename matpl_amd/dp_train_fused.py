@@ -357,15 +357,20 @@ class DPTrainFused(torch.nn.Module):
             if e == a:
                 out += [None] * (3 * len(net.W))
                 continue
-            gW = [xw0[a:e].reshape(-1, D).t() @ zw0[a:e].reshape(-1, 64)[:, :H]]
-            gb = [zw0[a:e, 0, :H].sum(0)]
+            # products over the type's positions on whole (64-wide, contiguous) rows; sums of the
+            # bias, resnet_dt and output rows one reduction per buffer
+            gW = [(xw0[a:e].reshape(-1, D).t() @ zw0[a:e].reshape(-1, 64))[:, :H]]
+            zs = zw0[a:e].sum(0)
+            gb = [zs[0, :H]]
             gr = [None]
-            for l in range(1, L):
-                X = xh[a:e, l - 1].reshape(-1, 64)[:, :H]
-                Z = zh[a:e, l - 1].reshape(-1, 64)[:, :H]
-                gW.append(X.t() @ Z)
-                gb.append(zh[a:e, l - 1, 0, :H].sum(0))
-                gr.append(rb[a:e, l - 1, :H].sum(0) if net.has_rdt[l] else None)
+            if L > 1:
+                zhs, rbs = zh[a:e].sum(0), rb[a:e].sum(0)
+                for l in range(1, L):
+                    X = xh[a:e, l - 1].reshape(-1, 64)
+                    Z = zh[a:e, l - 1].reshape(-1, 64)
+                    gW.append((X.t() @ Z)[:H, :H])
+                    gb.append(zhs[l - 1, 0, :H])
+                    gr.append(rbs[l - 1, :H] if net.has_rdt[l] else None)
             gW.append(xo[a:e, :H].sum(0).reshape(H, 1))
             gb.append(gE_pos[a:e].sum().reshape(1))
             gr.append(None)
@@ -491,7 +496,12 @@ class DPTrainFused(torch.nn.Module):
         ti = b.type_map.to(torch.int64)
         proj = torch.einsum("nwe,nw->ne", ahat, self.ri_pad[ti])                        # [N, E]
         wsum = (c[:, None] * npad)[:, :, None] * proj[:, None, :]                        # [N, T, E]
-        u = torch.zeros((T, T, E), dtype=torch.float32, device=grad.device).index_add_(0, ti, wsum)
+        # per centre type sums over its atoms (one gather into type order, then contiguous sums; an
+        # index_add onto T rows would serialize on atomics)
+        perm, _, ranges = self._fit_tiles(b)
+        wp = wsum[perm.to(torch.int64)]
+        u = torch.stack([wp[a:e].sum(0) if e > a else torch.zeros((T, E), dtype=wsum.dtype, device=wsum.device)
+                         for a, e in ranges])
         has_atoms = [bool(idx.numel()) for idx in b.by_type]
         listed_any = b.type_present.any(0).cpu().tolist()
         # the empty-slot term's gradient, all pairs at once
