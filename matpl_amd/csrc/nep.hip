@@ -33,6 +33,7 @@ constexpr int NEP_WAVES = 4;
 constexpr int NB_MAX = 20;      // basis functions
 constexpr int NM_MAX = 8;       // radial functions per descriptor part
 constexpr int NLM = 24;
+constexpr int WS = NLM + 1;     // LDS row stride of a slot's harmonics (odd: lanes writing their rows hit distinct banks)
 constexpr double PI_D = 3.14159265358979323846;
 
 // f_k(r) and f_k'(r) for k = 0, 1, 2, ... in order, one per next() call
@@ -377,6 +378,7 @@ struct FeatArgs {
     void *gc;               // [T][T][nm][nb] out (accumulated)
     void *ggfeat;           // [N][ld] out (2nd order)
     int64_t N;
+    int pw;                 // coefficient-gradient tables per wavefront (plain adds) instead of per block (LDS atomics)
     int S, T, nm, nb, ld, col0, l3, l4, l5;
     double rc;
 };
@@ -391,11 +393,14 @@ __device__ __forceinline__ int slot_count(const int64_t *NL, int64_t i, int S) {
     return lo;
 }
 
+// The block's coefficient-gradient tables (ntab of count entries: one per wavefront, or one shared)
+// summed and added to global memory, one atomic per entry.
 template <typename T>
-__device__ __forceinline__ void block_flush_add(T *lds, T *glob, int count) {
+__device__ __forceinline__ void block_flush_add(T *lds, T *glob, int count, int ntab) {
     __syncthreads();
     for (int x = threadIdx.x; x < count; x += blockDim.x) {
-        const T v = lds[x];
+        T v = lds[x];
+        for (int w = 1; w < ntab; w++) v += lds[w * count + x];
         if (v != T(0)) atomic_add(glob + x, v);
     }
 }
@@ -425,12 +430,14 @@ nep_radial_kernel(FeatArgs a) {
     extern __shared__ unsigned char smem_raw[];
     const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
     const int ncoef = a.T * a.T * a.nm * a.nb;
-    T *gcl = reinterpret_cast<T *>(smem_raw);                    // [T*T*nm*nb] block table (MODE > 0)
+    const int ntab = (MODE > 0 && a.pw) ? NEP_WAVES : 1;
+    T *gcl = reinterpret_cast<T *>(smem_raw);                    // [ntab][T*T*nm*nb] tables (MODE > 0)
+    T *gtab = gcl + (ntab > 1 ? wid : 0) * ncoef;                // this wave's table
     // per-wave chunk: v[RCH][nb] and the neighbor type of each slot (MODE > 0)
-    T *vch = gcl + ncoef + (size_t)wid * RCH * a.nb;
-    int *tch = reinterpret_cast<int *>(gcl + ncoef + (size_t)NEP_WAVES * RCH * a.nb) + wid * RCH;
+    T *vch = gcl + (size_t)ntab * ncoef + (size_t)wid * RCH * a.nb;
+    int *tch = reinterpret_cast<int *>(gcl + (size_t)ntab * ncoef + (size_t)NEP_WAVES * RCH * a.nb) + wid * RCH;
     if (MODE > 0) {
-        for (int x = threadIdx.x; x < ncoef; x += blockDim.x) gcl[x] = T(0);
+        for (int x = threadIdx.x; x < ntab * ncoef; x += blockDim.x) gcl[x] = T(0);
         __syncthreads();
     }
     const T *c = reinterpret_cast<const T *>(a.c);
@@ -493,10 +500,14 @@ nep_radial_kernel(FeatArgs a) {
                     for (int kk = 0; kk < nk; kk++)
                         if (tch[kk] == tj) sv += vch[kk * a.nb + q];
                     if (sv != T(0)) {
-                        T *gcp = gcl + ((ti * a.T + tj) * a.nm) * a.nb + q;
+                        T *gcp = gtab + ((ti * a.T + tj) * a.nm) * a.nb + q;
+                        // lanes own distinct (tj, q): a wave's own table needs no atomics
 #pragma unroll
                         for (int n = 0; n < NM_MAX; n++)
-                            if (n < a.nm) atomicAdd(gcp + n * a.nb, g[n] * sv);
+                            if (n < a.nm) {
+                                if (ntab > 1) gcp[n * a.nb] += g[n] * sv;
+                                else atomicAdd(gcp + n * a.nb, g[n] * sv);
+                            }
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
@@ -513,7 +524,7 @@ nep_radial_kernel(FeatArgs a) {
             }
         }
     }
-    if (MODE > 0) block_flush_add(gcl, reinterpret_cast<T *>(a.gc), ncoef);
+    if (MODE > 0) block_flush_add(gcl, reinterpret_cast<T *>(a.gc), ncoef, ntab);
 }
 
 // LDS elements of one angular wave: per-atom arrays A, B, C of nm*24, and the
@@ -523,9 +534,9 @@ nep_radial_kernel(FeatArgs a) {
 template <int MODE>
 __host__ __device__ inline int ang_chunk_elems(int nm, int nb) {
     constexpr int CH = ang_ch<MODE>();
-    if (MODE == 0) return CH * (nm + NLM);
+    if (MODE == 0) return CH * (nm + WS);
     if (MODE == 1) return CH * (nm + nb);
-    return CH * (2 * nm + 2 * NLM) + CH * (2 * nm + 2 * nb);
+    return CH * (2 * nm + 2 * WS) + CH * (2 * nm + 2 * nb);
 }
 template <int MODE>
 __host__ __device__ inline int ang_wave_elems(int nm, int nb) { return 3 * nm * NLM + ang_chunk_elems<MODE>(nm, nb); }
@@ -540,20 +551,22 @@ nep_angular_kernel(FeatArgs a) {
     const int ncoef = a.T * a.T * a.nm * a.nb;
     const int per = a.nm * NLM;
     const int nm = a.nm, nb = a.nb;
-    T *gcl = reinterpret_cast<T *>(smem_raw);                                  // [ncoef]
-    T *wv = gcl + ncoef + (size_t)wid * ang_wave_elems<MODE>(nm, nb);
+    const int ntab = (MODE > 0 && a.pw) ? ANG_WAVES : 1;
+    T *gcl = reinterpret_cast<T *>(smem_raw);                                  // [ntab][ncoef]
+    T *gtab = gcl + (ntab > 1 ? wid : 0) * ncoef;                             // this wave's table
+    T *wv = gcl + (size_t)ntab * ncoef + (size_t)wid * ang_wave_elems<MODE>(nm, nb);
     T *A = wv, *B = wv + per, *C = wv + 2 * per;                               // s, D, P / E
     T *X = wv + 3 * per;                                                       // slot chunk
-    int *tch = reinterpret_cast<int *>(gcl + ncoef + (size_t)ANG_WAVES * ang_wave_elems<MODE>(nm, nb)) +
+    int *tch = reinterpret_cast<int *>(gcl + (size_t)ntab * ncoef + (size_t)ANG_WAVES * ang_wave_elems<MODE>(nm, nb)) +
                wid * ACH;
     // chunk views
     T *u1 = X, *u2 = X + CH * nm;                          // MODE 0/1: u = u1; MODE 2: gnp, gn
     T *w1 = X + (MODE == 2 ? 2 : 1) * CH * nm;             // MODE 0: b; MODE 2: gg0 b
-    T *w2 = w1 + CH * NLM;                                 // MODE 2: gg . db
-    T *z1 = X + 2 * CH * nm + 2 * CH * NLM, *z2 = z1 + CH * nm;   // MODE 2: B beta, B delta
+    T *w2 = w1 + CH * WS;                                  // MODE 2: gg . db
+    T *z1 = X + 2 * CH * nm + 2 * CH * WS, *z2 = z1 + CH * nm;    // MODE 2: B beta, B delta
     T *v1 = MODE == 1 ? X + CH * nm : z2 + CH * nm, *v2 = v1 + CH * nb;
     if (MODE > 0) {
-        for (int x = threadIdx.x; x < ncoef; x += blockDim.x) gcl[x] = T(0);
+        for (int x = threadIdx.x; x < ntab * ncoef; x += blockDim.x) gcl[x] = T(0);
         __syncthreads();
     }
     const T *c = reinterpret_cast<const T *>(a.c);
@@ -569,7 +582,9 @@ nep_angular_kernel(FeatArgs a) {
             for (int kk = 0; kk < nk; kk++) {
                 T d = p1[kk * nm + n] * q1[kk * nb + q];
                 if (p2) d += p2[kk * nm + n] * q2[kk * nb + q];
-                atomicAdd(gcl + (ti * a.T + tch[kk]) * nmb + o, d);
+                // lanes own distinct o: a wave's own table needs no atomics
+                if (ntab > 1) gtab[(ti * a.T + tch[kk]) * nmb + o] += d;
+                else atomicAdd(gtab + (ti * a.T + tch[kk]) * nmb + o, d);
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -632,7 +647,7 @@ nep_angular_kernel(FeatArgs a) {
 #pragma unroll
                     for (int n = 0; n < NM_MAX; n++)
                         if (n < nm) u1[lane * nm + n] = gn[n];
-                    blm_each<T>(ux, uy, uz, [&](int m, T bv, T, T, T) { w1[lane * NLM + m] = bv; });
+                    blm_each<T>(ux, uy, uz, [&](int m, T bv, T, T, T) { w1[lane * WS + m] = bv; });
                 } else if (MODE == 1) {
                     // gRi = sum_nm D_nm d(g_n b_m)/dRi; dc[n][q] = f_q (D b)_n
                     radial_g(r, cij, gn, gnp, v1 + lane * nb, nullptr);
@@ -684,8 +699,8 @@ nep_angular_kernel(FeatArgs a) {
                     });
 #pragma unroll
                     for (int m = 0; m < NLM; m++) {
-                        w1[lane * NLM + m] = be[m];
-                        w2[lane * NLM + m] = de[m];
+                        w1[lane * WS + m] = be[m];
+                        w2[lane * WS + m] = de[m];
                     }
 #pragma unroll
                     for (int n = 0; n < NM_MAX; n++)
@@ -713,8 +728,8 @@ nep_angular_kernel(FeatArgs a) {
                         const int n = o / NLM, m = o - n * NLM;
                         T sacc = T(0);
                         for (int kk = 0; kk < nk; kk++) {
-                            sacc += u1[kk * nm + n] * w1[kk * NLM + m];
-                            if (MODE == 2) sacc += u2[kk * nm + n] * w2[kk * NLM + m];
+                            sacc += u1[kk * nm + n] * w1[kk * WS + m];
+                            if (MODE == 2) sacc += u2[kk * nm + n] * w2[kk * WS + m];
                         }
                         red[j] += sacc;
                     }
@@ -791,7 +806,7 @@ nep_angular_kernel(FeatArgs a) {
         }
         __builtin_amdgcn_wave_barrier();
     }
-    if (MODE > 0) block_flush_add(gcl, reinterpret_cast<T *>(a.gc), ncoef);
+    if (MODE > 0) block_flush_add(gcl, reinterpret_cast<T *>(a.gc), ncoef, ntab);
 }
 
 // ------------------------------------------------------------------ force / virial ops
@@ -955,14 +970,23 @@ int feat_launch(FeatArgs &a, int angular, int mode, hipStream_t st) {
     MATPL_CHECK_ARG(!angular || a.l5 == 0 || a.l4 > 0, "nep_feat: the 5-body term needs the 4-body term");
     if (a.N == 0) return OK;
     const size_t ncoef = (size_t)a.T * a.T * a.nm * a.nb;
-    size_t lds = 0;
-    if (!angular && mode > 0)
-        lds = (ncoef + (size_t)NEP_WAVES * RCH * a.nb) * sizeof(T) + (size_t)NEP_WAVES * RCH * sizeof(int);
-    if (angular) {
+    // LDS of the block for ntab coefficient-gradient tables
+    auto lds_of = [&](size_t ntab) -> size_t {
+        if (!angular)
+            return mode > 0 ? (ntab * ncoef + (size_t)NEP_WAVES * RCH * a.nb) * sizeof(T) +
+                                  (size_t)NEP_WAVES * RCH * sizeof(int)
+                            : 0;
         const int we = mode == 0 ? ang_wave_elems<0>(a.nm, a.nb)
                      : mode == 1 ? ang_wave_elems<1>(a.nm, a.nb) : ang_wave_elems<2>(a.nm, a.nb);
-        lds = (ncoef + (size_t)ANG_WAVES * we) * sizeof(T) + (size_t)ANG_WAVES * ACH * sizeof(int);
-    }
+        return (ntab * ncoef + (size_t)ANG_WAVES * we) * sizeof(T) + (size_t)ANG_WAVES * ACH * sizeof(int);
+    };
+    // float: a table per wavefront (plain adds) when they fit, else one per block (LDS atomics).
+    // Measured on the NEP training batch: float LDS atomic adds keep the LDS busy 3-4x longer than
+    // double ones for the same instruction count (the per-wave tables take the float step from
+    // 9.46 to 9.05 ms), while for double the larger tables cost occupancy (10.3 -> 11.8 ms).
+    const size_t nw = angular ? ANG_WAVES : NEP_WAVES;
+    a.pw = (mode > 0 && sizeof(T) == 4 && lds_of(nw) <= 64 * 1024) ? 1 : 0;
+    const size_t lds = lds_of(a.pw ? nw : 1);
     MATPL_CHECK_ARG(lds <= 64 * 1024, "nep_feat: LDS %zu too large", lds);
     const int waves = angular ? ANG_WAVES : NEP_WAVES;
     int64_t blocks = (a.N + waves - 1) / waves;
