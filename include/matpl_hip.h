@@ -389,6 +389,22 @@ int matpl_dp_fit_train(const float *A, const float *At, const float *gE, const i
                        int64_t type_stride_vec, int E, int nlayers, int bf16, float *Ahat, float *xw0, float *zw0,
                        float *xh, float *zh, float *rb, float *xo, float *tz, void *stream);
 
+/*
+ * Y[t] = X_t^T Z_t [M][N] (float, written) for tall operands: the training
+ * step's fitting-weight gradients, sums over each type's atoms of products of
+ * per-atom rows (the reference leaves them to autograd's GEMMs,
+ * src/model/dp_embedding.py:96-163 through loss.backward()). Type t's rows are
+ * r in [2 start[t], 2 start[t+1]) (start: host array of ntypes + 1 entries,
+ * ntypes <= 16); row r of X is at X + (r / 2) * ldx2 + (r % 2) * ldx1 (M
+ * values), of Z likewise (N <= 64 values). work: matpl_tall_xtz_work(
+ * max atoms of a type, ntypes, M, N) floats. Deterministic (per-chunk partial
+ * products added in chunk order). No reference counterpart.
+ */
+int64_t matpl_tall_xtz_work(int64_t max_atoms, int ntypes, int M, int N);
+int matpl_tall_xtz(const float *X, int64_t ldx2, int64_t ldx1, const float *Z, int64_t ldz2, int64_t ldz1,
+                   const int64_t *start, int ntypes, int M, int N, float *Y, float *work, int64_t work_elems,
+                   void *stream);
+
 /* ------------------------------------------------------------ DP compress */
 /*
  * calculateCompress: G[s][i] = polynomial of degree coe_num-1 in f2[s] with

@@ -309,6 +309,15 @@ class DPTrainFused(torch.nn.Module):
         return (_ptr(self.tinfo), b.natoms, cfg.ntypes, mnl, cfg.m, self.E, int(self.bf16), float(cfg.rcut),
                 float(cfg.rcut_smooth), 1.0 / (cfg.m * cfg.ntypes))
 
+    def _listed_any(self, b: FusedBatch):
+        """Per neighbor type: whether any frame of the batch lists it (cached per batch, host side)."""
+        hit = getattr(self, "_listed_cache", None)
+        if hit is not None and hit[0] is b:
+            return hit[1]
+        val = b.type_present.any(0).cpu().tolist()
+        self._listed_cache = (b, val)
+        return val
+
     def _fit_tiles(self, b: FusedBatch):
         """perm (atoms grouped by type, int32), tiles of <= 32 positions of one type, and each type's
         position range, for the fitting kernels (cached per batch)."""
@@ -351,24 +360,38 @@ class DPTrainFused(torch.nn.Module):
                   _ptr(w0f), _ptr(w0b), _ptr(whf), _ptr(whb), _ptr(vec), *strides, E, L, int(self.bf16), _ptr(ahat),
                   _ptr(xw0), _ptr(zw0), _ptr(xh), _ptr(zh), _ptr(rb), _ptr(xo), _ptr(tz), _stream())
         gE_pos = gE[perm.to(torch.int64)]
+        # weight gradients: per type the products over its atoms' (value, tangent) rows, all types
+        # in one launch per layer (matpl_tall_xtz)
+        T = len(self.fit)
+        start = np.asarray([r[0] for r in ranges] + [ranges[-1][1]], np.int64)
+        maxa = int(max(e - a for a, e in ranges))
+        work = torch.empty(max(_lib.query("matpl_tall_xtz_work", maxa, T, D, 64), 1), dtype=f32, device=dev)
+        Y0 = torch.empty((T, D, 64), dtype=f32, device=dev)
+        _lib.call("matpl_tall_xtz", _ptr(xw0), 2 * D, D, _ptr(zw0), 128, 64, start.ctypes.data, T, D, 64, _ptr(Y0),
+                  _ptr(work), work.numel(), _stream())
+        Yh = []
+        for l in range(1, L):
+            Y = torch.empty((T, 64, 64), dtype=f32, device=dev)
+            xp = xh.data_ptr() + (l - 1) * 128 * 4
+            zp = zh.data_ptr() + (l - 1) * 128 * 4
+            _lib.call("matpl_tall_xtz", xp, (L - 1) * 128, 64, zp, (L - 1) * 128, 64, start.ctypes.data, T, 64, 64,
+                      _ptr(Y), _ptr(work), work.numel(), _stream())
+            Yh.append(Y)
         out = []
         for t, net in enumerate(self.fit):
             a, e = ranges[t]
             if e == a:
                 out += [None] * (3 * len(net.W))
                 continue
-            # products over the type's positions on whole (64-wide, contiguous) rows; sums of the
-            # bias, resnet_dt and output rows one reduction per buffer
-            gW = [(xw0[a:e].reshape(-1, D).t() @ zw0[a:e].reshape(-1, 64))[:, :H]]
+            # sums of the bias, resnet_dt and output rows: one reduction per buffer
+            gW = [Y0[t, :, :H]]
             zs = zw0[a:e].sum(0)
             gb = [zs[0, :H]]
             gr = [None]
             if L > 1:
                 zhs, rbs = zh[a:e].sum(0), rb[a:e].sum(0)
                 for l in range(1, L):
-                    X = xh[a:e, l - 1].reshape(-1, 64)
-                    Z = zh[a:e, l - 1].reshape(-1, 64)
-                    gW.append((X.t() @ Z)[:H, :H])
+                    gW.append(Yh[l - 1][t, :H, :H])
                     gb.append(zhs[l - 1, 0, :H])
                     gr.append(rbs[l - 1, :H] if net.has_rdt[l] else None)
             gW.append(xo[a:e, :H].sum(0).reshape(H, 1))
@@ -424,10 +447,12 @@ class DPTrainFused(torch.nn.Module):
         chunk = self.chunk or max(8, -(-N * T // 8192))
         _lib.call("matpl_dp_train_embgrad", *head, _ptr(vec), _ptr(mat), *tail, _ptr(resid), _ptr(abar), _ptr(ahat),
                   _ptr(grad), chunk, _ptr(status), _stream())
-        if int(status.item()) != 0:
-            raise OverflowError("more than maxNeighborNum neighbors of one type within rcut")
         gemb = self._emb_grads(grad, b, ahat, nin)
         loss = (self.pe * le + self.pf * lf).detach()
+        # the overflow flag is read once everything is enqueued, so that the host does not stall the
+        # step's launches on it
+        if int(status.item()) != 0:
+            raise OverflowError("more than maxNeighborNum neighbors of one type within rcut")
         return loss, gemb + gfit
 
     def _loss_and_grads_kernels(self, b: FusedBatch):
@@ -480,10 +505,12 @@ class DPTrainFused(torch.nn.Module):
         chunk = self.chunk or max(8, -(-N * T // 8192))
         _lib.call("matpl_dp_train_embgrad", *head, _ptr(vec), _ptr(mat), *tail, _ptr(resid), _ptr(abar), _ptr(ahat),
                   _ptr(grad), chunk, _ptr(status), _stream())
-        if int(status.item()) != 0:
-            raise OverflowError("more than maxNeighborNum neighbors of one type within rcut")
         gemb = self._emb_grads(grad, b, ahat, nin)
         loss = (self.pe * le + self.pf * lf).detach()
+        # the overflow flag is read once everything is enqueued, so that the host does not stall the
+        # step's launches on it
+        if int(status.item()) != 0:
+            raise OverflowError("more than maxNeighborNum neighbors of one type within rcut")
         return loss, gemb + gfit
 
     def _emb_grads(self, grad, b: FusedBatch, ahat, nin):
@@ -503,7 +530,7 @@ class DPTrainFused(torch.nn.Module):
         u = torch.stack([wp[a:e].sum(0) if e > a else torch.zeros((T, E), dtype=wsum.dtype, device=wsum.device)
                          for a, e in ranges])
         has_atoms = [bool(idx.numel()) for idx in b.by_type]
-        listed_any = b.type_present.any(0).cpu().tolist()
+        listed_any = self._listed_any(b)
         # the empty-slot term's gradient, all pairs at once
         with torch.enable_grad():
             leaves = [x.detach().requires_grad_() for x in self._stacked()]
