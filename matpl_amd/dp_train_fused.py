@@ -131,62 +131,88 @@ class _EmbPack:
 
 class _FitPack:
     """The fitting nets' parameters (torch, on the device) in the matrix-core fitting kernels' operand
-    layouts (matpl_amd.dp_fitting.FittingMFMA's packing, built on the device every step)."""
+    layouts (matpl_amd.dp_fitting.FittingMFMA's packing), built on the device every step: the
+    parameters are concatenated into one vector (plus a 0 and a 1) and each packed array is one
+    gather from it through an index map fixed at construction."""
 
-    def __init__(self, E: int, H: int, L: int, bf16: bool, device):
+    def __init__(self, E: int, H: int, L: int, T: int, has_rdt, bf16: bool, device):
         from .dp_fitting import _lane_maps
-        self.E, self.H, self.L, self.bf16 = E, H, L, bf16
+        self.E, self.H, self.L, self.T, self.bf16 = E, H, L, T, bf16
         D = 16 * E
         self.D, self.Dh, self.nb0 = D, D // 2, (D + 31) // 32
         hh, ii, fin = _lane_maps()
         kk = np.arange(self.Dh)
-        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.int64, device=device)
-        if bf16:
-            k = 16 * (kk[None, :] // 8) + 8 * hh[:, None] + (kk[None, :] % 8)
-        else:
-            k = 2 * kk[None, :] + hh[:, None]
-        self.k0 = t(k)                                                         # [64, Dh] DR index
-        self.out = t(np.stack([32 * ob + ii[:, None] for ob in range(2)]))     # [2, 64, 1]
-        self.rows = t(np.stack([32 * blk + ii[:, None] for blk in range(self.nb0)]))   # [nb0, 64, 1]
-        self.fin = t(fin)                                                      # [64, 32]
-        f = np.array([[[32 * blk + feat(rr, h) for rr in range(16)] for h in range(2)] for blk in range(2)])
-        self.vf = t(f)                                                         # [2 blk, 2 h, 16]
-        self.device = device
+        # offsets of each type's parameters in the concatenation, in the order of _flat_params
+        self.has_rdt = list(has_rdt)
+        sizes = [D * H, H] + [H * H] * (L - 1) + [H] * (L - 1) + [H * int(r) for r in self.has_rdt[1:L]] + [H, 1]
+        per = int(sum(sizes))
+        base = np.arange(T)[:, None, None, None] * per
+        off = np.cumsum([0] + sizes)
+        o_W0, o_b0 = off[0], off[1]
+        o_Wh = [off[2 + (l - 1)] for l in range(1, L)]
+        o_bh = [off[2 + (L - 1) + (l - 1)] for l in range(1, L)]
+        o_rh = [off[2 + 2 * (L - 1) + (l - 1)] for l in range(1, L)]
+        o_wo, o_bo = off[-3], off[-2]
+        ZERO, ONE = T * per, T * per + 1
+        self.nflat = T * per + 2
+
+        def pick(valid, idx):
+            return np.where(valid, idx, ZERO)
+        k = (16 * (kk[None, :] // 8) + 8 * hh[:, None] + (kk[None, :] % 8)) if bf16 else (2 * kk[None, :] + hh[:, None])
+        out = np.stack([32 * ob + ii[:, None] for ob in range(2)])                 # [2, 64, 1]
+        w0f = pick((k[None] < D) & (out < H), base + o_W0 + k[None] * H + out)
+        rows = np.stack([32 * blk + ii[:, None] for blk in range(self.nb0)])       # [nb0, 64, 1]
+        w0b = pick((rows < D) & (fin[None] < H), base + o_W0 + rows * H + fin[None])
+        self.i_w0f = torch.as_tensor(w0f.reshape(T, 2, 64, self.Dh), device=device)
+        self.i_w0b = torch.as_tensor(w0b.reshape(T, self.nb0, 64, 32), device=device)
+        nh = max(L - 1, 1)
+        whf = np.full((T, nh, 2, 64, 32), ZERO, np.int64)
+        whb = np.full((T, nh, 2, 64, 32), ZERO, np.int64)
+        tb = np.arange(T)[:, None, None, None] * per
+        for l in range(1, L):
+            o = o_Wh[l - 1]
+            whf[:, l - 1] = pick((fin[None] < H) & (out < H), tb + o + fin[None] * H + out)
+            whb[:, l - 1] = pick((out < H) & (fin[None] < H), tb + o + out * H + fin[None])
+        self.i_whf = torch.as_tensor(whf, device=device)
+        self.i_whb = torch.as_tensor(whb, device=device)
+        nvec = L * 3 * 2 * 2 * 16 + 2 * 2 * 16 + 16
+        vec = np.full((T, nvec), ZERO, np.int64)
+        f = np.array([[[32 * blk + feat(rr, h) for rr in range(16)] for h in range(2)] for blk in range(2)])   # [2,2,16]
+        ok = f < H
+        tt = np.arange(T)[:, None, None, None] * per
+        v = vec[:, :L * 3 * 64].reshape(T, L, 3, 2, 2, 16)
+        for l in range(L):
+            ob_ = o_b0 if l == 0 else o_bh[l - 1]
+            v[:, l, 0] = np.where(ok[None], tt + ob_ + f[None], ZERO)
+            if l >= 1 and self.has_rdt[l]:
+                v[:, l, 1] = np.where(ok[None], tt + o_rh[l - 1] + f[None], ZERO)
+            else:
+                v[:, l, 1] = np.broadcast_to(np.where(ok, ONE, ZERO), (T, 2, 2, 16))
+        vec[:, L * 3 * 64:L * 3 * 64 + 64] = np.where(ok[None], tt + o_wo + f[None], ZERO).reshape(T, 64)
+        vec[:, L * 3 * 64 + 64] = np.arange(T) * per + o_bo
+        self.i_vec = torch.as_tensor(vec, device=device)
+        self.const = torch.tensor([0.0, 1.0], dtype=torch.float32, device=device)
+        self.strides = (2 * 64 * self.Dh, self.nb0 * 64 * 32, nh * 2 * 64 * 32, nvec)
+
+    def _flat_params(self, nets):
+        ps = []
+        L = self.L
+        for n in nets:
+            ps += [n.W[0].reshape(-1), n.b[0]]
+            ps += [n.W[l].reshape(-1) for l in range(1, L)]
+            ps += [n.b[l] for l in range(1, L)]
+            ps += [n.r[l] for l in range(1, L) if self.has_rdt[l]]
+            ps += [n.W[L].reshape(-1), n.b[L].reshape(-1)]
+        return ps
 
     def __call__(self, nets):
-        """(w0f, w0b, whf, whb, vec, strides) for a list of per-type _Net fitting nets."""
-        E, H, L, D = self.E, self.H, self.L, self.D
-        T = len(nets)
-        dev = self.device
-        W0 = torch.zeros((T, self.nb0 * 32, 64), dtype=torch.float32, device=dev)
-        W0[:, :D, :H] = torch.stack([n.W[0] for n in nets])
-        w0f = W0[:, self.k0[None], self.out]                                   # [T, 2, 64, Dh]
-        w0b = W0[:, self.rows, self.fin[None]]                                 # [T, nb0, 64, 32]
-        if L > 1:
-            Wh = torch.zeros((T, L - 1, 64, 64), dtype=torch.float32, device=dev)
-            Wh[:, :, :H, :H] = torch.stack([torch.stack([n.W[l] for l in range(1, L)]) for n in nets])
-            whf = Wh[:, :, self.fin[None], self.out]                           # [T, L-1, 2, 64, 32]
-            whb = Wh[:, :, self.out, self.fin[None]]
-        else:
-            whf = whb = torch.zeros((T, 1, 2, 64, 32), dtype=torch.float32, device=dev)
-        b = torch.zeros((T, L, 64), dtype=torch.float32, device=dev)
-        r = torch.zeros((T, L, 64), dtype=torch.float32, device=dev)
-        b[:, :, :H] = torch.stack([torch.stack([n.b[l] for l in range(L)]) for n in nets])
-        r[:, :, :H] = torch.stack([torch.stack([n.r[l] if n.has_rdt[l] else torch.ones_like(n.b[l])
-                                                for l in range(L)]) for n in nets])
-        wo = torch.zeros((T, 64), dtype=torch.float32, device=dev)
-        wo[:, :H] = torch.stack([n.W[L][:, 0] for n in nets])
-        bo = torch.stack([n.b[L].reshape(()) for n in nets])
-        vec = torch.zeros((T, L * 3 * 2 * 2 * 16 + 2 * 2 * 16 + 16), dtype=torch.float32, device=dev)
-        v = vec[:, :L * 3 * 64].view(T, L, 3, 2, 2, 16)
-        v[:, :, 0] = b[:, :, self.vf]
-        v[:, :, 1] = r[:, :, self.vf]
-        vec[:, L * 3 * 64:L * 3 * 64 + 64] = wo[:, self.vf].reshape(T, 64)
-        vec[:, L * 3 * 64 + 64] = bo
-        dt = torch.bfloat16 if self.bf16 else torch.float32
-        w0f, w0b, whf, whb = (x.to(dt).contiguous() for x in (w0f, w0b, whf, whb))
-        strides = (w0f[0].numel(), w0b[0].numel(), whf[0].numel(), vec[0].numel())
-        return w0f, w0b, whf, whb, vec.contiguous(), strides
+        """(w0f, w0b, whf, whb, vec, strides) for the list of per-type _Net fitting nets."""
+        with torch.no_grad():
+            flat = torch.cat(self._flat_params(nets) + [self.const])
+            dt = torch.bfloat16 if self.bf16 else torch.float32
+            w0f, w0b, whf, whb = (flat[i].to(dt) for i in (self.i_w0f, self.i_w0b, self.i_whf, self.i_whb))
+            vec = flat[self.i_vec]
+        return w0f, w0b, whf, whb, vec, self.strides
 
 
 class DPTrainFused(torch.nn.Module):
@@ -239,7 +265,7 @@ class DPTrainFused(torch.nn.Module):
             raise ValueError("the fitting kernels need M2 = 16, 1..4 hidden layers of one width <= 64, biases")
         self.fitting = fitting
         self.H, self.L = H, L
-        self._fp = _FitPack(self.E, H, L, self.bf16, dev) if fitting == "kernel" else None
+        self._fp = _FitPack(self.E, H, L, T, self.fit[0].has_rdt, self.bf16, dev) if fitting == "kernel" else None
         self._tiles = {}
 
     # ---------------------------------------------------------------- weights
@@ -521,7 +547,7 @@ class DPTrainFused(torch.nn.Module):
         c = 1.0 / (m * listed.sum(1))                                                   # [N]
         npad = (m - nin.to(torch.float32)) * listed                                     # [N, T]
         ti = b.type_map.to(torch.int64)
-        proj = torch.einsum("nwe,nw->ne", ahat, self.ri_pad[ti])                        # [N, E]
+        proj = (ahat * self.ri_pad[ti][:, :, None]).sum(1)                                 # [N, E]
         wsum = (c[:, None] * npad)[:, :, None] * proj[:, None, :]                        # [N, T, E]
         # per centre type sums over its atoms (one gather into type order, then contiguous sums; an
         # index_add onto T rows would serialize on atomics)

@@ -17,3 +17,29 @@ def test_training_weight_packing_equals_host_packing():
             vec, mat = DPTrainFused(w, prec, "cpu")._pack()
             assert torch.equal(mat.float(), ref.mat.float())
             assert torch.allclose(vec, ref.vec, rtol=0, atol=2e-6)
+
+
+def test_fitting_weight_packing_equals_host_packing():
+    """The training step's device-side packing of the fitting nets (one gather from the concatenated
+    parameters) builds the matrix-core fitting kernels' operands exactly as the host packing
+    (matpl_amd.dp_fitting.FittingMFMA)."""
+    import numpy as np
+    import torch
+    from matpl_amd.dp import _Net
+    from matpl_amd.dp_fitting import FittingMFMA
+    from matpl_amd.dp_train_fused import _FitPack
+    from matpl_amd.dpnet import DPConfig, DPWeights
+    rng = np.random.default_rng(1)
+    for T in (1, 2, 4):
+        cfg = DPConfig(ntypes=T, m=100)
+        w = DPWeights.random(cfg, seed=3, davg=rng.uniform(0.1, 0.2, (T, 4)), dstd=rng.uniform(0.5, 1.0, (T, 4)))
+        nets = [_Net(w.fit[t], 1, torch.float32, "cpu") for t in range(T)]
+        H, L = w.fit[0][0][0].shape[1], len(w.fit[0]) - 1
+        for prec in ("fp32", "bf16"):
+            ref = FittingMFMA(w, [(0, 0, 1)], prec, "cpu")
+            w0f, w0b, whf, whb, vec, strides = _FitPack(cfg.emb_out, H, L, T, nets[0].has_rdt, prec == "bf16",
+                                                        "cpu")(nets)
+            for a, b in ((w0f, ref.w0f), (w0b, ref.w0b), (whf, ref.whf), (whb, ref.whb)):
+                assert a.shape == b.shape and torch.equal(a.float(), b.float())
+            assert torch.allclose(vec, ref.vec, rtol=0, atol=1e-7)
+            assert tuple(strides) == tuple(ref.strides)
