@@ -405,6 +405,22 @@ int matpl_tall_xtz(const float *X, int64_t ldx2, int64_t ldx1, const float *Z, i
                    const int64_t *start, int ntypes, int M, int N, float *Y, float *work, int64_t work_elems,
                    void *stream);
 
+/*
+ * Column sums of per-type row ranges over up to 8 buffers in one launch: the
+ * training step's bias, resnet_dt and output-layer gradients (the reference
+ * leaves them to autograd, src/model/dp_embedding.py:96-163, FittingNet, through loss.backward()).
+ * For buffer s (ptrs, lds, cols: host arrays of nseg entries; ptrs holds
+ * device addresses of float buffers) and type t,
+ * out[t][off_s + j] = sum_{p = start[t]}^{start[t+1]-1} ptrs[s][p * lds[s] + j],
+ * j < cols[s], off_s = cols[0] + ... + cols[s-1]; out is [ntypes][sum cols].
+ * start: host array of ntypes + 1 entries (ntypes <= 16). work:
+ * matpl_seg_rowsum_work(max positions of a type, ntypes, sum cols) floats.
+ * Deterministic (fixed summation order). No reference counterpart.
+ */
+int64_t matpl_seg_rowsum_work(int64_t max_atoms, int ntypes, int total_cols);
+int matpl_seg_rowsum(int nseg, const uint64_t *ptrs, const int64_t *lds, const int32_t *cols, const int64_t *start,
+                     int ntypes, float *out, float *work, int64_t work_elems, void *stream);
+
 /* ------------------------------------------------------------ DP compress */
 /*
  * calculateCompress: G[s][i] = polynomial of degree coe_num-1 in f2[s] with

@@ -65,6 +65,8 @@ SIGNATURES = {
     "matpl_dp_fit_train": (I32, [P, P, P, P, P, I32, P, P, P, P, P, I64, I64, I64, I64, I32, I32, I32] + [P] * 9),
     "matpl_tall_xtz_work": (I64, [I64, I32, I32, I32]),
     "matpl_tall_xtz": (I32, [P, I64, I64, P, I64, I64, P, I32, I32, I32, P, P, I64, P]),
+    "matpl_seg_rowsum_work": (I64, [I64, I32, I32]),
+    "matpl_seg_rowsum": (I32, [I32, P, P, P, P, I32, P, P, I64, P]),
     # DP compress
     "matpl_dp_compress_f32": (I32, [P, P, I64, I32, I32, P, P]),
     "matpl_dp_compress_f64": (I32, [P, P, I64, I32, I32, P, P]),

@@ -403,25 +403,37 @@ class DPTrainFused(torch.nn.Module):
             _lib.call("matpl_tall_xtz", xp, (L - 1) * 128, 64, zp, (L - 1) * 128, 64, start.ctypes.data, T, 64, 64,
                       _ptr(Y), _ptr(work), work.numel(), _stream())
             Yh.append(Y)
+        # sums of the bias, resnet_dt and output rows over each type's positions, one launch:
+        # zw0 value rows, zh value rows per layer, rb per layer, xo, gE
+        segs = [(zw0.data_ptr(), 128, H)]
+        segs += [(zh.data_ptr() + (l - 1) * 128 * 4, (L - 1) * 128, H) for l in range(1, L)]
+        segs += [(rb.data_ptr() + (l - 1) * 64 * 4, (L - 1) * 64, H) for l in range(1, L)]
+        segs += [(xo.data_ptr(), 64, H), (gE_pos.data_ptr(), 1, 1)]
+        sptr = np.asarray([x[0] for x in segs], np.uint64)
+        sld = np.asarray([x[1] for x in segs], np.int64)
+        scol = np.asarray([x[2] for x in segs], np.int32)
+        totc = int(scol.sum())
+        swork = torch.empty(max(_lib.query("matpl_seg_rowsum_work", maxa, T, totc), 1), dtype=f32, device=dev)
+        sums = torch.empty((T, totc), dtype=f32, device=dev)
+        _lib.call("matpl_seg_rowsum", len(segs), sptr.ctypes.data, sld.ctypes.data, scol.ctypes.data,
+                  start.ctypes.data, T, _ptr(sums), _ptr(swork), swork.numel(), _stream())
+        cs = np.concatenate([[0], np.cumsum(scol)])
+        col = lambda k: sums[:, int(cs[k]):int(cs[k + 1])]
         out = []
         for t, net in enumerate(self.fit):
             a, e = ranges[t]
             if e == a:
                 out += [None] * (3 * len(net.W))
                 continue
-            # sums of the bias, resnet_dt and output rows: one reduction per buffer
             gW = [Y0[t, :, :H]]
-            zs = zw0[a:e].sum(0)
-            gb = [zs[0, :H]]
+            gb = [col(0)[t]]
             gr = [None]
-            if L > 1:
-                zhs, rbs = zh[a:e].sum(0), rb[a:e].sum(0)
-                for l in range(1, L):
-                    gW.append(Yh[l - 1][t, :H, :H])
-                    gb.append(zhs[l - 1, 0, :H])
-                    gr.append(rbs[l - 1, :H] if net.has_rdt[l] else None)
-            gW.append(xo[a:e, :H].sum(0).reshape(H, 1))
-            gb.append(gE_pos[a:e].sum().reshape(1))
+            for l in range(1, L):
+                gW.append(Yh[l - 1][t, :H, :H])
+                gb.append(col(l)[t])
+                gr.append(col(L - 1 + l)[t] if net.has_rdt[l] else None)
+            gW.append(col(2 * L - 1)[t].reshape(H, 1))
+            gb.append(col(2 * L)[t].reshape(1))
             gr.append(None)
             out += gW + gb + gr
         return ahat, out

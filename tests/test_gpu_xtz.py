@@ -47,3 +47,34 @@ def test_tall_xtz_matches_float64(dev, M, N, counts):
         bound = Xt.abs().t() @ Zt.abs()
         err = (Y[t].double() - want).abs()
         assert bool((err <= 1e-5 * bound + 1e-30).all()), (t, float((err / (bound + 1e-30)).max()))
+
+
+@pytest.mark.parametrize("counts", [[2500, 0, 731, 1], [3000], [0, 5, 1030]])
+def test_seg_rowsum_matches_float64(dev, counts):
+    """matpl_seg_rowsum: per-type column sums of several strided buffers in one launch."""
+    rng = np.random.default_rng(len(counts))
+    npos = sum(counts)
+    shapes = [(128, 50), (256, 50), (64, 64), (1, 1), (70, 3)]       # (ld, cols) per buffer
+    bufs = [torch.tensor(rng.normal(size=max(npos, 1) * ld), dtype=torch.float32, device=dev) for ld, _ in shapes]
+    start = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    T = len(counts)
+    ptrs = np.asarray([b.data_ptr() for b in bufs], np.uint64)
+    lds = np.asarray([ld for ld, _ in shapes], np.int64)
+    cols = np.asarray([c for _, c in shapes], np.int32)
+    totc = int(cols.sum())
+    work = torch.empty(_lib.query("matpl_seg_rowsum_work", max(counts), T, totc), dtype=torch.float32, device=dev)
+    out = torch.full((T, totc), float("nan"), dtype=torch.float32, device=dev)
+    print(f"case seg_rowsum counts={counts}", flush=True)
+    _lib.call("matpl_seg_rowsum", len(bufs), ptrs.ctypes.data, lds.ctypes.data, cols.ctypes.data, start.ctypes.data,
+              T, _ptr(out), _ptr(work), work.numel(), _stream())
+    torch.cuda.synchronize()
+    off = 0
+    for b, (ld, c) in zip(bufs, shapes):
+        rows = b[:max(npos, 1) * ld].double().reshape(-1, ld)[:, :c]
+        for t in range(T):
+            a, e = int(start[t]), int(start[t + 1])
+            want = rows[a:e].sum(0)
+            bound = rows[a:e].abs().sum(0)
+            err = (out[t, off:off + c].double() - want).abs()
+            assert bool((err <= 1e-5 * bound + 1e-30).all()), (t, off, float((err / (bound + 1e-30)).max()))
+        off += c
