@@ -391,16 +391,18 @@ class DPTrainFused(torch.nn.Module):
         T = len(self.fit)
         start = np.asarray([r[0] for r in ranges] + [ranges[-1][1]], np.int64)
         maxa = int(max(e - a for a, e in ranges))
-        work = torch.empty(max(_lib.query("matpl_tall_xtz_work", maxa, T, D, 64), 1), dtype=f32, device=dev)
-        Y0 = torch.empty((T, D, 64), dtype=f32, device=dev)
-        _lib.call("matpl_tall_xtz", _ptr(xw0), 2 * D, D, _ptr(zw0), 128, 64, start.ctypes.data, T, D, 64, _ptr(Y0),
+        # outputs [T][in][H]: each type's block is the contiguous gradient of its weight matrix (the
+        # optimizer's multi-tensor kernels need gradients laid out like their parameters)
+        work = torch.empty(max(_lib.query("matpl_tall_xtz_work", maxa, T, max(D, H), H), 1), dtype=f32, device=dev)
+        Y0 = torch.empty((T, D, H), dtype=f32, device=dev)
+        _lib.call("matpl_tall_xtz", _ptr(xw0), 2 * D, D, _ptr(zw0), 128, 64, start.ctypes.data, T, D, H, _ptr(Y0),
                   _ptr(work), work.numel(), _stream())
         Yh = []
         for l in range(1, L):
-            Y = torch.empty((T, 64, 64), dtype=f32, device=dev)
+            Y = torch.empty((T, H, H), dtype=f32, device=dev)
             xp = xh.data_ptr() + (l - 1) * 128 * 4
             zp = zh.data_ptr() + (l - 1) * 128 * 4
-            _lib.call("matpl_tall_xtz", xp, (L - 1) * 128, 64, zp, (L - 1) * 128, 64, start.ctypes.data, T, 64, 64,
+            _lib.call("matpl_tall_xtz", xp, (L - 1) * 128, 64, zp, (L - 1) * 128, 64, start.ctypes.data, T, H, H,
                       _ptr(Y), _ptr(work), work.numel(), _stream())
             Yh.append(Y)
         # sums of the bias, resnet_dt and output rows over each type's positions, one launch:
@@ -425,11 +427,11 @@ class DPTrainFused(torch.nn.Module):
             if e == a:
                 out += [None] * (3 * len(net.W))
                 continue
-            gW = [Y0[t, :, :H]]
+            gW = [Y0[t]]
             gb = [col(0)[t]]
             gr = [None]
             for l in range(1, L):
-                gW.append(Yh[l - 1][t, :H, :H])
+                gW.append(Yh[l - 1][t])
                 gb.append(col(l)[t])
                 gr.append(col(L - 1 + l)[t] if net.has_rdt[l] else None)
             gW.append(col(2 * L - 1)[t].reshape(H, 1))

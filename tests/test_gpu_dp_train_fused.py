@@ -45,8 +45,11 @@ def _compare(frames, cfg, w, dev, precision, seed=3, fitting="kernel"):
     names = [n for n, _ in model.named_parameters()]
     assert len(g) == len(ref_g) == len(names)
     worst, where = 0.0, None
-    for name, a, b in zip(names, g, ref_g):
+    params = [p for _, p in model.named_parameters()]
+    for name, a, b, prm in zip(names, g, ref_g, params):
         assert (a is None) == (b is None), name
+        # laid out like the parameter, so that the optimizer's multi-tensor kernels apply
+        assert a is None or (a.shape == prm.shape and a.stride() == prm.stride()), name
         if a is not None and b.norm() > 0:
             e = float((a.double() - b.double()).norm() / b.double().norm())
             if e > worst:
