@@ -282,7 +282,7 @@ def dp_train_fused_leg(steps=5, precisions=("bf16", "fp32")):
     for prec in precisions:
         fm = DPTrainFused(w, prec, dev)
         fparams = list(fm.parameters())
-        fopt = torch.optim.Adam(fparams, lr=1e-4, foreach=True)
+        fopt = torch.optim.Adam(fparams, lr=1e-4, fused=True)
 
         def fstep():
             loss, g = fm.loss_and_grads(fb)
@@ -336,7 +336,7 @@ def dp_train_leg(steps=5):
     for prec in ("bf16", "fp32"):
         model = DPTrain(w, prec, dev)
         params = [q for q in model.parameters() if q.requires_grad]
-        opt = torch.optim.Adam(params, lr=1e-4, foreach=True)
+        opt = torch.optim.Adam(params, lr=1e-4, fused=True)
 
         def step(model=model, params=params, opt=opt):
             loss, g = train_grads(model, batch)
@@ -349,7 +349,7 @@ def dp_train_leg(steps=5):
         timed(prec, step, steps)
     ref = DPTorch(w, torch.float64, dev, tall=True)
     rparams = [q for q in ref.parameters() if q.requires_grad]
-    ropt = torch.optim.Adam(rparams, lr=1e-4, foreach=True)
+    ropt = torch.optim.Adam(rparams, lr=1e-4, fused=True)
     grads["fp64"] = [None if x is None else x.detach().clone() for x in plain_grads(ref, batch, chunk_frames=32)[1]]
 
     def rstep():
@@ -366,7 +366,7 @@ def dp_train_leg(steps=5):
     for prec in ("bf16", "fp32"):
         fm = DPTrainFused(w, prec, dev)
         fparams = list(fm.parameters())
-        fopt = torch.optim.Adam(fparams, lr=1e-4, foreach=True)
+        fopt = torch.optim.Adam(fparams, lr=1e-4, fused=True)
         grads[prec + "_fused"] = [None if x is None else x.detach().clone() for x in fm.loss_and_grads(fb)[1]]
 
         def fstep(fm=fm, fparams=fparams, fopt=fopt):
