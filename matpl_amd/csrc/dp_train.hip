@@ -26,6 +26,11 @@
 //       products with the identity (no LDS). Each wavefront keeps the
 //       gradient of one type pair in registers while its atoms keep that
 //       pair and adds it to the global accumulators when the pair changes.
+//       Consecutive atoms of one frame and type whose bucket-t entries fit
+//       one 64-lane pass are compacted together and share tiles (a bucket
+//       holds ~10 in-cutoff neighbors in a 4-type system: one atom per tile
+//       would leave two thirds of the lanes idle); a and b are then formed
+//       per lane from its own atom's Ahat and Abar rows.
 //
 // The empty-slot term of A (constant in positions) is differentiated by the
 // caller. The kernels assume the reference's default embedding: no
@@ -35,6 +40,9 @@
 
 namespace matpl {
 namespace {
+
+// Atoms whose bucket entries the gradient kernel takes together (one compaction pass, shared tiles).
+constexpr int EG_ATOMS = 8;
 
 // Per-pair gradient accumulator layout (floats): W3 [E][E], b3 [E], W2 [E][E],
 // b2 [E], w1 [E], b1 [E]; W[in][out] as the net applies x @ W.
@@ -296,9 +304,12 @@ dp_train_embgrad_kernel(TrainParams tp) {
     extern __shared__ float4 smem[];
     const int lane = threadIdx.x & (WAVE - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int h = lane >> 5, n = lane & 31;
-    // per wave: the bucket's compacted neighbors [cap], Ahat and Abar of the atom [32 features][4]
-    float4 *nb = smem + (size_t)wid * (p.cap + 64);
-    float4 *ahs = nb + p.cap, *abs = ahs + 32;
+    // per wave: the batch's compacted neighbors [NBW] (dx, dy, dz, atom of the batch), their
+    // residual differences r_j - r_i [NBW], Ahat and Abar of the batch's atoms [EG_ATOMS][32 features][4]
+    const int NBW = p.cap > WAVE ? p.cap : WAVE;
+    float4 *nb = smem + (size_t)wid * (2 * NBW + 2 * EG_ATOMS * 32);
+    float4 *us = nb + NBW;
+    float4 *ahs = us + NBW, *abs = ahs + EG_ATOMS * 32;
     const int64_t wg = (int64_t)blockIdx.x * FUSED_WAVES + wid;
     const int t = (int)(wg % p.T);
     const int64_t i0 = (wg / p.T) * tp.chunk;
@@ -311,82 +322,123 @@ dp_train_embgrad_kernel(TrainParams tp) {
 #pragma unroll
     for (int r = 0; r < 16; r++) { g3[r] = 0.0f; g2[r] = 0.0f; w1s[r] = 0.0f; b1s[r] = 0.0f; }
     int cur = -1;       // pair held in the accumulators
-    for (int64_t i = i0; i < i1; i++) {
-        const int ti = p.type_map[i];
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int64_t i = i0; i < i1;) {
+        // a batch: atoms i .. i + na - 1 of one frame and one type, at most EG_ATOMS of them, whose
+        // bucket-t entries fit one pass of the wavefront together (or a single atom with more), so
+        // that short buckets share tiles
+        const int64_t ia_l = i + (lane < EG_ATOMS ? lane : 0);
+        const bool in_l = lane < EG_ATOMS && ia_l < i1;
+        const int c_l = in_l ? p.counts[ia_l * p.T + t] : 0;
+        const int ty_l = in_l ? p.type_map[ia_l] : -1;
+        const int fr_l = in_l ? (p.atom_frame ? p.atom_frame[ia_l] : 0) : -1;
+        const int ti = __builtin_amdgcn_readfirstlane(ty_l);
+        const int fr = __builtin_amdgcn_readfirstlane(fr_l);
+        int off[EG_ATOMS + 1];
+        off[0] = 0;
+        int na = 0;
+        bool open = true;
+#pragma unroll
+        for (int q = 0; q < EG_ATOMS; q++) {
+            const int cq = __builtin_amdgcn_readlane(c_l, q);
+            const int tq = __builtin_amdgcn_readlane(ty_l, q);
+            const int fq = __builtin_amdgcn_readlane(fr_l, q);
+            open = open && (q == 0 || (tq == ti && fq == fr && off[q] + cq <= WAVE));
+            if (open) na = q + 1;
+            off[q + 1] = off[q] + (open ? cq : 0);
+        }
+        const int tot = off[EG_ATOMS];
         const int pair = ti * p.T + t;
         if (pair != cur) {
             if (cur >= 0) flush_pair<E>(tp.grad, cur, lane, g3, g2, w1s, b1s);
             cur = pair;
         }
-        const int cnt = p.counts[i * p.T + t];
-        if (cnt == 0) continue;
+        if (tot == 0) {
+            i += na;
+            continue;
+        }
         float scale;
         frame_type_mask(p, i, scale);
         const float *tinfo = p.tinfo + ti * NTI;
-        const AtomGeo g = atom_geo(p, i);
-        const float ri[3] = {tp.resid[3 * i], tp.resid[3 * i + 1], tp.resid[3 * i + 2]};
-        // the atom's Ahat and Abar, scaled by c, feature-major
+        const AtomGeo g0 = atom_geo(p, i);
+        // the batch's Ahat and Abar, scaled by c, feature-major
         {
-            const float *ah = tp.ahat + (size_t)i * 4 * E, *ab = tp.abar + (size_t)i * 4 * E;
-#pragma unroll
-            for (int u = 0; u < 2; u++) {
-                const int x = lane + u * WAVE, f = x >> 2, w = x & 3;
-                reinterpret_cast<float *>(ahs)[x] = f < E ? ah[w * E + f] * scale : 0.0f;
-                reinterpret_cast<float *>(abs)[x] = f < E ? ab[w * E + f] * scale : 0.0f;
+            float *ahf = reinterpret_cast<float *>(ahs), *abf = reinterpret_cast<float *>(abs);
+            for (int x = lane; x < na * 128; x += WAVE) {
+                const int q = x >> 7, f = (x & 127) >> 2, w = x & 3;
+                const size_t src = (size_t)(i + q) * 4 * E + w * E + f;
+                ahf[x] = f < E ? tp.ahat[src] * scale : 0.0f;
+                abf[x] = f < E ? tp.abar[src] * scale : 0.0f;
             }
         }
-        // in-cutoff neighbors of bucket t
+        // in-cutoff neighbors of bucket t, atom by atom in order, with r_j - r_i (loaded with the
+        // neighbor's position, so that the tiles below read nothing from global memory)
         int nv = 0;
-        const uint64_t below = (1ull << lane) - 1ull;
-        for (int e0 = 0; e0 < cnt; e0 += WAVE) {
-            const int k = e0 + lane;
+        for (int e0 = 0; e0 < tot; e0 += WAVE) {
+            const int e = e0 + lane;
+            int q = 0;
+#pragma unroll
+            for (int qq = 1; qq < EG_ATOMS; qq++) q += (e >= off[qq]) ? 1 : 0;
             double dx = 0.0, dy = 0.0, dz = 0.0;
-            const int64_t slot = (i * p.T + t) * (int64_t)p.mnl + k;
-            if (k < cnt) neighbor_disp(p, g, slot, dx, dy, dz);
+            float u0 = 0.0f, u1 = 0.0f, u2 = 0.0f;
+            if (e < tot) {
+                const int64_t ia = i + q;
+                AtomGeo gl = g0;
+                gl.xi = p.pos[3 * ia]; gl.yi = p.pos[3 * ia + 1]; gl.zi = p.pos[3 * ia + 2];
+                const int64_t slot = (ia * p.T + t) * (int64_t)p.mnl + (e - off[q]);
+                const int64_t j = neighbor_disp(p, gl, slot, dx, dy, dz);
+                u0 = tp.resid[3 * j] - tp.resid[3 * ia];
+                u1 = tp.resid[3 * j + 1] - tp.resid[3 * ia + 1];
+                u2 = tp.resid[3 * j + 2] - tp.resid[3 * ia + 2];
+            }
             const double r2 = dx * dx + dy * dy + dz * dz;
-            const bool ok = k < cnt && r2 < p.rc2 && r2 > 1e-10;
+            const bool ok = e < tot && r2 < p.rc2 && r2 > 1e-10;
             const uint64_t bal = __ballot(ok);
             const int idx = nv + __popcll(bal & below);
-            if (ok && idx < p.cap) nb[idx] = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(k));
+            if (ok && idx < NBW) {
+                nb[idx] = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(q));
+                us[idx] = make_float4(u0, u1, u2, 0.0f);
+            }
             nv += __popcll(bal);
         }
-        if (nv > p.cap) nv = p.cap;
+        if (nv > NBW) nv = NBW;
         __builtin_amdgcn_wave_barrier();
-        // A operands of the adjoint products (32 features x 4): lane (f, h) holds M[f][h], M[f][2 + h]
-        const float ah0 = reinterpret_cast<const float *>(ahs)[4 * n + h];
-        const float ah1 = reinterpret_cast<const float *>(ahs)[4 * n + 2 + h];
-        const float ab0 = reinterpret_cast<const float *>(abs)[4 * n + h];
-        const float ab1 = reinterpret_cast<const float *>(abs)[4 * n + 2 + h];
         for (int k0 = 0; k0 < nv; k0 += TILE) {
             const int k = k0 + n;
             const bool live = k < nv;
             const float4 e = live ? nb[k] : make_float4(1.f, 0.f, 0.f, 0.f);
             float Ri[4] = {0.f, 0.f, 0.f, 0.f}, v[4] = {0.f, 0.f, 0.f, 0.f};
             float r = 1.f, s = 0.f, ds = 0.f;
+            int qa = 0;     // the neighbor's atom in the batch
             if (live) {
                 ri_of(e, tinfo, p.rmin, p.rmax, p.iw, Ri, r, s, ds);
-                const int64_t slot = (i * p.T + t) * (int64_t)p.mnl + __float_as_int(e.w);
-                const int64_t j = g.fbase + p.list[slot] - 1;
-                const float u[3] = {tp.resid[3 * j] - ri[0], tp.resid[3 * j + 1] - ri[1], tp.resid[3 * j + 2] - ri[2]};
+                const float4 uu = us[k];
+                const float u[3] = {uu.x, uu.y, uu.z};
                 ri_tangent(e, tinfo, r, s, ds, u, v);
+                qa = __float_as_int(e.w);
             }
-            int pr = pair;
-            asm volatile("" : "+v"(pr));
+            const int pr = pair;
             EmbTan<E, BF16> et;
             et.run(p.vec, reinterpret_cast<const unsigned char *>(p.mat), pr, lane, Ri[0]);
-            // upstream of G: a = Ahat Ri - Abar v; of G': b = -v0 Abar Ri (FP32 matrix cores, K = 4 + 4)
+            // upstream of G: a = Ahat Ri - Abar v; of G': b = -v0 Abar Ri, with the rows of the lane's
+            // own atom of the batch (FP32)
             floatx16 a, b;
+            {
+                const float4 *ahq = ahs + qa * 32, *abq = abs + qa * 32;
 #pragma unroll
-            for (int rr = 0; rr < 16; rr++) { a[rr] = 0.0f; b[rr] = 0.0f; }
-            a = __builtin_amdgcn_mfma_f32_32x32x2f32(ah0, h ? Ri[1] : Ri[0], a, 0, 0, 0);
-            a = __builtin_amdgcn_mfma_f32_32x32x2f32(ah1, h ? Ri[3] : Ri[2], a, 0, 0, 0);
-            a = __builtin_amdgcn_mfma_f32_32x32x2f32(-ab0, h ? v[1] : v[0], a, 0, 0, 0);
-            a = __builtin_amdgcn_mfma_f32_32x32x2f32(-ab1, h ? v[3] : v[2], a, 0, 0, 0);
-            b = __builtin_amdgcn_mfma_f32_32x32x2f32(ab0, h ? Ri[1] : Ri[0], b, 0, 0, 0);
-            b = __builtin_amdgcn_mfma_f32_32x32x2f32(ab1, h ? Ri[3] : Ri[2], b, 0, 0, 0);
-            const float nv0 = -v[0];
-#pragma unroll
-            for (int rr = 0; rr < 16; rr++) b[rr] = rr < NR ? b[rr] * nv0 : 0.0f;
+                for (int rr = 0; rr < 16; rr++) {
+                    a[rr] = 0.0f;
+                    b[rr] = 0.0f;
+                    if (rr < NR) {
+                        const int f = feat(rr, h);
+                        const float4 A = ahq[f], B = abq[f];
+                        const float bri = B.x * Ri[0] + B.y * Ri[1] + B.z * Ri[2] + B.w * Ri[3];
+                        a[rr] = (A.x * Ri[0] + A.y * Ri[1] + A.z * Ri[2] + A.w * Ri[3]) -
+                                (B.x * v[0] + B.y * v[1] + B.z * v[2] + B.w * v[3]);
+                        b[rr] = -v[0] * bri;
+                    }
+                }
+            }
             // layer 3 (G = h2 + t3, G' = ḣ2 + (1 - t3^2) ż3): adjoints of z3 and ż3
             float zb[16], zbd[16], x[16], xd[16];
 #pragma unroll
@@ -442,6 +494,7 @@ dp_train_embgrad_kernel(TrainParams tp) {
             }
         }
         __builtin_amdgcn_wave_barrier();
+        i += na;
     }
     if (cur >= 0) flush_pair<E>(tp.grad, cur, lane, g3, g2, w1s, b1s);
 }
@@ -459,7 +512,8 @@ int launch_train(TrainParams &tp, int which, hipStream_t st) {
                            tp);
         return check_launch("dp_train_tangent_kernel");
     }
-    const size_t lds = sizeof(float4) * FUSED_WAVES * (size_t)(p.cap + 64);
+    const size_t nbw = p.cap > WAVE ? p.cap : WAVE;
+    const size_t lds = sizeof(float4) * FUSED_WAVES * (2 * nbw + 2 * EG_ATOMS * 32);
     MATPL_CHECK_ARG(lds <= 64 * 1024, "dp_train_embgrad: list width too large for LDS (%zu bytes)", lds);
     const int64_t chunks = (p.natoms + tp.chunk - 1) / tp.chunk;
     const int64_t waves = chunks * p.T;
