@@ -336,7 +336,7 @@ def dp_train_leg(steps=5):
     for prec in ("bf16", "fp32"):
         model = DPTrain(w, prec, dev)
         params = [q for q in model.parameters() if q.requires_grad]
-        opt = torch.optim.Adam(params, lr=1e-4, fused=True)
+        opt = torch.optim.Adam(params, lr=1e-4, foreach=True)
 
         def step(model=model, params=params, opt=opt):
             loss, g = train_grads(model, batch)
@@ -349,7 +349,7 @@ def dp_train_leg(steps=5):
         timed(prec, step, steps)
     ref = DPTorch(w, torch.float64, dev, tall=True)
     rparams = [q for q in ref.parameters() if q.requires_grad]
-    ropt = torch.optim.Adam(rparams, lr=1e-4, fused=True)
+    ropt = torch.optim.Adam(rparams, lr=1e-4, foreach=True)
     grads["fp64"] = [None if x is None else x.detach().clone() for x in plain_grads(ref, batch, chunk_frames=32)[1]]
 
     def rstep():
