@@ -41,8 +41,12 @@
 namespace matpl {
 namespace {
 
-// Atoms whose bucket entries the gradient kernel takes together (one compaction pass, shared tiles).
-constexpr int EG_ATOMS = 8;
+// Atoms whose bucket entries the gradient kernel takes together (one compaction pass, shared tiles):
+// 8 in BF16; 4 in FP32, whose transposes take a per-wavefront LDS tile instead.
+template <bool BF16>
+constexpr int eg_atoms() { return BF16 ? 8 : 4; }
+constexpr int EG_TSTRIDE = 33;                        // FP32 transpose tile [32][33] floats
+constexpr int EG_TFLOAT4 = 32 * EG_TSTRIDE / 4;       // its size in float4
 
 // Per-pair gradient accumulator layout (floats): W3 [E][E], b3 [E], W2 [E][E],
 // b2 [E], w1 [E], b1 [E]; W[in][out] as the net applies x @ W.
@@ -210,32 +214,42 @@ dp_train_tangent_kernel(TrainParams tp) {
 template <bool BF16>
 struct Transposer {
     bf16x8 id0, id1;      // BF16: identity B operand, K-chunks 0 and 1 (registers 0-7, 8-15)
-    float idf[16];        // FP32: identity B operand, K-chunk r
+    float *scr;           // FP32: this wavefront's [32][33] LDS tile
+    int h, n;
 
-    __device__ __forceinline__ void init(int lane) {
-        const int h = lane >> 5, n = lane & 31;
+    __device__ __forceinline__ void init(int lane, float *scratch) {
+        h = lane >> 5;
+        n = lane & 31;
+        scr = scratch;
+        if constexpr (BF16) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            id0[q] = (__bf16)(feat(q, h) == n ? 1.0f : 0.0f);
-            id1[q] = (__bf16)(feat(8 + q, h) == n ? 1.0f : 0.0f);
+            for (int q = 0; q < 8; q++) {
+                id0[q] = (__bf16)(feat(q, h) == n ? 1.0f : 0.0f);
+                id1[q] = (__bf16)(feat(8 + q, h) == n ? 1.0f : 0.0f);
+            }
         }
-#pragma unroll
-        for (int r = 0; r < 16; r++) idf[r] = feat(r, h) == n ? 1.0f : 0.0f;
     }
 
     __device__ __forceinline__ floatx16 apply(const float x[16]) const {
         floatx16 d;
-#pragma unroll
-        for (int r = 0; r < 16; r++) d[r] = 0.0f;
         if constexpr (BF16) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) d[r] = 0.0f;
             bf16x8 a0, a1;
 #pragma unroll
             for (int q = 0; q < 8; q++) { a0[q] = (__bf16)x[q]; a1[q] = (__bf16)x[8 + q]; }
             d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, id0, d, 0, 0, 0);
             d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, id1, d, 0, 0, 0);
         } else {
+            // FP32: through LDS (sixteen K = 2 matrix-core products against the identity cost more
+            // than the tile's outer products): lane (n, h) writes row feat(r, h), column n; lane
+            // (f, h) reads row f, columns feat(r, h). The odd stride keeps both phases conflict-free.
 #pragma unroll
-            for (int r = 0; r < 16; r++) d = __builtin_amdgcn_mfma_f32_32x32x2f32(x[r], idf[r], d, 0, 0, 0);
+            for (int r = 0; r < 16; r++) scr[feat(r, h) * EG_TSTRIDE + n] = x[r];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int r = 0; r < 16; r++) d[r] = scr[n * EG_TSTRIDE + feat(r, h)];
+            __builtin_amdgcn_wave_barrier();
         }
         return d;
     }
@@ -305,18 +319,20 @@ dp_train_embgrad_kernel(TrainParams tp) {
     const int lane = threadIdx.x & (WAVE - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int h = lane >> 5, n = lane & 31;
     // per wave: the batch's compacted neighbors [NBW] (dx, dy, dz, atom of the batch), their
-    // residual differences r_j - r_i [NBW], Ahat and Abar of the batch's atoms [EG_ATOMS][32 features][4]
+    // residual differences r_j - r_i [NBW], Ahat and Abar of the batch's atoms [EGA][32 features][4]
+    constexpr int EGA = eg_atoms<BF16>();
     const int NBW = p.cap > WAVE ? p.cap : WAVE;
-    float4 *nb = smem + (size_t)wid * (2 * NBW + 2 * EG_ATOMS * 32);
+    float4 *nb = smem + (size_t)wid * (2 * NBW + 2 * EGA * 32 + (BF16 ? 0 : EG_TFLOAT4));
     float4 *us = nb + NBW;
-    float4 *ahs = us + NBW, *abs = ahs + EG_ATOMS * 32;
+    float4 *ahs = us + NBW, *abs = ahs + EGA * 32;
+    float *tscr = reinterpret_cast<float *>(abs + EGA * 32);     // FP32 transpose tile
     const int64_t wg = (int64_t)blockIdx.x * FUSED_WAVES + wid;
     const int t = (int)(wg % p.T);
     const int64_t i0 = (wg / p.T) * tp.chunk;
     int64_t i1 = i0 + tp.chunk;
     if (i1 > p.natoms) i1 = p.natoms;
     Transposer<BF16> tr;
-    tr.init(lane);
+    tr.init(lane, tscr);
     floatx16 g3, g2;
     float w1s[16], b1s[16];
 #pragma unroll
@@ -324,22 +340,22 @@ dp_train_embgrad_kernel(TrainParams tp) {
     int cur = -1;       // pair held in the accumulators
     const uint64_t below = (1ull << lane) - 1ull;
     for (int64_t i = i0; i < i1;) {
-        // a batch: atoms i .. i + na - 1 of one frame and one type, at most EG_ATOMS of them, whose
+        // a batch: atoms i .. i + na - 1 of one frame and one type, at most EGA of them, whose
         // bucket-t entries fit one pass of the wavefront together (or a single atom with more), so
         // that short buckets share tiles
-        const int64_t ia_l = i + (lane < EG_ATOMS ? lane : 0);
-        const bool in_l = lane < EG_ATOMS && ia_l < i1;
+        const int64_t ia_l = i + (lane < EGA ? lane : 0);
+        const bool in_l = lane < EGA && ia_l < i1;
         const int c_l = in_l ? p.counts[ia_l * p.T + t] : 0;
         const int ty_l = in_l ? p.type_map[ia_l] : -1;
         const int fr_l = in_l ? (p.atom_frame ? p.atom_frame[ia_l] : 0) : -1;
         const int ti = __builtin_amdgcn_readfirstlane(ty_l);
         const int fr = __builtin_amdgcn_readfirstlane(fr_l);
-        int off[EG_ATOMS + 1];
+        int off[EGA + 1];
         off[0] = 0;
         int na = 0;
         bool open = true;
 #pragma unroll
-        for (int q = 0; q < EG_ATOMS; q++) {
+        for (int q = 0; q < EGA; q++) {
             const int cq = __builtin_amdgcn_readlane(c_l, q);
             const int tq = __builtin_amdgcn_readlane(ty_l, q);
             const int fq = __builtin_amdgcn_readlane(fr_l, q);
@@ -347,7 +363,7 @@ dp_train_embgrad_kernel(TrainParams tp) {
             if (open) na = q + 1;
             off[q + 1] = off[q] + (open ? cq : 0);
         }
-        const int tot = off[EG_ATOMS];
+        const int tot = off[EGA];
         const int pair = ti * p.T + t;
         if (pair != cur) {
             if (cur >= 0) flush_pair<E>(tp.grad, cur, lane, g3, g2, w1s, b1s);
@@ -378,7 +394,7 @@ dp_train_embgrad_kernel(TrainParams tp) {
             const int e = e0 + lane;
             int q = 0;
 #pragma unroll
-            for (int qq = 1; qq < EG_ATOMS; qq++) q += (e >= off[qq]) ? 1 : 0;
+            for (int qq = 1; qq < EGA; qq++) q += (e >= off[qq]) ? 1 : 0;
             double dx = 0.0, dy = 0.0, dz = 0.0;
             float u0 = 0.0f, u1 = 0.0f, u2 = 0.0f;
             if (e < tot) {
@@ -513,7 +529,7 @@ int launch_train(TrainParams &tp, int which, hipStream_t st) {
         return check_launch("dp_train_tangent_kernel");
     }
     const size_t nbw = p.cap > WAVE ? p.cap : WAVE;
-    const size_t lds = sizeof(float4) * FUSED_WAVES * (2 * nbw + 2 * EG_ATOMS * 32);
+    const size_t lds = sizeof(float4) * FUSED_WAVES * (2 * nbw + 2 * eg_atoms<BF16>() * 32 + (BF16 ? 0 : EG_TFLOAT4));
     MATPL_CHECK_ARG(lds <= 64 * 1024, "dp_train_embgrad: list width too large for LDS (%zu bytes)", lds);
     const int64_t chunks = (p.natoms + tp.chunk - 1) / tp.chunk;
     const int64_t waves = chunks * p.T;
