@@ -242,6 +242,11 @@ class NEPTorch(torch.nn.Module):
                 plan = FitPlan(type_index, f64, D, H, atom_type_map.device)
                 self._plans = {key: plan}
             return plan
+        # one plan per type map (and its version): the per-type counts are read on the host once
+        return nep_ops._cached(atom_type_map, ("fit_plan", f64, D, H, self.cfg.ntypes),
+                               lambda: self._plan_of(atom_type_map, f64, D, H))
+
+    def _plan_of(self, atom_type_map, f64, D, H):
         tm = atom_type_map.reshape(-1)
         perm = torch.argsort(tm, stable=True)
         counts = torch.bincount(tm, minlength=self.cfg.ntypes).cpu().tolist()

@@ -87,10 +87,39 @@ def static_frames(num_atom: torch.Tensor):
         _STATIC = prev
 
 
+# Host-side input checks read device values (a synchronization each). Their results are kept on the
+# checked tensor, keyed by its version counter (bumped by every in-place change), so that a training
+# loop passing the same num_atom / atom_type_map every step, and the lists that calculate_neighbor
+# builds (valid by construction), pay for each check once.
+def _cached(t, key, fn):
+    try:
+        ver = t._version
+    except RuntimeError:           # inference-mode tensors keep no version counter: no caching
+        return fn()
+    hit = getattr(t, "_matpl_checks", None)
+    if hit is not None and hit[0] == ver and key in hit[1]:
+        return hit[1][key]
+    if hit is None or hit[0] != ver:
+        hit = (ver, {})
+        t._matpl_checks = hit
+    val = fn()
+    hit[1][key] = val
+    return val
+
+
+def _int_max(t):
+    """The largest entry (for the lists calculate_neighbor builds, its bound N - 1)."""
+    return _cached(t, "max", lambda: int(t.max().item()) if t.numel() else -1)
+
+
+def _int_sum(t):
+    return _cached(t, "sum", lambda: int(t.sum().item()))
+
+
 def _check_list(NL, natoms):
     """Neighbor indices must address atoms of the packed batch (the kernels index with them)."""
-    if _STATIC is None and NL.numel() and int(NL.max().item()) >= natoms:
-        raise ValueError(f"CalcOps_cuda NEP ops: neighbor index {int(NL.max().item())} >= {natoms} atoms")
+    if _STATIC is None and NL.numel() and _int_max(NL) >= natoms:
+        raise ValueError(f"CalcOps_cuda NEP ops: neighbor index {_int_max(NL)} >= {natoms} atoms")
 
 
 def _prep(c, Ri, NL, atom_map):
@@ -103,7 +132,7 @@ def _prep(c, Ri, NL, atom_map):
         raise ValueError("CalcOps_cuda NEP ops: NL must be [N, S] with one atom_map entry per atom")
     _expect(Ri, NL.numel() * 4, "Ri [N, S, 4]")
     _check_list(NL, NL.shape[0])
-    if _STATIC is None and atom_map.numel() and int(atom_map.max().item()) >= c.shape[0]:
+    if _STATIC is None and atom_map.numel() and _int_max(atom_map) >= c.shape[0]:
         raise ValueError("CalcOps_cuda NEP ops: atom type beyond the coefficient table")
     return (c.to(Ri.dtype).contiguous(), Ri, NL, atom_map)
 
@@ -223,11 +252,14 @@ class NepForce(torch.autograd.Function):
 def _frame_index(num_atom, device):
     if _STATIC is not None and num_atom is _STATIC.num_atom:
         return _STATIC.start, _STATIC.af
-    na = num_atom.reshape(-1).to(torch.int64)
-    start = torch.zeros(na.numel() + 1, dtype=torch.int64, device=device)
-    start[1:] = torch.cumsum(na.to(device), 0)
-    af = torch.repeat_interleave(torch.arange(na.numel(), device=device, dtype=torch.int32), na.to(device))
-    return start, af
+
+    def build():
+        na = num_atom.reshape(-1).to(torch.int64)
+        start = torch.zeros(na.numel() + 1, dtype=torch.int64, device=device)
+        start[1:] = torch.cumsum(na.to(device), 0)
+        af = torch.repeat_interleave(torch.arange(na.numel(), device=device, dtype=torch.int32), na.to(device))
+        return start, af
+    return _cached(num_atom, ("frames", str(device)), build)
 
 
 class NepVirial(torch.autograd.Function):
@@ -244,8 +276,8 @@ class NepVirial(torch.autograd.Function):
         _expect(Ri_d, N * S * 12, "Ri_d [N, S, 4, 3]")
         _expect(Rij, N * S * 3, "Rij [N, S, 3]")
         _check_list(NL, N)
-        if _STATIC is None and int(num_atom.sum().item()) != N:
-            raise ValueError(f"calculateNepVirial: num_atom sums to {int(num_atom.sum().item())}, list has {N} atoms")
+        if _STATIC is None and _int_sum(num_atom) != N:
+            raise ValueError(f"calculateNepVirial: num_atom sums to {_int_sum(num_atom)}, list has {N} atoms")
         start, af = _frame_index(num_atom, dE.device)
         nf = start.numel() - 1
         virial = torch.empty((nf, 9), dtype=dE.dtype, device=dE.device)
@@ -329,7 +361,7 @@ def nep_centre_force(dE, Ri_d):
 def _nep_geo(num_atoms, box, box_orig, num_cell, position):
     dev = position.device
     nf = num_atoms.numel()
-    if (_STATIC is None and int(num_atoms.sum().item()) != position.shape[0]) or \
+    if (_STATIC is None and _int_sum(num_atoms) != position.shape[0]) or \
             position.numel() != 3 * position.shape[0]:
         raise ValueError("NEP neighbor search: position must be [sum(num_atoms), 3]")
     if box.numel() != 18 * nf or box_orig.numel() != 9 * nf or num_cell.numel() != 3 * nf:
@@ -344,7 +376,7 @@ def calculate_maxneigh(num_atoms, box, box_orig, num_cell, position, cutoff_2b, 
     start, af, box, bo, nc, pos = _nep_geo(num_atoms, box, box_orig, num_cell, position)
     N = pos.shape[0]
     if with_type and (atom_type_map.numel() != N or
-                      (_STATIC is None and N and int(atom_type_map.max().item()) >= atom_type_num)):
+                      (_STATIC is None and N and _int_max(atom_type_map) >= atom_type_num)):
         raise ValueError("calculate_maxneigh: atom_type_map must give each atom a type < atom_type_num")
     shape = (N, atom_type_num) if with_type else (N,)
     NN_r = torch.empty(shape, dtype=torch.int64, device=pos.device)
@@ -377,7 +409,11 @@ def calculate_neighbor(num_atoms, atom_type_map, atom_types, box, box_orig, num_
     if s:
         raise OverflowError(f"calculate_neighbor: more than max_NN neighbors (atom {s - 1})")
     it, dt = num_atoms.dtype, position.dtype
-    return [NN_r.to(it), NN_a.to(it), NL_r.to(it), NL_a.to(it), Ri_r.to(dt), Ri_a.to(dt)]
+    out = [NN_r.to(it), NN_a.to(it), NL_r.to(it), NL_a.to(it), Ri_r.to(dt), Ri_a.to(dt)]
+    # the lists address atoms of this batch only (-1 marks an empty slot): record it for the ops' checks
+    for NL in out[2:4]:
+        _cached(NL, "max", lambda: N - 1)
+    return out
 
 
 def calculate_descriptor(weight_radial, weight_angular, Ri_radial, NL_radial, atom_type_map, cutoff_radial,
