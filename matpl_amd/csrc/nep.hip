@@ -961,6 +961,162 @@ __global__ void nep_virial_grad_kernel(const int64_t *__restrict__ NL, const T *
     }
 }
 
+// The angular descriptor's first derivative (mode 1: gRi and dL/dc) with the slots of several
+// atoms in one 64-lane chunk. An atom has ~16 angular neighbors at rc = 4 A, so one atom per
+// wavefront left three quarters of the lanes idle in the per-slot work; here a wavefront walks a
+// contiguous range of atoms in batches of at most AB_ATOMS whose slots fit one chunk (or one atom
+// with more, in several chunks), each lane taking its slot's atom's dE/ds rows (B) from LDS and
+// recording its slot's coefficient-table row. The per-slot arithmetic and the transposed
+// coefficient-gradient adds are those of nep_angular_kernel<T, 1>.
+constexpr int AB_ATOMS = 4;
+
+__host__ __device__ inline int angb_wave_elems(int nm, int nb) { return AB_ATOMS * nm * NLM + WAVE * (nm + nb); }
+
+template <typename T>
+__global__ void __launch_bounds__(ANG_WAVES * WAVE)
+nep_angular_bwd_kernel(FeatArgs a) {
+    extern __shared__ unsigned char smem_raw[];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ncoef = a.T * a.T * a.nm * a.nb;
+    const int per = a.nm * NLM;
+    const int nm = a.nm, nb = a.nb, nmb = nm * nb;
+    const int ntab = a.pw ? ANG_WAVES : 1;
+    T *gcl = reinterpret_cast<T *>(smem_raw);                                  // [ntab][ncoef]
+    T *gtab = gcl + (ntab > 1 ? wid : 0) * ncoef;                             // this wave's table
+    T *Bq = gcl + (size_t)ntab * ncoef + (size_t)wid * angb_wave_elems(nm, nb);   // [AB_ATOMS][nm][24]
+    T *u1 = Bq + AB_ATOMS * per;                                               // [WAVE][nm]
+    T *v1 = u1 + WAVE * nm;                                                    // [WAVE][nb]
+    int *trow = reinterpret_cast<int *>(gcl + (size_t)ntab * ncoef + (size_t)ANG_WAVES * angb_wave_elems(nm, nb)) +
+                wid * WAVE;                                                    // [WAVE] table row per slot
+    for (int x = threadIdx.x; x < ntab * ncoef; x += blockDim.x) gcl[x] = T(0);
+    __syncthreads();
+    const T *c = reinterpret_cast<const T *>(a.c);
+    const T *Ri = reinterpret_cast<const T *>(a.Ri);
+    const T *gfe = reinterpret_cast<const T *>(a.gfeat);
+    T *gRiA = reinterpret_cast<T *>(a.gRi);
+    const int nf = a.l3 + (a.l4 > 0) + (a.l5 > 0);
+    // this wavefront's contiguous range of atoms
+    const int64_t nw = (int64_t)gridDim.x * ANG_WAVES, gw = (int64_t)blockIdx.x * ANG_WAVES + wid;
+    const int64_t i1 = a.N * (gw + 1) / nw;
+    for (int64_t i = a.N * gw / nw; i < i1;) {
+        const int64_t il = i + (lane < AB_ATOMS ? lane : 0);
+        const int c_l = (lane < AB_ATOMS && il < i1) ? slot_count(a.NL, il, a.S) : -1;
+        int off[AB_ATOMS + 1];
+        off[0] = 0;
+        int na = 0;
+        bool open = true;
+#pragma unroll
+        for (int q = 0; q < AB_ATOMS; q++) {
+            const int cq = __builtin_amdgcn_readlane(c_l, q);
+            open = open && (q == 0 || (cq >= 0 && off[q] + cq <= WAVE));
+            if (open) na = q + 1;
+            off[q + 1] = off[q] + (open ? cq : 0);
+        }
+        const int tot = off[AB_ATOMS];
+        // unused slots of each atom get gRi = 0
+        for (int q = 0; q < na; q++) {
+            const int64_t ia = i + q;
+            for (int k = off[q + 1] - off[q] + lane; k < a.S; k += WAVE) {
+                T *g = gRiA + (ia * a.S + k) * 4;
+                g[0] = g[1] = g[2] = g[3] = T(0);
+            }
+        }
+        // B = D = dE/ds of each atom of the batch (lanes over (atom, n))
+        if (lane < na * nm) {
+            const int q = lane / nm, n = lane - q * nm;
+            const int64_t ia = i + q;
+            T gq[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) gq[f] = f < nf ? gfe[ia * a.ld + a.col0 + aidx(f, n, nm)] : T(0);
+            T D[NLM];
+            dq_ds<T, T>(reinterpret_cast<const T *>(a.s_save) + ia * per + n * NLM, gq, a.l3, a.l4, a.l5, D);
+            for (int m = 0; m < NLM; m++) Bq[q * per + n * NLM + m] = D[m];
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int e0 = 0; e0 < tot; e0 += WAVE) {
+            const int nk = tot - e0 < WAVE ? tot - e0 : WAVE;
+            const int e = e0 + lane;
+            if (lane < nk) {
+                int q = 0;
+#pragma unroll
+                for (int qq = 1; qq < AB_ATOMS; qq++) q += (e >= off[qq]) ? 1 : 0;
+                const int64_t ia = i + q;
+                const int k = e - off[q];
+                const int ti = (int)a.atom_map[ia];
+                const int tj = (int)a.atom_map[a.NL[ia * a.S + k]];
+                const T *er = Ri + (ia * a.S + k) * 4;
+                const T r = er[0], ir = T(1) / r;
+                const T ux = er[1] * ir, uy = er[2] * ir, uz = er[3] * ir;
+                const T *cij = c + ((int64_t)(ti * a.T + tj) * nm) * nb;
+                trow[lane] = ti * a.T + tj;
+                // g_n, g'_n over the streamed basis, f_q to this slot's v1 row
+                T gn[NM_MAX], gnp[NM_MAX];
+                {
+                    Basis<T> bs(r, (T)a.rc);
+#pragma unroll
+                    for (int n = 0; n < NM_MAX; n++) { gn[n] = T(0); gnp[n] = T(0); }
+                    for (int qb = 0; qb < nb; qb++) {
+                        T f, fp;
+                        bs.next(f, fp);
+#pragma unroll
+                        for (int n = 0; n < NM_MAX; n++)
+                            if (n < nm) { gn[n] += cij[n * nb + qb] * f; gnp[n] += cij[n * nb + qb] * fp; }
+                        v1[lane * nb + qb] = f;
+                    }
+                }
+                // gRi = sum_nm D_nm d(g_n b_m)/dRi; dc[n][q] = f_q (D b)_n
+                const T *B = Bq + q * per;
+                T b[NLM];
+                blm_each<T>(ux, uy, uz, [&](int m, T bv, T, T, T) { b[m] = bv; });
+                T Dg[NLM], Db[NM_MAX];
+#pragma unroll
+                for (int n = 0; n < NM_MAX; n++) Db[n] = T(0);
+                T gr0 = T(0);
+#pragma unroll
+                for (int m = 0; m < NLM; m++) {
+                    T dg = T(0), dgp = T(0);
+#pragma unroll
+                    for (int n = 0; n < NM_MAX; n++)
+                        if (n < nm) {
+                            const T Bv = B[n * NLM + m];
+                            dg += Bv * gn[n];
+                            dgp += Bv * gnp[n];
+                            Db[n] += Bv * b[m];
+                        }
+                    Dg[m] = dg;
+                    gr0 += dgp * b[m];
+                }
+#pragma unroll
+                for (int n = 0; n < NM_MAX; n++)
+                    if (n < nm) u1[lane * nm + n] = Db[n];
+                T V0 = T(0), V1 = T(0), V2 = T(0);
+                blm_each<T>(ux, uy, uz, [&](int m, T, T d0, T d1, T d2) {
+                    V0 += Dg[m] * d0; V1 += Dg[m] * d1; V2 += Dg[m] * d2;
+                });
+                T *g = gRiA + (ia * a.S + k) * 4;
+                g[0] = gr0 - (ux * V0 + uy * V1 + uz * V2) * ir;
+                g[1] = V0 * ir;
+                g[2] = V1 * ir;
+                g[3] = V2 * ir;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // dc[row of slot kk][n][q] += u1[kk][n] v1[kk][q]: lanes over (n, q), no two lanes on one entry
+            for (int o = lane; o < nmb; o += WAVE) {
+                const int n = o / nb, qb = o - n * nb;
+                for (int kk = 0; kk < nk; kk++) {
+                    const T d = u1[kk * nm + n] * v1[kk * nb + qb];
+                    if (ntab > 1) gtab[trow[kk] * nmb + o] += d;
+                    else atomicAdd(gtab + trow[kk] * nmb + o, d);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        i += na;
+    }
+    block_flush_add(gcl, reinterpret_cast<T *>(a.gc), ncoef, ntab);
+}
+
 template <typename T>
 int feat_launch(FeatArgs &a, int angular, int mode, hipStream_t st) {
     MATPL_CHECK_ARG(a.nb >= 1 && a.nb <= NB_MAX && a.nm >= 1 && a.nm <= NM_MAX, "nep_feat: basis sizes out of range");
@@ -976,8 +1132,10 @@ int feat_launch(FeatArgs &a, int angular, int mode, hipStream_t st) {
             return mode > 0 ? (ntab * ncoef + (size_t)NEP_WAVES * RCH * a.nb) * sizeof(T) +
                                   (size_t)NEP_WAVES * RCH * sizeof(int)
                             : 0;
-        const int we = mode == 0 ? ang_wave_elems<0>(a.nm, a.nb)
-                     : mode == 1 ? ang_wave_elems<1>(a.nm, a.nb) : ang_wave_elems<2>(a.nm, a.nb);
+        if (mode == 1)
+            return (ntab * ncoef + (size_t)ANG_WAVES * angb_wave_elems(a.nm, a.nb)) * sizeof(T) +
+                   (size_t)ANG_WAVES * WAVE * sizeof(int);
+        const int we = mode == 0 ? ang_wave_elems<0>(a.nm, a.nb) : ang_wave_elems<2>(a.nm, a.nb);
         return (ntab * ncoef + (size_t)ANG_WAVES * we) * sizeof(T) + (size_t)ANG_WAVES * ACH * sizeof(int);
     };
     // float: a table per wavefront (plain adds) when they fit, else one per block (LDS atomics).
@@ -1003,8 +1161,11 @@ int feat_launch(FeatArgs &a, int angular, int mode, hipStream_t st) {
         else hipLaunchKernelGGL((nep_radial_kernel<T, 2>), grid, block, lds, st, a);
         return check_launch("nep_radial_kernel");
     }
+    if (mode == 1) {
+        hipLaunchKernelGGL((nep_angular_bwd_kernel<T>), grid, block, lds, st, a);
+        return check_launch("nep_angular_bwd_kernel");
+    }
     if (mode == 0) hipLaunchKernelGGL((nep_angular_kernel<T, 0>), grid, block, lds, st, a);
-    else if (mode == 1) hipLaunchKernelGGL((nep_angular_kernel<T, 1>), grid, block, lds, st, a);
     else hipLaunchKernelGGL((nep_angular_kernel<T, 2>), grid, block, lds, st, a);
     return check_launch("nep_angular_kernel");
 }
