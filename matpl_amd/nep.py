@@ -84,13 +84,34 @@ def pack_frames(frames, cutoff_radial, device="cuda"):
 
 def neighbors(packed, rc_r, rc_a, ntypes, max_r=None, max_a=None):
     """calculate_maxneigh then calculate_neighbor with Rij (as the reference's NEP data path,
-    nep_data_loader.py / nep_net.py), lists sized by the largest count unless given."""
+    nep_data_loader.py / nep_net.py), lists sized by the largest count unless given.
+
+    The list widths found by calculate_maxneigh are kept in ``packed`` (per cutoff pair): a later
+    call on the same batch (a training loop revisiting it, or MD between rebuilds of the batch)
+    builds its lists at those widths directly and runs the counting pass again only when they
+    overflow. The lists are then at least as wide as the largest count, never narrower; the extra
+    slots are empty (-1), which the descriptor and force ops skip."""
     p = packed
+    key = ("_nep_list_widths", float(rc_r), float(rc_a))
     if max_r is None or max_a is None:
+        widths = p.get(key)
+        # (inside nep_ops.static_frames overflow is recorded, not raised: no reuse there)
+        if widths is not None and nep_ops._STATIC is None:
+            try:
+                return _build_lists(p, rc_r, rc_a, max_r or widths[0], max_a or widths[1])
+            except OverflowError:
+                pass
         NNr, NNa = nep_ops.calculate_maxneigh(p["num_atoms"], p["box"], p["box_orig"], p["num_cell"], p["position"],
                                               rc_r, rc_a, ntypes, p["atom_type_map"], False)
-        max_r = max_r or max(int(NNr.max().item()) if NNr.numel() else 1, 1)
-        max_a = max_a or max(int(NNa.max().item()) if NNa.numel() else 1, 1)
+        wr = max(int(NNr.max().item()) if NNr.numel() else 1, 1)
+        wa = max(int(NNa.max().item()) if NNa.numel() else 1, 1)
+        if max_r is None and max_a is None:
+            p[key] = (wr, wa)
+        max_r, max_a = max_r or wr, max_a or wa
+    return _build_lists(p, rc_r, rc_a, max_r, max_a)
+
+
+def _build_lists(p, rc_r, rc_a, max_r, max_a):
     NN_r, NN_a, NL_r, NL_a, Ri_r, Ri_a = nep_ops.calculate_neighbor(
         p["num_atoms"], p["atom_type_map"], p["atom_type_map"], p["box"], p["box_orig"], p["num_cell"], p["position"],
         rc_r, rc_a, max_r, max_a, True)

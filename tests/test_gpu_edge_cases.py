@@ -106,3 +106,33 @@ def test_nep_fit_kernels_no_atoms(dev, dtype):
     grads = torch.autograd.grad(Ei.sum() + g.sum(), [q, W0, b0, w1, b1])
     assert grads[0].shape == (0, D)
     assert all(float(x.abs().max()) == 0.0 for x in grads[1:])   # no atoms: zero parameter gradients
+
+
+def test_nep_list_widths_reused_and_refreshed(dev):
+    """mn.neighbors keeps the list widths of a batch: a second call builds the same lists without
+    the counting pass, and after the atoms move closer (more neighbors than the kept widths) the
+    overflow makes it count again and widen the lists."""
+    from matpl_amd import structures as st2
+    frames = st2.agausic_frames(nframes=2, nmin=40, nmax=60, seed=5)
+    cfg = NEPConfig(ntypes=4, l_max=[4, 2, 0])
+    gp = mn.pack_frames(frames, cfg.rcut_radial, dev)
+    nb1 = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes)
+    nb2 = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes)
+    for k in ("NN_r", "NN_a", "NL_r", "NL_a", "Ri_r", "Ri_a"):
+        assert torch.equal(nb1[k], nb2[k]), k
+    w = nb1["NL_r"].shape[1], nb1["NL_a"].shape[1]
+    print(f"case list widths {w}", flush=True)
+    # compress every frame about its first atom: more neighbors within the cutoffs
+    pos = gp["position"]
+    start = 0
+    for f in frames:
+        c = pos[start:start + f.natoms].mean(0)
+        pos[start:start + f.natoms] = c + 0.7 * (pos[start:start + f.natoms] - c)
+        start += f.natoms
+    nb3 = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes)
+    fresh = dict(gp)
+    fresh.pop(("_nep_list_widths", float(cfg.rcut_radial), float(cfg.rcut_angular)))
+    nb4 = mn.neighbors(fresh, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes)
+    assert int(nb4["NN_r"].max()) > w[0] or int(nb4["NN_a"].max()) > w[1]   # the kept widths overflowed
+    for k in ("NN_r", "NN_a", "NL_r", "NL_a", "Ri_r", "Ri_a"):
+        assert torch.equal(nb3[k], nb4[k]), k
