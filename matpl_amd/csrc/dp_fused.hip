@@ -364,11 +364,11 @@ dp_fused_bwd_kernel(FusedParams p) {
                 // layer 3: G = h2 + r3 t3 -> dh2 = dG + (W3 diag r3) (dG (1-t3^2))
                 float dz[16];
 #pragma unroll
-                for (int rr = 0; rr < 16; rr++) dz[rr] = acc[rr] * g3.get(rr);
+                for (int rr = 0; rr < 16; rr++) dz[rr] = rr < NR ? acc[rr] * g3.get(rr) : 0.0f;
                 mfma_layer<BF16, LW>(mat_ptr<BF16>(L.mat, tp, M_W3B), lane, dz, acc);
                 // layer 2: h2 = h1 + r2 t2 -> dh1 = dh2 + (W2 diag r2) (dh2 (1-t2^2))
 #pragma unroll
-                for (int rr = 0; rr < 16; rr++) dz[rr] = acc[rr] * g2.get(rr);
+                for (int rr = 0; rr < 16; rr++) dz[rr] = rr < NR ? acc[rr] * g2.get(rr) : 0.0f;
                 mfma_layer<BF16, LW>(mat_ptr<BF16>(L.mat, tp, M_W2B), lane, dz, acc);
                 // layer 1: h1 = tanh(s w1 + b1) -> ds = sum_f dh1 (1-h1^2) w1
 #pragma unroll
@@ -376,8 +376,9 @@ dp_fused_bwd_kernel(FusedParams p) {
                 if constexpr (BF16) dsum *= 1.0f / TANH_K;     // w1 is stored pre-scaled
                 }
                 dRi[0] += dsum;
-#pragma unroll
-                for (int w = 0; w < 4; w++) dRi[w] += other_half(dRi[w]);
+                // sums over the two halves' features, on the half-0 lanes that form the force
+                fold_halves2(dRi[0], dRi[1]);
+                fold_halves2(dRi[2], dRi[3]);
                 // force of this slot: f_a = sum_w dE/dRi_w * Ri_d[w][a], Ri_d = -dfeat/dstd
                 if (live && h == 0) {
                     const float ir = __builtin_amdgcn_rcpf(r);

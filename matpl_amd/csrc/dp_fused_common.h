@@ -219,7 +219,7 @@ __device__ __forceinline__ void embed_fwd(const float *vecb, const unsigned char
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         t2[r] = (r < NR) ? act<BF16>(acc[r]) : 0.0f;
-        h2[r] = h1[r] + r2[r] * t2[r];
+        h2[r] = (r < NR) ? h1[r] + r2[r] * t2[r] : 0.0f;     // padding rows stay literal zeros
     }
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = b3[r];
@@ -227,7 +227,7 @@ __device__ __forceinline__ void embed_fwd(const float *vecb, const unsigned char
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         t3[r] = (r < NR) ? act<BF16>(acc[r]) : 0.0f;
-        G[r] = h2[r] + r3[r] * t3[r];
+        G[r] = (r < NR) ? h2[r] + r3[r] * t3[r] : 0.0f;
     }
 }
 
@@ -258,7 +258,7 @@ __device__ __forceinline__ void embed_fwd_tan(const float *vecb, const unsigned 
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const float t2 = (r < NR) ? act<BF16>(acc[r]) : 0.0f;
-        h2[r] = h1[r] + r2[r] * t2;
+        h2[r] = (r < NR) ? h1[r] + r2[r] * t2 : 0.0f;
         hd2[r] = (r < NR) ? hd1[r] + r2[r] * ((1.0f - t2 * t2) * (tac[r] * IK)) : 0.0f;
     }
 #pragma unroll
@@ -268,7 +268,7 @@ __device__ __forceinline__ void embed_fwd_tan(const float *vecb, const unsigned 
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const float t3 = (r < NR) ? act<BF16>(acc[r]) : 0.0f;
-        G[r] = h2[r] + r3[r] * t3;
+        G[r] = (r < NR) ? h2[r] + r3[r] * t3 : 0.0f;
         Gd[r] = (r < NR) ? hd2[r] + r3[r] * ((1.0f - t3 * t3) * (tac[r] * IK)) : 0.0f;
     }
 }
@@ -551,6 +551,18 @@ __device__ __forceinline__ float other_half(float v) {
     const unsigned u = __float_as_uint(v);
     const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
     return __uint_as_float((threadIdx.x & 32) ? r[0] : r[1]);
+}
+
+// x and y summed over the two half-waves, valid on lanes 0-31 (lanes 32-63: unspecified). One
+// v_permlane32_swap exchanges x's upper half with y's lower half, so that one add gives the x sum
+// on lanes 0-31 and the y sum on lanes 32-63; a second swap brings the y sum down. Four ops for two
+// values where other_half takes five for one.
+__device__ __forceinline__ void fold_halves2(float &x, float &y) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    const float s = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(s), __float_as_uint(s), false, false);
+    x = s;
+    y = __uint_as_float(q[1]);
 }
 
 // Sum over all 64 lanes, without LDS.
