@@ -1117,6 +1117,253 @@ nep_angular_bwd_kernel(FeatArgs a) {
     block_flush_add(gcl, reinterpret_cast<T *>(a.gc), ncoef, ntab);
 }
 
+// The angular descriptor's second-order pass (mode 2: ggfeat and dL/dc) with the slots of up to AB
+// atoms in one chunk of CHB lanes, as nep_angular_bwd_kernel does for mode 1: per batch the dE/ds
+// rows (B) of its atoms, a first pass over the slots (the directional derivative P of each atom's
+// s, summed per atom with lanes over (n, m), and the explicit coefficient gradient), then per atom
+// ggfeat = J P and E = H P, and a second pass (the coefficient gradient through s). Float takes 48
+// slots of up to 4 atoms; double keeps 16 slots of one atom (its per-slot rows, 96 values of 8
+// bytes, leave no room for more in LDS).
+template <typename T>
+struct Ang2Shape {
+    static constexpr int CHB = sizeof(T) == 4 ? 48 : 16;
+    static constexpr int AB = sizeof(T) == 4 ? 4 : 1;
+};
+
+template <typename T>
+__host__ __device__ inline int angb2_wave_elems(int nm, int nb) {
+    return Ang2Shape<T>::AB * 2 * nm * NLM + Ang2Shape<T>::CHB * (4 * nm + 2 * WS + 2 * nb);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(ANG_WAVES * WAVE)
+nep_angular_bwd2_kernel(FeatArgs a) {
+    constexpr int CHB = Ang2Shape<T>::CHB, AB = Ang2Shape<T>::AB;
+    constexpr int NRED = (NM_MAX * NLM + WAVE - 1) / WAVE;
+    extern __shared__ unsigned char smem_raw[];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int ncoef = a.T * a.T * a.nm * a.nb;
+    const int per = a.nm * NLM;
+    const int nm = a.nm, nb = a.nb, nmb = nm * nb;
+    const int ntab = a.pw ? ANG_WAVES : 1;
+    T *gcl = reinterpret_cast<T *>(smem_raw);                                  // [ntab][ncoef]
+    T *gtab = gcl + (ntab > 1 ? wid : 0) * ncoef;                             // this wave's table
+    T *Bq = gcl + (size_t)ntab * ncoef + (size_t)wid * angb2_wave_elems<T>(nm, nb);   // [AB][nm][24] D
+    T *Cq = Bq + AB * per;                                                    // [AB][nm][24] P, then E
+    T *u1 = Cq + AB * per, *u2 = u1 + CHB * nm;                                // g'_n, g_n (pass 2: (E b)_n)
+    T *w1 = u2 + CHB * nm, *w2 = w1 + CHB * WS;                                // beta_m, delta_m
+    T *z1 = w2 + CHB * WS, *z2 = z1 + CHB * nm;                                // (B beta)_n, (B delta)_n
+    T *v1 = z2 + CHB * nm, *v2 = v1 + CHB * nb;                                // f'_q, f_q (pass 2: f_q)
+    int *trow = reinterpret_cast<int *>(gcl + (size_t)ntab * ncoef + (size_t)ANG_WAVES * angb2_wave_elems<T>(nm, nb)) +
+                wid * CHB;                                                     // [CHB] table row per slot
+    for (int x = threadIdx.x; x < ntab * ncoef; x += blockDim.x) gcl[x] = T(0);
+    __syncthreads();
+    const T *c = reinterpret_cast<const T *>(a.c);
+    const T *Ri = reinterpret_cast<const T *>(a.Ri);
+    const T *gfe = reinterpret_cast<const T *>(a.gfeat);
+    const T *sv = reinterpret_cast<const T *>(a.s_save);
+    T *go = reinterpret_cast<T *>(a.ggfeat);
+    const int nf = a.l3 + (a.l4 > 0) + (a.l5 > 0);
+    // dc[row of slot kk][n][q] += p1[kk][n] q1[kk][q] (+ p2 q2): lanes over (n, q)
+    auto add_dc_rows = [&](int nk, const T *p1, const T *q1, const T *p2, const T *q2) {
+        for (int o = lane; o < nmb; o += WAVE) {
+            const int n = o / nb, qb = o - n * nb;
+            for (int kk = 0; kk < nk; kk++) {
+                T d = p1[kk * nm + n] * q1[kk * nb + qb];
+                if (p2) d += p2[kk * nm + n] * q2[kk * nb + qb];
+                if (ntab > 1) gtab[trow[kk] * nmb + o] += d;
+                else atomicAdd(gtab + trow[kk] * nmb + o, d);
+            }
+        }
+    };
+    const int64_t nw = (int64_t)gridDim.x * ANG_WAVES, gw = (int64_t)blockIdx.x * ANG_WAVES + wid;
+    const int64_t i1 = a.N * (gw + 1) / nw;
+    for (int64_t i = a.N * gw / nw; i < i1;) {
+        const int64_t il = i + (lane < AB ? lane : 0);
+        const int c_l = (lane < AB && il < i1) ? slot_count(a.NL, il, a.S) : -1;
+        int off[AB + 1];
+        off[0] = 0;
+        int na = 0;
+        bool open = true;
+#pragma unroll
+        for (int q = 0; q < AB; q++) {
+            const int cq = __builtin_amdgcn_readlane(c_l, q);
+            open = open && (q == 0 || (cq >= 0 && off[q] + cq <= CHB));
+            if (open) na = q + 1;
+            off[q + 1] = off[q] + (open ? cq : 0);
+        }
+        const int tot = off[AB];
+        auto atom_of = [&](int e) {
+            int q = 0;
+#pragma unroll
+            for (int qq = 1; qq < AB; qq++) q += (e >= off[qq]) ? 1 : 0;
+            return q;
+        };
+        // B = D = dE/ds of each atom (lanes over (atom, n))
+        if (lane < na * nm) {
+            const int q = lane / nm, n = lane - q * nm;
+            const int64_t ia = i + q;
+            T gq[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) gq[f] = f < nf ? gfe[ia * a.ld + a.col0 + aidx(f, n, nm)] : T(0);
+            T D[NLM];
+            dq_ds<T, T>(sv + ia * per + n * NLM, gq, a.l3, a.l4, a.l5, D);
+            for (int m = 0; m < NLM; m++) Bq[q * per + n * NLM + m] = D[m];
+        }
+        __builtin_amdgcn_wave_barrier();
+        T red[AB][NRED];
+#pragma unroll
+        for (int q = 0; q < AB; q++)
+#pragma unroll
+            for (int j = 0; j < NRED; j++) red[q][j] = T(0);
+        // pass 1: P_nm = sum_slots gg . d(g_n b_m)/dRi = g'_n beta_m + g_n delta_m with beta_m = gg0 b_m,
+        // delta_m = (db_m/du . (gg_xyz - gg0 u)) / r; explicit c-gradient
+        // sum_m B_nm dP_nm/dc[n][q] = f'_q (B beta)_n + f_q (B delta)_n
+        for (int e0 = 0; e0 < tot; e0 += CHB) {
+            const int nk = tot - e0 < CHB ? tot - e0 : CHB;
+            if (lane < nk) {
+                const int e = e0 + lane, q = atom_of(e);
+                const int64_t ia = i + q;
+                const int k = e - off[q];
+                const int ti = (int)a.atom_map[ia];
+                const int tj = (int)a.atom_map[a.NL[ia * a.S + k]];
+                const T *er = Ri + (ia * a.S + k) * 4;
+                const T r = er[0], ir = T(1) / r;
+                const T ux = er[1] * ir, uy = er[2] * ir, uz = er[3] * ir;
+                const T *cij = c + ((int64_t)(ti * a.T + tj) * nm) * nb;
+                trow[lane] = ti * a.T + tj;
+                const T *gg = reinterpret_cast<const T *>(a.gg) + (ia * a.S + k) * 4;
+                const T g0 = gg[0];
+                const T gx = gg[1] - g0 * ux, gy = gg[2] - g0 * uy, gz = gg[3] - g0 * uz;
+                T gn[NM_MAX], gnp[NM_MAX];
+                {
+                    Basis<T> bs(r, (T)a.rc);
+#pragma unroll
+                    for (int n = 0; n < NM_MAX; n++) { gn[n] = T(0); gnp[n] = T(0); }
+                    for (int qb = 0; qb < nb; qb++) {
+                        T f, fp;
+                        bs.next(f, fp);
+#pragma unroll
+                        for (int n = 0; n < NM_MAX; n++)
+                            if (n < nm) { gn[n] += cij[n * nb + qb] * f; gnp[n] += cij[n * nb + qb] * fp; }
+                        v2[lane * nb + qb] = f;
+                        v1[lane * nb + qb] = fp;
+                    }
+                }
+                T be[NLM], de[NLM];
+                blm_each<T>(ux, uy, uz, [&](int m, T bv, T d0, T d1, T d2) {
+                    be[m] = g0 * bv;
+                    de[m] = (d0 * gx + d1 * gy + d2 * gz) * ir;
+                });
+#pragma unroll
+                for (int m = 0; m < NLM; m++) {
+                    w1[lane * WS + m] = be[m];
+                    w2[lane * WS + m] = de[m];
+                }
+                const T *B = Bq + q * per;
+#pragma unroll
+                for (int n = 0; n < NM_MAX; n++)
+                    if (n < nm) {
+                        u1[lane * nm + n] = gnp[n];
+                        u2[lane * nm + n] = gn[n];
+                        T sb = T(0), sd = T(0);
+#pragma unroll
+                        for (int m = 0; m < NLM; m++) {
+                            sb += B[n * NLM + m] * be[m];
+                            sd += B[n * NLM + m] * de[m];
+                        }
+                        z1[lane * nm + n] = sb;
+                        z2[lane * nm + n] = sd;
+                    }
+            }
+            __builtin_amdgcn_wave_barrier();
+            // P of each atom over its slots of this chunk: lanes own (n, m) outputs
+#pragma unroll
+            for (int q = 0; q < AB; q++) {
+                if (q >= na) break;
+                const int k0 = off[q] - e0 > 0 ? off[q] - e0 : 0;
+                const int k1 = off[q + 1] - e0 < nk ? off[q + 1] - e0 : nk;
+#pragma unroll
+                for (int j = 0; j < NRED; j++) {
+                    const int o = lane + j * WAVE;
+                    if (o < per) {
+                        const int n = o / NLM, m = o - n * NLM;
+                        T sacc = T(0);
+                        for (int kk = k0; kk < k1; kk++)
+                            sacc += u1[kk * nm + n] * w1[kk * WS + m] + u2[kk * nm + n] * w2[kk * WS + m];
+                        red[q][j] += sacc;
+                    }
+                }
+            }
+            add_dc_rows(nk, z1, v1, z2, v2);
+            __builtin_amdgcn_wave_barrier();
+        }
+        // per atom: ggfeat = J P (directional derivative of q along P); E = H(gq) P
+#pragma unroll
+        for (int q = 0; q < AB; q++) {
+            if (q >= na) break;
+#pragma unroll
+            for (int j = 0; j < NRED; j++)
+                if (lane + j * WAVE < per) Cq[q * per + lane + j * WAVE] = red[q][j];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < na * nm) {
+            const int q = lane / nm, n = lane - q * nm;
+            const int64_t ia = i + q;
+            Dual<T> sd[NLM];
+            for (int m = 0; m < NLM; m++) sd[m] = Dual<T>(sv[ia * per + n * NLM + m], Cq[q * per + n * NLM + m]);
+            Dual<T> qv[6];
+            const int nq = q_of_s<T, Dual<T>>(sd, a.l3, a.l4, a.l5, qv);
+#pragma unroll
+            for (int f = 0; f < 6; f++)
+                if (f < nq) go[ia * a.ld + a.col0 + aidx(f, n, nm)] = qv[f].e;
+            T gq[6];
+#pragma unroll
+            for (int f = 0; f < 6; f++) gq[f] = f < nf ? gfe[ia * a.ld + a.col0 + aidx(f, n, nm)] : T(0);
+            Dual<T> D[NLM];
+            dq_ds<T, Dual<T>>(sd, gq, a.l3, a.l4, a.l5, D);
+            for (int m = 0; m < NLM; m++) Cq[q * per + n * NLM + m] = D[m].e;     // E = (H P)_m
+        }
+        __builtin_amdgcn_wave_barrier();
+        // pass 2: c-gradient through s, dc[ti][tj][n][q] += (E b)_n f_q over the slots
+        for (int e0 = 0; e0 < tot; e0 += CHB) {
+            const int nk = tot - e0 < CHB ? tot - e0 : CHB;
+            if (lane < nk) {
+                const int e = e0 + lane, q = atom_of(e);
+                const int64_t ia = i + q;
+                const int k = e - off[q];
+                const int ti = (int)a.atom_map[ia];
+                trow[lane] = ti * a.T + (int)a.atom_map[a.NL[ia * a.S + k]];
+                const T *er = Ri + (ia * a.S + k) * 4;
+                const T ir = T(1) / er[0];
+                Basis<T> bs(er[0], (T)a.rc);
+                for (int qb = 0; qb < nb; qb++) {
+                    T f, fp;
+                    bs.next(f, fp);
+                    v1[lane * nb + qb] = f;
+                }
+                T b[NLM];
+                blm_each<T>(er[1] * ir, er[2] * ir, er[3] * ir, [&](int m, T bv, T, T, T) { b[m] = bv; });
+                const T *E = Cq + q * per;
+#pragma unroll
+                for (int n = 0; n < NM_MAX; n++)
+                    if (n < nm) {
+                        T Eb = T(0);
+#pragma unroll
+                        for (int m = 0; m < NLM; m++) Eb += E[n * NLM + m] * b[m];
+                        u1[lane * nm + n] = Eb;
+                    }
+            }
+            __builtin_amdgcn_wave_barrier();
+            add_dc_rows(nk, u1, v1, nullptr, nullptr);
+            __builtin_amdgcn_wave_barrier();
+        }
+        i += na;
+    }
+    block_flush_add(gcl, reinterpret_cast<T *>(a.gc), ncoef, ntab);
+}
+
 template <typename T>
 int feat_launch(FeatArgs &a, int angular, int mode, hipStream_t st) {
     MATPL_CHECK_ARG(a.nb >= 1 && a.nb <= NB_MAX && a.nm >= 1 && a.nm <= NM_MAX, "nep_feat: basis sizes out of range");
@@ -1135,6 +1382,9 @@ int feat_launch(FeatArgs &a, int angular, int mode, hipStream_t st) {
         if (mode == 1)
             return (ntab * ncoef + (size_t)ANG_WAVES * angb_wave_elems(a.nm, a.nb)) * sizeof(T) +
                    (size_t)ANG_WAVES * WAVE * sizeof(int);
+        if (mode == 2)
+            return (ntab * ncoef + (size_t)ANG_WAVES * angb2_wave_elems<T>(a.nm, a.nb)) * sizeof(T) +
+                   (size_t)ANG_WAVES * Ang2Shape<T>::CHB * sizeof(int);
         const int we = mode == 0 ? ang_wave_elems<0>(a.nm, a.nb) : ang_wave_elems<2>(a.nm, a.nb);
         return (ntab * ncoef + (size_t)ANG_WAVES * we) * sizeof(T) + (size_t)ANG_WAVES * ACH * sizeof(int);
     };
@@ -1165,8 +1415,11 @@ int feat_launch(FeatArgs &a, int angular, int mode, hipStream_t st) {
         hipLaunchKernelGGL((nep_angular_bwd_kernel<T>), grid, block, lds, st, a);
         return check_launch("nep_angular_bwd_kernel");
     }
-    if (mode == 0) hipLaunchKernelGGL((nep_angular_kernel<T, 0>), grid, block, lds, st, a);
-    else hipLaunchKernelGGL((nep_angular_kernel<T, 2>), grid, block, lds, st, a);
+    if (mode == 2) {
+        hipLaunchKernelGGL((nep_angular_bwd2_kernel<T>), grid, block, lds, st, a);
+        return check_launch("nep_angular_bwd2_kernel");
+    }
+    hipLaunchKernelGGL((nep_angular_kernel<T, 0>), grid, block, lds, st, a);
     return check_launch("nep_angular_kernel");
 }
 
