@@ -405,9 +405,10 @@ __device__ __forceinline__ void block_flush_add(T *lds, T *glob, int count, int 
     }
 }
 
-// Feature kernels. One wavefront per centre atom. Per-neighbor work runs with
-// lanes over the atom's list entries (slots) and writes per-slot factors to a
-// per-wave LDS chunk. Every sum over slots then runs transposed, with lanes
+// Feature kernels. Radial: one wavefront per centre atom; angular: a wavefront
+// walks a contiguous range of atoms, several atoms' slots to a chunk (below).
+// Per-neighbor work runs with lanes over the list entries (slots) and writes
+// per-slot factors to a per-wave LDS chunk. Every sum over slots then runs transposed, with lanes
 // over its outputs reading the chunk as broadcasts, so no two lanes of a
 // wavefront ever add to the same address:
 //   radial  q_n                  lanes over slots, wave sums (registers)
@@ -419,9 +420,6 @@ __device__ __forceinline__ void block_flush_add(T *lds, T *glob, int count, int 
 // per entry at the end.
 // mode: 0 = forward features, 1 = backward (gRi, gc), 2 = second order (ggfeat, gc)
 constexpr int RCH = WAVE;        // radial slots per chunk
-constexpr int ACH = 32;          // angular slots per chunk (at most)
-template <int MODE>
-__host__ __device__ constexpr int ang_ch() { return MODE == 2 ? 16 : ACH; }
 constexpr int ANG_WAVES = 2;     // wavefronts per angular block (LDS per wave is large)
 
 template <typename T, int MODE>
@@ -527,286 +525,127 @@ nep_radial_kernel(FeatArgs a) {
     if (MODE > 0) block_flush_add(gcl, reinterpret_cast<T *>(a.gc), ncoef, ntab);
 }
 
-// LDS elements of one angular wave: per-atom arrays A, B, C of nm*24, and the
-// slot chunk (CH = ang_ch slots): MODE 0 u[CH][nm] + w[CH][24]; MODE 1
-// u[CH][nm] + v[CH][nb]; MODE 2 two (u, w) pairs for P, then two (u, v) pairs
-// for the explicit c-gradient.
-template <int MODE>
-__host__ __device__ inline int ang_chunk_elems(int nm, int nb) {
-    constexpr int CH = ang_ch<MODE>();
-    if (MODE == 0) return CH * (nm + WS);
-    if (MODE == 1) return CH * (nm + nb);
-    return CH * (2 * nm + 2 * WS) + CH * (2 * nm + 2 * nb);
-}
-template <int MODE>
-__host__ __device__ inline int ang_wave_elems(int nm, int nb) { return 3 * nm * NLM + ang_chunk_elems<MODE>(nm, nb); }
+// Angular features (mode 0): s_nm = sum over an atom's slots of g_n(r) b_m(u), with the slots of up
+// to AB_ATOMS atoms in one 64-lane chunk (an atom has ~16 angular neighbors at rc = 4 A). Lanes take
+// slots for g_n and b_m (written to per-wave LDS rows), then lanes own (n, m) outputs and sum each
+// atom's rows; s goes to s_save (the derivative passes read it) and q_of_s(s) to the features.
+constexpr int AB_ATOMS = 4;      // atoms whose angular slots share one chunk
 
-template <typename T, int MODE>
+__host__ __device__ inline int angf_wave_elems(int nm) { return AB_ATOMS * nm * NLM + WAVE * (nm + WS); }
+
+template <typename T>
 __global__ void __launch_bounds__(ANG_WAVES * WAVE)
-nep_angular_kernel(FeatArgs a) {
-    extern __shared__ unsigned char smem_raw[];
-    constexpr int CH = ang_ch<MODE>();
+nep_angular_fwd_kernel(FeatArgs a) {
     constexpr int NRED = (NM_MAX * NLM + WAVE - 1) / WAVE;
-    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-    const int ncoef = a.T * a.T * a.nm * a.nb;
+    extern __shared__ unsigned char smem_raw[];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
     const int per = a.nm * NLM;
     const int nm = a.nm, nb = a.nb;
-    const int ntab = (MODE > 0 && a.pw) ? ANG_WAVES : 1;
-    T *gcl = reinterpret_cast<T *>(smem_raw);                                  // [ntab][ncoef]
-    T *gtab = gcl + (ntab > 1 ? wid : 0) * ncoef;                             // this wave's table
-    T *wv = gcl + (size_t)ntab * ncoef + (size_t)wid * ang_wave_elems<MODE>(nm, nb);
-    T *A = wv, *B = wv + per, *C = wv + 2 * per;                               // s, D, P / E
-    T *X = wv + 3 * per;                                                       // slot chunk
-    int *tch = reinterpret_cast<int *>(gcl + (size_t)ntab * ncoef + (size_t)ANG_WAVES * ang_wave_elems<MODE>(nm, nb)) +
-               wid * ACH;
-    // chunk views
-    T *u1 = X, *u2 = X + CH * nm;                          // MODE 0/1: u = u1; MODE 2: gnp, gn
-    T *w1 = X + (MODE == 2 ? 2 : 1) * CH * nm;             // MODE 0: b; MODE 2: gg0 b
-    T *w2 = w1 + CH * WS;                                  // MODE 2: gg . db
-    T *z1 = X + 2 * CH * nm + 2 * CH * WS, *z2 = z1 + CH * nm;    // MODE 2: B beta, B delta
-    T *v1 = MODE == 1 ? X + CH * nm : z2 + CH * nm, *v2 = v1 + CH * nb;
-    if (MODE > 0) {
-        for (int x = threadIdx.x; x < ntab * ncoef; x += blockDim.x) gcl[x] = T(0);
-        __syncthreads();
-    }
+    T *Sq = reinterpret_cast<T *>(smem_raw) + (size_t)wid * angf_wave_elems(nm);   // [AB_ATOMS][nm][24]
+    T *u1 = Sq + AB_ATOMS * per;                                                    // [WAVE][nm] g_n
+    T *w1 = u1 + WAVE * nm;                                                         // [WAVE][WS] b_m
     const T *c = reinterpret_cast<const T *>(a.c);
     const T *Ri = reinterpret_cast<const T *>(a.Ri);
-    const T *gfe = reinterpret_cast<const T *>(a.gfeat);
-    const int nf = a.l3 + (a.l4 > 0) + (a.l5 > 0);
-    const int nmb = nm * nb;
-    // dc[ti][tch[kk]][n][q] += sum of the u*v pairs of each slot kk of a chunk: lanes over (n, q)
-    auto add_dc = [&](int ti, int nk, const T *p1, const T *q1, const T *p2, const T *q2) {
-        __builtin_amdgcn_wave_barrier();
-        for (int o = lane; o < nmb; o += WAVE) {
-            const int n = o / nb, q = o - n * nb;
-            for (int kk = 0; kk < nk; kk++) {
-                T d = p1[kk * nm + n] * q1[kk * nb + q];
-                if (p2) d += p2[kk * nm + n] * q2[kk * nb + q];
-                // lanes own distinct o: a wave's own table needs no atomics
-                if (ntab > 1) gtab[(ti * a.T + tch[kk]) * nmb + o] += d;
-                else atomicAdd(gtab + (ti * a.T + tch[kk]) * nmb + o, d);
-            }
+    T *fo = reinterpret_cast<T *>(a.feats);
+    T *so = reinterpret_cast<T *>(a.s_save);
+    const int64_t nw = (int64_t)gridDim.x * ANG_WAVES, gw = (int64_t)blockIdx.x * ANG_WAVES + wid;
+    const int64_t i1 = a.N * (gw + 1) / nw;
+    for (int64_t i = a.N * gw / nw; i < i1;) {
+        const int64_t il = i + (lane < AB_ATOMS ? lane : 0);
+        const int c_l = (lane < AB_ATOMS && il < i1) ? slot_count(a.NL, il, a.S) : -1;
+        int off[AB_ATOMS + 1];
+        off[0] = 0;
+        int na = 0;
+        bool open = true;
+#pragma unroll
+        for (int q = 0; q < AB_ATOMS; q++) {
+            const int cq = __builtin_amdgcn_readlane(c_l, q);
+            open = open && (q == 0 || (cq >= 0 && off[q] + cq <= WAVE));
+            if (open) na = q + 1;
+            off[q + 1] = off[q] + (open ? cq : 0);
         }
-        __builtin_amdgcn_wave_barrier();
-    };
-    // g_n = sum_q c[n][q] f_q and g'_n over the streamed basis; optionally stores f_q / f'_q to LDS rows
-    auto radial_g = [&](T r, const T *cij, T gn[NM_MAX], T gnp[NM_MAX], T *vf, T *vfp) {
-        Basis<T> bs(r, (T)a.rc);
+        const int tot = off[AB_ATOMS];
+        T red[AB_ATOMS][NRED];
 #pragma unroll
-        for (int n = 0; n < NM_MAX; n++) { gn[n] = T(0); gnp[n] = T(0); }
-        for (int q = 0; q < nb; q++) {
-            T f, fp;
-            bs.next(f, fp);
+        for (int q = 0; q < AB_ATOMS; q++)
 #pragma unroll
-            for (int n = 0; n < NM_MAX; n++)
-                if (n < nm) { gn[n] += cij[n * nb + q] * f; gnp[n] += cij[n * nb + q] * fp; }
-            if (vf) vf[q] = f;
-            if (vfp) vfp[q] = fp;
-        }
-    };
-    for (int64_t i = (int64_t)blockIdx.x * ANG_WAVES + wid; i < a.N; i += (int64_t)gridDim.x * ANG_WAVES) {
-        const int ti = (int)a.atom_map[i];
-        const int cnt = slot_count(a.NL, i, a.S);
-        const T *s_at = reinterpret_cast<const T *>(a.s_save) + i * per;
-        if (MODE > 0) {
-            for (int x = lane; x < per; x += WAVE) A[x] = s_at[x];                // A = s
-            __builtin_amdgcn_wave_barrier();
-            // B = D = dE/ds (per radial index n, lanes over n)
-            for (int n = lane; n < nm; n += WAVE) {
-                T gq[6];
-#pragma unroll
-                for (int f = 0; f < 6; f++) gq[f] = f < nf ? gfe[i * a.ld + a.col0 + aidx(f, n, nm)] : T(0);
-                T D[NLM];
-                dq_ds<T, T>(A + n * NLM, gq, a.l3, a.l4, a.l5, D);
-                for (int m = 0; m < NLM; m++) B[n * NLM + m] = D[m];
-            }
-        }
-        if (MODE == 1)
-            for (int k = cnt + lane; k < a.S; k += WAVE) {
-                T *gRi = reinterpret_cast<T *>(a.gRi) + (i * a.S + k) * 4;
-                gRi[0] = gRi[1] = gRi[2] = gRi[3] = T(0);
-            }
-        __builtin_amdgcn_wave_barrier();
-        // accumulators of the transposed sums (MODE 0: s, MODE 2: P); lane owns outputs lane + 64 j
-        T red[NRED];
-#pragma unroll
-        for (int j = 0; j < NRED; j++) red[j] = T(0);
-        for (int k0 = 0; k0 < cnt; k0 += CH) {
-            const int nk = cnt - k0 < CH ? cnt - k0 : CH;
-            const int k = k0 + lane;
+            for (int j = 0; j < NRED; j++) red[q][j] = T(0);
+        for (int e0 = 0; e0 < tot; e0 += WAVE) {
+            const int nk = tot - e0 < WAVE ? tot - e0 : WAVE;
             if (lane < nk) {
-                const int tj = (int)a.atom_map[a.NL[i * a.S + k]];
-                const T *e = Ri + (i * a.S + k) * 4;
-                const T r = e[0], ir = T(1) / r;
-                const T ux = e[1] * ir, uy = e[2] * ir, uz = e[3] * ir;
+                const int e = e0 + lane;
+                int q = 0;
+#pragma unroll
+                for (int qq = 1; qq < AB_ATOMS; qq++) q += (e >= off[qq]) ? 1 : 0;
+                const int64_t ia = i + q;
+                const int k = e - off[q];
+                const int ti = (int)a.atom_map[ia];
+                const int tj = (int)a.atom_map[a.NL[ia * a.S + k]];
+                const T *er = Ri + (ia * a.S + k) * 4;
+                const T r = er[0], ir = T(1) / r;
                 const T *cij = c + ((int64_t)(ti * a.T + tj) * nm) * nb;
-                tch[lane] = tj;
-                T gn[NM_MAX], gnp[NM_MAX];
-                if (MODE == 0) {
-                    radial_g(r, cij, gn, gnp, nullptr, nullptr);
+                T gn[NM_MAX];
+                {
+                    Basis<T> bs(r, (T)a.rc);
 #pragma unroll
-                    for (int n = 0; n < NM_MAX; n++)
-                        if (n < nm) u1[lane * nm + n] = gn[n];
-                    blm_each<T>(ux, uy, uz, [&](int m, T bv, T, T, T) { w1[lane * WS + m] = bv; });
-                } else if (MODE == 1) {
-                    // gRi = sum_nm D_nm d(g_n b_m)/dRi; dc[n][q] = f_q (D b)_n
-                    radial_g(r, cij, gn, gnp, v1 + lane * nb, nullptr);
-                    T b[NLM];
-                    blm_each<T>(ux, uy, uz, [&](int m, T bv, T, T, T) { b[m] = bv; });
-                    T Dg[NLM], Db[NM_MAX];
-#pragma unroll
-                    for (int n = 0; n < NM_MAX; n++) Db[n] = T(0);
-                    T gr0 = T(0);
-#pragma unroll
-                    for (int m = 0; m < NLM; m++) {
-                        T dg = T(0), dgp = T(0);
+                    for (int n = 0; n < NM_MAX; n++) gn[n] = T(0);
+                    for (int qb = 0; qb < nb; qb++) {
+                        T f, fp;
+                        bs.next(f, fp);
 #pragma unroll
                         for (int n = 0; n < NM_MAX; n++)
-                            if (n < nm) {
-                                const T Bv = B[n * NLM + m];
-                                dg += Bv * gn[n];
-                                dgp += Bv * gnp[n];
-                                Db[n] += Bv * b[m];
-                            }
-                        Dg[m] = dg;
-                        gr0 += dgp * b[m];
+                            if (n < nm) gn[n] += cij[n * nb + qb] * f;
                     }
-#pragma unroll
-                    for (int n = 0; n < NM_MAX; n++)
-                        if (n < nm) u1[lane * nm + n] = Db[n];
-                    // V = sum_m Dg_m db_m/du
-                    T V0 = T(0), V1 = T(0), V2 = T(0);
-                    blm_each<T>(ux, uy, uz, [&](int m, T, T d0, T d1, T d2) {
-                        V0 += Dg[m] * d0; V1 += Dg[m] * d1; V2 += Dg[m] * d2;
-                    });
-                    T *gRi = reinterpret_cast<T *>(a.gRi) + (i * a.S + k) * 4;
-                    gRi[0] = gr0 - (ux * V0 + uy * V1 + uz * V2) * ir;
-                    gRi[1] = V0 * ir;
-                    gRi[2] = V1 * ir;
-                    gRi[3] = V2 * ir;
-                } else {
-                    // P_nm = sum_slots gg . d(g_n b_m)/dRi = g'_n beta_m + g_n delta_m with
-                    // beta_m = gg0 b_m, delta_m = (db_m/du . (gg_xyz - gg0 u)) / r; explicit c-gradient
-                    // sum_m B_nm dP_nm/dc[n][q] = f'_q (B beta)_n + f_q (B delta)_n
-                    const T *gg = reinterpret_cast<const T *>(a.gg) + (i * a.S + k) * 4;
-                    const T g0 = gg[0];
-                    const T gx = gg[1] - g0 * ux, gy = gg[2] - g0 * uy, gz = gg[3] - g0 * uz;
-                    radial_g(r, cij, gn, gnp, v2 + lane * nb, v1 + lane * nb);
-                    T be[NLM], de[NLM];
-                    blm_each<T>(ux, uy, uz, [&](int m, T bv, T d0, T d1, T d2) {
-                        be[m] = g0 * bv;
-                        de[m] = (d0 * gx + d1 * gy + d2 * gz) * ir;
-                    });
-#pragma unroll
-                    for (int m = 0; m < NLM; m++) {
-                        w1[lane * WS + m] = be[m];
-                        w2[lane * WS + m] = de[m];
-                    }
-#pragma unroll
-                    for (int n = 0; n < NM_MAX; n++)
-                        if (n < nm) {
-                            u1[lane * nm + n] = gnp[n];
-                            u2[lane * nm + n] = gn[n];
-                            T sb = T(0), sd = T(0);
-#pragma unroll
-                            for (int m = 0; m < NLM; m++) {
-                                sb += B[n * NLM + m] * be[m];
-                                sd += B[n * NLM + m] * de[m];
-                            }
-                            z1[lane * nm + n] = sb;
-                            z2[lane * nm + n] = sd;
-                        }
                 }
+#pragma unroll
+                for (int n = 0; n < NM_MAX; n++)
+                    if (n < nm) u1[lane * nm + n] = gn[n];
+                blm_each<T>(er[1] * ir, er[2] * ir, er[3] * ir, [&](int m, T bv, T, T, T) { w1[lane * WS + m] = bv; });
             }
             __builtin_amdgcn_wave_barrier();
-            if (MODE == 0 || MODE == 2) {
-                // transposed sum over the chunk's slots: lanes own (n, m) outputs
+#pragma unroll
+            for (int q = 0; q < AB_ATOMS; q++) {
+                if (q >= na) break;
+                const int k0 = off[q] - e0 > 0 ? off[q] - e0 : 0;
+                const int k1 = off[q + 1] - e0 < nk ? off[q + 1] - e0 : nk;
 #pragma unroll
                 for (int j = 0; j < NRED; j++) {
                     const int o = lane + j * WAVE;
                     if (o < per) {
                         const int n = o / NLM, m = o - n * NLM;
                         T sacc = T(0);
-                        for (int kk = 0; kk < nk; kk++) {
-                            sacc += u1[kk * nm + n] * w1[kk * WS + m];
-                            if (MODE == 2) sacc += u2[kk * nm + n] * w2[kk * WS + m];
-                        }
-                        red[j] += sacc;
+                        for (int kk = k0; kk < k1; kk++) sacc += u1[kk * nm + n] * w1[kk * WS + m];
+                        red[q][j] += sacc;
                     }
                 }
-            }
-            if (MODE == 1) add_dc(ti, nk, u1, v1, nullptr, nullptr);
-            if (MODE == 2) add_dc(ti, nk, z1, v1, z2, v2);
-            if (MODE == 0) __builtin_amdgcn_wave_barrier();
-        }
-        if (MODE == 0 || MODE == 2) {
-            T *dst = MODE == 0 ? A : C;
-#pragma unroll
-            for (int j = 0; j < NRED; j++)
-                if (lane + j * WAVE < per) dst[lane + j * WAVE] = red[j];
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (MODE == 0) {
-            T *fo = reinterpret_cast<T *>(a.feats);
-            T *so = reinterpret_cast<T *>(a.s_save) + i * per;
-            for (int x = lane; x < per; x += WAVE) so[x] = A[x];
-            for (int n = lane; n < nm; n += WAVE) {
-                T q[6];
-                const int nq = q_of_s<T, T>(A + n * NLM, a.l3, a.l4, a.l5, q);
-#pragma unroll
-                for (int f = 0; f < 6; f++)
-                    if (f < nq) fo[i * a.ld + a.col0 + aidx(f, n, nm)] = q[f];
-            }
-        } else if (MODE == 2) {
-            // ggfeat = J P (directional derivative of q along P); E = H(gq) P -> stored in C
-            T *go = reinterpret_cast<T *>(a.ggfeat);
-            for (int n = lane; n < nm; n += WAVE) {
-                Dual<T> sd[NLM];
-                for (int m = 0; m < NLM; m++) sd[m] = Dual<T>(A[n * NLM + m], C[n * NLM + m]);
-                Dual<T> q[6];
-                const int nq = q_of_s<T, Dual<T>>(sd, a.l3, a.l4, a.l5, q);
-#pragma unroll
-                for (int f = 0; f < 6; f++)
-                    if (f < nq) go[i * a.ld + a.col0 + aidx(f, n, nm)] = q[f].e;
-                T gq[6];
-#pragma unroll
-                for (int f = 0; f < 6; f++) gq[f] = f < nf ? gfe[i * a.ld + a.col0 + aidx(f, n, nm)] : T(0);
-                Dual<T> D[NLM];
-                dq_ds<T, Dual<T>>(sd, gq, a.l3, a.l4, a.l5, D);
-                for (int m = 0; m < NLM; m++) C[n * NLM + m] = D[m].e;     // E = (H P)_m
             }
             __builtin_amdgcn_wave_barrier();
-            // c-gradient through s: dc[ti][tj][n][q] += (E b)_n f_q over the slots
-            for (int k0 = 0; k0 < cnt; k0 += CH) {
-                const int nk = cnt - k0 < CH ? cnt - k0 : CH;
-                if (lane < nk) {
-                    const int k = k0 + lane;
-                    const T *e = Ri + (i * a.S + k) * 4;
-                    const T ir = T(1) / e[0];
-                    Basis<T> bs(e[0], (T)a.rc);
-                    for (int q = 0; q < nb; q++) {
-                        T f, fp;
-                        bs.next(f, fp);
-                        v1[lane * nb + q] = f;
-                    }
-                    T b[NLM];
-                    blm_each<T>(e[1] * ir, e[2] * ir, e[3] * ir, [&](int m, T bv, T, T, T) { b[m] = bv; });
-                    tch[lane] = (int)a.atom_map[a.NL[i * a.S + k]];
+        }
 #pragma unroll
-                    for (int n = 0; n < NM_MAX; n++)
-                        if (n < nm) {
-                            T Eb = T(0);
+        for (int q = 0; q < AB_ATOMS; q++) {
+            if (q >= na) break;
 #pragma unroll
-                            for (int m = 0; m < NLM; m++) Eb += C[n * NLM + m] * b[m];
-                            u1[lane * nm + n] = Eb;
-                        }
+            for (int j = 0; j < NRED; j++) {
+                const int o = lane + j * WAVE;
+                if (o < per) {
+                    Sq[q * per + o] = red[q][j];
+                    so[(i + q) * per + o] = red[q][j];
                 }
-                add_dc(ti, nk, u1, v1, nullptr, nullptr);
             }
         }
         __builtin_amdgcn_wave_barrier();
+        if (lane < na * nm) {
+            const int q = lane / nm, n = lane - q * nm;
+            T qv[6];
+            const int nq = q_of_s<T, T>(Sq + q * per + n * NLM, a.l3, a.l4, a.l5, qv);
+#pragma unroll
+            for (int f = 0; f < 6; f++)
+                if (f < nq) fo[(i + q) * a.ld + a.col0 + aidx(f, n, nm)] = qv[f];
+        }
+        __builtin_amdgcn_wave_barrier();
+        i += na;
     }
-    if (MODE > 0) block_flush_add(gcl, reinterpret_cast<T *>(a.gc), ncoef, ntab);
 }
 
 // ------------------------------------------------------------------ force / virial ops
@@ -961,15 +800,10 @@ __global__ void nep_virial_grad_kernel(const int64_t *__restrict__ NL, const T *
     }
 }
 
-// The angular descriptor's first derivative (mode 1: gRi and dL/dc) with the slots of several
-// atoms in one 64-lane chunk. An atom has ~16 angular neighbors at rc = 4 A, so one atom per
-// wavefront left three quarters of the lanes idle in the per-slot work; here a wavefront walks a
-// contiguous range of atoms in batches of at most AB_ATOMS whose slots fit one chunk (or one atom
-// with more, in several chunks), each lane taking its slot's atom's dE/ds rows (B) from LDS and
-// recording its slot's coefficient-table row. The per-slot arithmetic and the transposed
-// coefficient-gradient adds are those of nep_angular_kernel<T, 1>.
-constexpr int AB_ATOMS = 4;
-
+// The angular descriptor's first derivative (mode 1: gRi and dL/dc), batched like the features: per
+// batch the dE/ds rows (B) of its atoms in LDS, then per slot (lanes) gRi = sum_nm D_nm d(g_n b_m)/dRi
+// and the rows of dc[n][q] = f_q (D b)_n, added with lanes over (n, q) into the table row of each
+// slot's (centre type, neighbor type).
 __host__ __device__ inline int angb_wave_elems(int nm, int nb) { return AB_ATOMS * nm * NLM + WAVE * (nm + nb); }
 
 template <typename T>
@@ -1385,8 +1219,7 @@ int feat_launch(FeatArgs &a, int angular, int mode, hipStream_t st) {
         if (mode == 2)
             return (ntab * ncoef + (size_t)ANG_WAVES * angb2_wave_elems<T>(a.nm, a.nb)) * sizeof(T) +
                    (size_t)ANG_WAVES * Ang2Shape<T>::CHB * sizeof(int);
-        const int we = mode == 0 ? ang_wave_elems<0>(a.nm, a.nb) : ang_wave_elems<2>(a.nm, a.nb);
-        return (ntab * ncoef + (size_t)ANG_WAVES * we) * sizeof(T) + (size_t)ANG_WAVES * ACH * sizeof(int);
+        return (size_t)ANG_WAVES * angf_wave_elems(a.nm) * sizeof(T);
     };
     // float: a table per wavefront (plain adds) when they fit, else one per block (LDS atomics).
     // Measured on the NEP training batch: float LDS atomic adds keep the LDS busy 3-4x longer than
@@ -1419,8 +1252,8 @@ int feat_launch(FeatArgs &a, int angular, int mode, hipStream_t st) {
         hipLaunchKernelGGL((nep_angular_bwd2_kernel<T>), grid, block, lds, st, a);
         return check_launch("nep_angular_bwd2_kernel");
     }
-    hipLaunchKernelGGL((nep_angular_kernel<T, 0>), grid, block, lds, st, a);
-    return check_launch("nep_angular_kernel");
+    hipLaunchKernelGGL((nep_angular_fwd_kernel<T>), grid, block, lds, st, a);
+    return check_launch("nep_angular_fwd_kernel");
 }
 
 template <typename T>
