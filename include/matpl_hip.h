@@ -556,6 +556,37 @@ int matpl_nep_virial_grad(int f64, const int64_t *NL, const void *Rij, const voi
                           const int32_t *atom_frame, void *g, int64_t N, int S, void *stream);
 
 /*
+ * The NEP model's force and frame virial of one neighbor list in one pass,
+ * from dE [N][S][4] and Ri [N][S][4] = [r, x, y, z] (both 16-byte aligned):
+ * the position derivative Ri_d (row 0 = (x, y, z)/r, rows 1-3 = identity,
+ * zero where |r| <= 1e-5) is formed in registers instead of read. Fuses the
+ * model's sequence ri_d (src/model/nep_net.py:440-468), the centre term
+ * (nep_net.py:608), calculateNepForce (src/op/src/CalcOps.cpp:75-158) and the
+ * frame sum of calculateNepVirial (CalcOps.cpp:239-328, nep_net.py:613-622):
+ * F [N][3] = the whole force (centre + neighbor terms; written, not
+ * accumulated), virial [nframes][9] per frame. Frame f holds atoms
+ * frame_start[f]..frame_start[f+1]; atom_frame [N]; every list entry must lie
+ * in its atom's frame; max_frame >= the largest frame's atom count. work:
+ * matpl_nep_force_virial_ri_work(f64, N, max_frame) elements of the dtype
+ * (per-atom virial rows and per-block LDS force copies). Entries with
+ * NL < 0 contribute nothing.
+ */
+int64_t matpl_nep_force_virial_ri_work(int f64, int64_t N, int64_t max_frame);
+int matpl_nep_force_virial_ri(int f64, const int64_t *NL, const void *dE, const void *Ri, const int64_t *frame_start,
+                              const int32_t *atom_frame, int64_t nframes, int64_t N, int S, int64_t max_frame,
+                              void *F, void *virial, void *work, int64_t work_elems, void *stream);
+
+/*
+ * Backward of matpl_nep_force_virial_ri with respect to dE: g [N][S][4]
+ * (written) from gF [N][3] and gV [nframes][9] (either may be NULL for zero;
+ * atom_frame is needed with gV): with q = gF[j] - gF[i] + gV[frame(i)] Rij,
+ * g = [q . (x, y, z)/r, q] on used entries with |r| > 1e-5, else 0. The sum of
+ * matpl_nep_force_grad and matpl_nep_virial_grad without Ri_d.
+ */
+int matpl_nep_force_virial_ri_grad(int f64, const int64_t *NL, const void *Ri, const void *gF, const void *gV,
+                                   const int32_t *atom_frame, int64_t N, int S, void *g, void *stream);
+
+/*
  * NEP fitting nets (one hidden layer of H tanh units, linear output; the
  * reference's FittingNet with network_size [H, 1], src/model/nep_fitting.py:
  * 40-163, run per atom type at src/model/nep_net.py:470-501) for all types in

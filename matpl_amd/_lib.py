@@ -81,6 +81,9 @@ SIGNATURES = {
     "matpl_nep_frame_virial": (I32, [I32, P, P, P, P, P, I64, I64, I32, P, P, P]),
     "matpl_nep_force_grad": (I32, [I32, P, P, P, P, I64, I32, P]),
     "matpl_nep_virial_grad": (I32, [I32, P, P, P, P, P, P, I64, I32, P]),
+    "matpl_nep_force_virial_ri_work": (I64, [I32, I64, I64]),
+    "matpl_nep_force_virial_ri": (I32, [I32, P, P, P, P, P, I64, I64, I32, I64, P, P, P, I64, P]),
+    "matpl_nep_force_virial_ri_grad": (I32, [I32, P, P, P, P, P, I64, I32, P, P]),
     "matpl_nep_fit_tile_atoms": (I32, [I32, I32, I32]),
     "matpl_nep_fit_part_width": (I32, [I32, I32]),
     "matpl_nep_fit_forward": (I32, [I32, P, P, P, P, I32, I32, P, P, P, P, I32, I32, P, P, P]),

@@ -218,18 +218,18 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
-// Displacement of replica (ia, ib, ic) of atom n2 from n1, minimum image in the expanded box, and
-// its square; the expression order is the reference's (calculateNepNeighbor.cu), so that the lists
-// and [r, x, y, z] are bit-identical.
-__device__ __forceinline__ double replica_disp(const double *h, const double *bo, const double *pos, int64_t n2,
+// Displacement of replica (ia, ib, ic) of the atom at (px, py, pz) from n1, minimum image in the
+// expanded box h, and its square; the expression order is the reference's (calculateNepNeighbor.cu),
+// so that the lists and [r, x, y, z] are bit-identical.
+__device__ __forceinline__ double replica_disp(const double *h, const double *bo, double px, double py, double pz,
                                                int ia, int ib, int ic, double x1, double y1, double z1, double &x12,
                                                double &y12, double &z12) {
     const double dx = bo[0] * (double)ia + bo[1] * (double)ib + bo[2] * (double)ic;
     const double dy = bo[3] * (double)ia + bo[4] * (double)ib + bo[5] * (double)ic;
     const double dz = bo[6] * (double)ia + bo[7] * (double)ib + bo[8] * (double)ic;
-    x12 = pos[3 * n2] + dx - x1;
-    y12 = pos[3 * n2 + 1] + dy - y1;
-    z12 = pos[3 * n2 + 2] + dz - z1;
+    x12 = px + dx - x1;
+    y12 = py + dy - y1;
+    z12 = pz + dz - z1;
     apply_mic(h, x12, y12, z12);
     return x12 * x12 + y12 * y12 + z12 * z12;
 }
@@ -242,23 +242,32 @@ __device__ __forceinline__ void put_entry(int64_t *NL, double *Ri, int64_t idx, 
     else { o[0] = x; o[1] = y; o[2] = z; }
 }
 
-// mode 0: lists; 1: counts; 2: per-type counts. One wavefront per atom n1, lanes over the frame's
+// MODE 0: lists; 1: counts; 2: per-type counts. One wavefront per atom n1, lanes over the frame's
 // atoms n2, each with its replicas (ia, ib, ic) in the reference's order; a list holds the hits in
 // (n2, replica) order. With one replica (the box is at least 2 rc thick) a lane has at most one hit
 // per list and one pass both counts and writes; several replicas take a counting pass, a scan and a
-// writing pass.
+// writing pass. The next chunk's positions are loaded before the current chunk is tested, and the
+// per-type counts of MODE 2 are held by lane t for type t, so that the lists and counts modes keep
+// their register count low (several wavefronts per SIMD hide the load latency).
+template <int MODE>
 __global__ void __launch_bounds__(256)
 nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int32_t *__restrict__ atom_frame,
                     const int64_t *__restrict__ type_map, const double *__restrict__ pos, double rc_r, double rc_a,
-                    int64_t maxr, int64_t maxa, int64_t n, int mode, int ntypes, int rij4, int64_t *__restrict__ NN_r,
+                    int64_t maxr, int64_t maxa, int64_t n, int ntypes, int rij4, int64_t *__restrict__ NN_r,
                     int64_t *__restrict__ NL_r, int64_t *__restrict__ NN_a, int64_t *__restrict__ NL_a,
                     double *__restrict__ Ri_r, double *__restrict__ Ri_a, int32_t *__restrict__ status) {
     const int lane = threadIdx.x & (WAVE - 1);
-    const int64_t n1 = blockIdx.x * (int64_t)(blockDim.x / WAVE) + threadIdx.x / WAVE;
+    // the wave index is uniform: the atom, its frame and the frame's boxes stay in scalar registers
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int64_t n1 = blockIdx.x * (int64_t)(blockDim.x / WAVE) + wid;
     if (n1 >= n) return;
     const int f = atom_frame[n1];
     const int64_t N1 = frame_start[f], N2 = frame_start[f + 1];
-    const double *h = g.box + 18 * (int64_t)f, *bo = g.box_orig + 9 * (int64_t)f;
+    double h[18], bo[9];
+#pragma unroll
+    for (int q = 0; q < 18; q++) h[q] = g.box[18 * (int64_t)f + q];
+#pragma unroll
+    for (int q = 0; q < 9; q++) bo[q] = g.box_orig[9 * (int64_t)f + q];
     const int64_t *nc = g.num_cell + 3 * (int64_t)f;
     const int na = (int)nc[0], nb = (int)nc[1], ncc = (int)nc[2];
     const bool one = na == 1 && nb == 1 && ncc == 1;
@@ -267,38 +276,48 @@ nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int
     const int w = rij4 ? 4 : 3;
     const uint64_t below = (1ull << lane) - 1ull;
     int64_t baser = 0, basea = 0;
-    int64_t tr[16], ta[16];
-    for (int t = 0; t < 16; t++) tr[t] = ta[t] = 0;
+    int64_t mytr = 0, myta = 0;     // MODE 2: the counts of type `lane`
     bool overflow = false;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (N1 + lane < N2) {
+        const int64_t n2 = N1 + lane;
+        px = pos[3 * n2]; py = pos[3 * n2 + 1]; pz = pos[3 * n2 + 2];
+    }
     for (int64_t c0 = N1; c0 < N2; c0 += WAVE) {
         const int64_t n2 = c0 + lane;
         const bool valid = n2 < N2;
+        const double cx = px, cy = py, cz = pz;
+        if (n2 + WAVE < N2) {
+            const int64_t m2 = n2 + WAVE;
+            px = pos[3 * m2]; py = pos[3 * m2 + 1]; pz = pos[3 * m2 + 2];
+        }
         int cr = 0, ca = 0;
-        // type_map is only given (and only read) for the per-type counts of mode 2
-        const int t2 = (valid && mode == 2) ? (int)type_map[n2] : 0;
+        // type_map is only given (and only read) for the per-type counts of MODE 2
+        const int t2 = (MODE == 2 && valid) ? (int)type_map[n2] : 0;
         if (one) {
-            // one replica: count and, in mode 0, write in the same pass
+            // one replica: count and, in MODE 0, write in the same pass
             double x12 = 0.0, y12 = 0.0, z12 = 0.0, d2 = 0.0;
             if (valid && n1 != n2) {
-                d2 = replica_disp(h, bo, pos, n2, 0, 0, 0, x1, y1, z1, x12, y12, z12);
+                d2 = replica_disp(h, bo, cx, cy, cz, 0, 0, 0, x1, y1, z1, x12, y12, z12);
                 cr = d2 < rr2 ? 1 : 0;
                 ca = d2 < ra2 ? 1 : 0;
             }
-            if (mode == 2) {
+            if constexpr (MODE == 2) {
                 for (int t = 0; t < ntypes; t++) {
-                    tr[t] += __popcll(__ballot(cr && t2 == t));
-                    ta[t] += __popcll(__ballot(ca && t2 == t));
+                    const int ur = __popcll(__ballot(cr && t2 == t)), ua = __popcll(__ballot(ca && t2 == t));
+                    if (lane == t) { mytr += ur; myta += ua; }
                 }
-                continue;
+            } else {
+                const uint64_t br = __ballot(cr), ba = __ballot(ca);
+                const int64_t kr = baser + __popcll(br & below), ka = basea + __popcll(ba & below);
+                baser += __popcll(br);
+                basea += __popcll(ba);
+                if constexpr (MODE == 0) {
+                    if (baser > maxr || basea > maxa) overflow = true;
+                    if (cr && kr < maxr) put_entry(NL_r, Ri_r, n1 * maxr + kr, w, rij4, n2, d2, x12, y12, z12);
+                    if (ca && ka < maxa) put_entry(NL_a, Ri_a, n1 * maxa + ka, w, rij4, n2, d2, x12, y12, z12);
+                }
             }
-            const uint64_t br = __ballot(cr), ba = __ballot(ca);
-            const int64_t kr = baser + __popcll(br & below), ka = basea + __popcll(ba & below);
-            baser += __popcll(br);
-            basea += __popcll(ba);
-            if (mode == 1) continue;
-            if (baser > maxr || basea > maxa) overflow = true;
-            if (cr && kr < maxr) put_entry(NL_r, Ri_r, n1 * maxr + kr, w, rij4, n2, d2, x12, y12, z12);
-            if (ca && ka < maxa) put_entry(NL_a, Ri_a, n1 * maxa + ka, w, rij4, n2, d2, x12, y12, z12);
             continue;
         }
         // pass 1: counts
@@ -308,17 +327,17 @@ nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int
                     for (int ic = 0; ic < ncc; ic++) {
                         if (ia == 0 && ib == 0 && ic == 0 && n1 == n2) continue;
                         double x12, y12, z12;
-                        const double d2 = replica_disp(h, bo, pos, n2, ia, ib, ic, x1, y1, z1, x12, y12, z12);
+                        const double d2 = replica_disp(h, bo, cx, cy, cz, ia, ib, ic, x1, y1, z1, x12, y12, z12);
                         if (d2 < rr2) cr++;
                         if (d2 < ra2) ca++;
                     }
         }
-        if (mode == 2) {
+        if constexpr (MODE == 2) {
             for (int t = 0; t < ntypes; t++) {
                 const int vr = (valid && t2 == t) ? cr : 0, va = (valid && t2 == t) ? ca : 0;
                 const int sr = wave_incl_scan(vr), sa = wave_incl_scan(va);
-                tr[t] += __shfl(sr, WAVE - 1, WAVE);
-                ta[t] += __shfl(sa, WAVE - 1, WAVE);
+                const int ur = __shfl(sr, WAVE - 1, WAVE), ua = __shfl(sa, WAVE - 1, WAVE);
+                if (lane == t) { mytr += ur; myta += ua; }
             }
             continue;
         }
@@ -327,7 +346,7 @@ nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int
         const int64_t offr = baser + ir - cr, offa = basea + ia_ - ca;
         baser += sumr;
         basea += suma;
-        if (mode == 1) continue;
+        if constexpr (MODE == 1) continue;
         if (baser > maxr || basea > maxa) { overflow = true; continue; }
         // pass 2: write in (n2, replica) order
         if (valid && (cr > 0 || ca > 0)) {
@@ -337,22 +356,22 @@ nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int
                     for (int ic = 0; ic < ncc; ic++) {
                         if (ia == 0 && ib == 0 && ic == 0 && n1 == n2) continue;
                         double x12, y12, z12;
-                        const double d2 = replica_disp(h, bo, pos, n2, ia, ib, ic, x1, y1, z1, x12, y12, z12);
+                        const double d2 = replica_disp(h, bo, cx, cy, cz, ia, ib, ic, x1, y1, z1, x12, y12, z12);
                         if (d2 < rr2) put_entry(NL_r, Ri_r, n1 * maxr + kr++, w, rij4, n2, d2, x12, y12, z12);
                         if (d2 < ra2) put_entry(NL_a, Ri_a, n1 * maxa + ka++, w, rij4, n2, d2, x12, y12, z12);
                     }
         }
     }
-    if (lane == 0) {
-        if (mode == 2) {
-            for (int t = 0; t < ntypes; t++) { NN_r[n1 * ntypes + t] = tr[t]; NN_a[n1 * ntypes + t] = ta[t]; }
-        } else {
-            NN_r[n1] = baser;
-            NN_a[n1] = basea;
-            if (overflow) atomicCAS(status, 0, (int)(n1 + 1));
-        }
+    if constexpr (MODE == 2) {
+        if (lane < ntypes) { NN_r[n1 * ntypes + lane] = mytr; NN_a[n1 * ntypes + lane] = myta; }
+        return;
     }
-    if (mode != 0) return;
+    if (lane == 0) {
+        NN_r[n1] = baser;
+        NN_a[n1] = basea;
+        if (overflow) atomicCAS(status, 0, (int)(n1 + 1));
+    }
+    if constexpr (MODE != 0) return;
     // pad the tails (-1 / 0)
     for (int64_t k = (baser < maxr ? baser : maxr) + lane; k < maxr; k += WAVE) {
         NL_r[n1 * maxr + k] = -1;
@@ -758,6 +777,160 @@ __global__ void __launch_bounds__(256) nep_virial_reduce_kernel(const T *__restr
         __syncthreads();
     }
     if (threadIdx.x == 0) virial[9 * f + v] = red[0];
+}
+
+// ------------------------------------------------------------------ fused force + frame virial
+// F and the per-frame virial of the NEP model from dE and Ri = [r, x, y, z] directly: the position
+// derivative Ri_d (row 0 = (x, y, z)/r, rows 1-3 = identity, zero where |r| <= 1e-5; nep_net.py:440-468)
+// is formed in registers, so f(s) = dE[s][0] (x, y, z)/r + dE[s][1:4] per used slot s of atom i.
+// calculateNepForce's terms (CalcOps.cpp:75-158): F[i] -= f(s), F[j] += f(s); calculateNepVirial's
+// frame sum (calculateNepVirial.cu:19-96): V[frame] += f(s) (x) Rij[s].
+// A block takes FV_CH consecutive atoms i and adds the force terms of their slots into an LDS copy of
+// F over the frames those atoms belong to (every j of a slot lies in its atom's frame), then writes
+// that copy to its own row of `part`; fv_reduce_kernel adds the rows of the blocks covering each
+// atom. A block whose frames exceed the LDS copy adds its terms to F with global atomics instead, as
+// does any entry naming an atom outside its block's frames.
+// Unlike per-slot global atomics, which on this GPU are carried out beyond the per-XCD L2 caches, the
+// LDS adds stay on the CU.
+constexpr int FV_CH = 64;       // atoms per block
+constexpr int FV_WAVES = 4;
+
+template <typename T>
+struct V4 { T x, y, z, w; };
+
+template <typename T>
+__device__ __forceinline__ V4<T> ld4(const T *p) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(p);
+        return {v.x, v.y, v.z, v.w};
+    } else {
+        const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+        return {a.x, a.y, b.x, b.y};
+    }
+}
+
+// the frames of atoms [i0, i1): [lo, hi)
+__device__ __forceinline__ void fv_range(const int64_t *frame_start, const int32_t *atom_frame, int64_t N, int64_t b,
+                                         int64_t &lo, int64_t &hi) {
+    const int64_t i0 = b * FV_CH, i1 = i0 + FV_CH < N ? i0 + FV_CH : N;
+    lo = frame_start[atom_frame[i0]];
+    hi = frame_start[atom_frame[i1 - 1] + 1];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(FV_WAVES * WAVE)
+nep_fv_kernel(const int64_t *__restrict__ NL, const T *__restrict__ dE, const T *__restrict__ Ri,
+              const int64_t *__restrict__ frame_start, const int32_t *__restrict__ atom_frame, int64_t N, int S,
+              int64_t rcap, T *__restrict__ F, T *__restrict__ part, T *__restrict__ rows) {
+    extern __shared__ double fv_smem[];
+    T *acc = reinterpret_cast<T *>(fv_smem);
+    const int lane = threadIdx.x & (WAVE - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int64_t b = blockIdx.x, i0 = b * FV_CH, i1 = i0 + FV_CH < N ? i0 + FV_CH : N;
+    int64_t lo, hi;
+    fv_range(frame_start, atom_frame, N, b, lo, hi);
+    const bool lds = hi - lo <= rcap;
+    if (lds)
+        for (int64_t r = threadIdx.x; r < 3 * (hi - lo); r += blockDim.x) acc[r] = T(0);
+    __syncthreads();
+    for (int64_t i = i0 + wid; i < i1; i += FV_WAVES) {
+        T c[3] = {T(0), T(0), T(0)}, v[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) v[q] = T(0);
+        for (int k = lane; k < S; k += WAVE) {
+            const int64_t s = i * S + k;
+            const int64_t j = NL[s];
+            const V4<T> q = ld4(Ri + 4 * s), e = ld4(dE + 4 * s);
+            if (j < 0 || !(fabs(q.x) > T(1e-5))) continue;
+            const T f[3] = {e.x * (q.y / q.x) + e.y, e.x * (q.z / q.x) + e.z, e.x * (q.w / q.x) + e.w};
+            const T x[3] = {q.y, q.z, q.w};
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                c[a] += f[a];
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++) v[3 * a + cc] += f[a] * x[cc];
+                if (lds && j >= lo && j < hi) atomicAdd(acc + 3 * (j - lo) + a, f[a]);
+                else atomic_add(F + 3 * j + a, f[a]);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const T x = wave_sum(c[a]);
+            if (lane == 0) {
+                if (lds) atomicAdd(acc + 3 * (i - lo) + a, -x);
+                else atomic_add(F + 3 * i + a, -x);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const T x = wave_sum(v[q]);
+            if (lane == 0) rows[9 * i + q] = x;
+        }
+    }
+    __syncthreads();
+    if (lds)
+        for (int64_t r = threadIdx.x; r < 3 * (hi - lo); r += blockDim.x) part[b * 3 * rcap + r] = acc[r];
+}
+
+// F[j] += the LDS copies of the blocks whose atoms share j's frame (blocks fv_range-covering it)
+template <typename T>
+__global__ void __launch_bounds__(256)
+nep_fv_reduce_kernel(const int64_t *__restrict__ frame_start, const int32_t *__restrict__ atom_frame, int64_t N,
+                     int64_t rcap, const T *__restrict__ part, T *__restrict__ F) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < N; j += (int64_t)gridDim.x * blockDim.x) {
+        const int f = atom_frame[j];
+        const int64_t fs = frame_start[f], fe = frame_start[f + 1];
+        if (fe - fs > rcap) continue;             // no block covering this frame used LDS
+        T s[3] = {T(0), T(0), T(0)};
+        for (int64_t b = fs / FV_CH; b <= (fe - 1) / FV_CH; b++) {
+            int64_t lo, hi;
+            fv_range(frame_start, atom_frame, N, b, lo, hi);
+            if (hi - lo > rcap) continue;
+            const T *p = part + b * 3 * rcap + 3 * (j - lo);
+            s[0] += p[0]; s[1] += p[1]; s[2] += p[2];
+        }
+        F[3 * j] += s[0]; F[3 * j + 1] += s[1]; F[3 * j + 2] += s[2];
+    }
+}
+
+// Backward of nep_fv_kernel wrt dE: with q = gF[j] - gF[i] + gV[frame(i)] Rij[s] (either upstream
+// gradient may be absent), g[s] = [q . (x, y, z)/r, q] on used slots with |r| > 1e-5, else 0
+// (calculateNepForce.cu:74-122 and the virial's transpose, in one pass). One wavefront per atom.
+template <typename T>
+__global__ void __launch_bounds__(256)
+nep_fv_grad_kernel(const int64_t *__restrict__ NL, const T *__restrict__ Ri, const T *__restrict__ gF,
+                   const T *__restrict__ gV, const int32_t *__restrict__ atom_frame, int64_t N, int S,
+                   T *__restrict__ g) {
+    const int lane = threadIdx.x & (WAVE - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    for (int64_t i = blockIdx.x * (int64_t)(blockDim.x / WAVE) + wid; i < N;
+         i += (int64_t)gridDim.x * (blockDim.x / WAVE)) {
+        T fi[3] = {T(0), T(0), T(0)}, gv[9];
+        if (gF) { fi[0] = gF[3 * i]; fi[1] = gF[3 * i + 1]; fi[2] = gF[3 * i + 2]; }
+#pragma unroll
+        for (int q = 0; q < 9; q++) gv[q] = gV ? gV[9 * (int64_t)atom_frame[i] + q] : T(0);
+        for (int k = lane; k < S; k += WAVE) {
+            const int64_t s = i * S + k;
+            const int64_t j = NL[s];
+            const V4<T> r = ld4(Ri + 4 * s);
+            T o[4] = {T(0), T(0), T(0), T(0)};
+            if (j >= 0 && fabs(r.x) > T(1e-5)) {
+                const T x[3] = {r.y, r.z, r.w};
+                T q[3];
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    q[a] = gv[3 * a] * x[0] + gv[3 * a + 1] * x[1] + gv[3 * a + 2] * x[2];
+                    if (gF) q[a] += gF[3 * j + a] - fi[a];
+                }
+                o[0] = q[0] * (x[0] / r.x) + q[1] * (x[1] / r.x) + q[2] * (x[2] / r.x);
+                o[1] = q[0]; o[2] = q[1]; o[3] = q[2];
+            }
+            if constexpr (sizeof(T) == 4) {
+                *reinterpret_cast<float4 *>(g + 4 * s) = make_float4(o[0], o[1], o[2], o[3]);
+            } else {
+                reinterpret_cast<double2 *>(g + 4 * s)[0] = make_double2(o[0], o[1]);
+                reinterpret_cast<double2 *>(g + 4 * s)[1] = make_double2(o[2], o[3]);
+            }
+        }
+    }
 }
 
 // backward of calculateNepForce wrt dE (calculateNepForce.cu:74-122): the centre
@@ -1343,9 +1516,17 @@ MATPL_EXPORT int matpl_nep_neighbor(const double *box, const double *box_orig, c
     MATPL_CHECK_ARG(mode != 0 || (NL_r && NL_a && Ri_r && Ri_a && status), "nep_neighbor: null list output");
     MATPL_CHECK_ARG(mode != 2 || type_map, "nep_neighbor: typed counts need type_map");
     NepGeo g{box, box_orig, num_cell};
-    hipLaunchKernelGGL(nep_neighbor_kernel, dim3(ceil_div(n, 4)), dim3(256), 0, (hipStream_t)stream, g, frame_start,
-                       atom_frame, type_map, pos, rc_r, rc_a, maxr, maxa, n, mode, ntypes, rij4, NN_r, NL_r, NN_a,
-                       NL_a, Ri_r, Ri_a, status);
+    const dim3 grid(ceil_div(n, 4)), block(256);
+    hipStream_t st = (hipStream_t)stream;
+    if (mode == 0)
+        hipLaunchKernelGGL(nep_neighbor_kernel<0>, grid, block, 0, st, g, frame_start, atom_frame, type_map, pos, rc_r,
+                           rc_a, maxr, maxa, n, ntypes, rij4, NN_r, NL_r, NN_a, NL_a, Ri_r, Ri_a, status);
+    else if (mode == 1)
+        hipLaunchKernelGGL(nep_neighbor_kernel<1>, grid, block, 0, st, g, frame_start, atom_frame, type_map, pos, rc_r,
+                           rc_a, maxr, maxa, n, ntypes, rij4, NN_r, NL_r, NN_a, NL_a, Ri_r, Ri_a, status);
+    else
+        hipLaunchKernelGGL(nep_neighbor_kernel<2>, grid, block, 0, st, g, frame_start, atom_frame, type_map, pos, rc_r,
+                           rc_a, maxr, maxa, n, ntypes, rij4, NN_r, NL_r, NN_a, NL_a, Ri_r, Ri_a, status);
     return check_launch("nep_neighbor_kernel");
 }
 
@@ -1401,6 +1582,75 @@ MATPL_EXPORT int matpl_nep_force_grad(int f64, const int64_t *NL, const void *Ri
         hipLaunchKernelGGL(nep_force_grad_kernel<float>, dim3(grid_cap(N * S)), dim3(256), 0, st, NL,
                            (const float *)Ri_d, (const float *)gF, (float *)g, S, N * S);
     return check_launch("nep_force_grad_kernel");
+}
+
+// LDS copy of F per block: at most 64 KB, and no more than the largest span of frames a block of
+// FV_CH atoms can touch (two partial frames and whole ones in between)
+inline int64_t fv_rcap(int64_t max_frame, size_t tsize) {
+    const int64_t cap = (int64_t)(65536 / (3 * tsize));
+    const int64_t want = 2 * max_frame + FV_CH;
+    return want < cap ? want : cap;
+}
+
+MATPL_EXPORT int64_t matpl_nep_force_virial_ri_work(int f64, int64_t N, int64_t max_frame) {
+    if (N <= 0) return 0;
+    const int64_t rcap = fv_rcap(max_frame, f64 ? 8 : 4);
+    return 9 * N + (int64_t)ceil_div(N, FV_CH) * 3 * rcap;
+}
+
+template <typename T>
+int fv_launch(const int64_t *NL, const T *dE, const T *Ri, const int64_t *frame_start, const int32_t *atom_frame,
+              int64_t nframes, int64_t N, int S, int64_t max_frame, T *F, T *virial, T *work, hipStream_t st) {
+    int rc = fill_async<T>(F, T(0), 3 * N, st);
+    if (rc) return rc;
+    if (N == 0 || S == 0) return fill_async<T>(virial, T(0), 9 * nframes, st);
+    const int64_t rcap = fv_rcap(max_frame, sizeof(T));
+    T *rows = work, *part = work + 9 * N;
+    const int64_t blocks = ceil_div(N, FV_CH);
+    hipLaunchKernelGGL(nep_fv_kernel<T>, dim3((unsigned)blocks), dim3(FV_WAVES * WAVE), 3 * rcap * sizeof(T), st, NL,
+                       dE, Ri, frame_start, atom_frame, N, S, rcap, F, part, rows);
+    if ((rc = check_launch("nep_fv_kernel"))) return rc;
+    hipLaunchKernelGGL(nep_fv_reduce_kernel<T>, dim3(grid_cap(N)), dim3(256), 0, st, frame_start, atom_frame, N, rcap,
+                       part, F);
+    if ((rc = check_launch("nep_fv_reduce_kernel"))) return rc;
+    hipLaunchKernelGGL(nep_virial_reduce_kernel<T>, dim3(9, (unsigned)nframes), dim3(256), 0, st, rows, frame_start,
+                       virial);
+    return check_launch("nep_virial_reduce_kernel");
+}
+
+MATPL_EXPORT int matpl_nep_force_virial_ri(int f64, const int64_t *NL, const void *dE, const void *Ri,
+                                           const int64_t *frame_start, const int32_t *atom_frame, int64_t nframes,
+                                           int64_t N, int S, int64_t max_frame, void *F, void *virial, void *work,
+                                           int64_t work_elems, void *stream) {
+    MATPL_CHECK_ARG(N >= 0 && S >= 0 && nframes >= 1 && max_frame >= 1, "nep_force_virial_ri: bad sizes");
+    MATPL_CHECK_ARG(frame_start && F && virial, "nep_force_virial_ri: null pointer");
+    MATPL_CHECK_ARG(N * S == 0 || (NL && dE && Ri && atom_frame && work), "nep_force_virial_ri: null input");
+    MATPL_CHECK_ARG(aligned16(dE) && aligned16(Ri), "nep_force_virial_ri: dE and Ri must be 16-byte aligned");
+    MATPL_CHECK_ARG(work_elems >= matpl_nep_force_virial_ri_work(f64, N, max_frame),
+                    "nep_force_virial_ri: work buffer too small");
+    hipStream_t st = (hipStream_t)stream;
+    if (f64)
+        return fv_launch<double>(NL, (const double *)dE, (const double *)Ri, frame_start, atom_frame, nframes, N, S,
+                                 max_frame, (double *)F, (double *)virial, (double *)work, st);
+    return fv_launch<float>(NL, (const float *)dE, (const float *)Ri, frame_start, atom_frame, nframes, N, S,
+                            max_frame, (float *)F, (float *)virial, (float *)work, st);
+}
+
+MATPL_EXPORT int matpl_nep_force_virial_ri_grad(int f64, const int64_t *NL, const void *Ri, const void *gF,
+                                                const void *gV, const int32_t *atom_frame, int64_t N, int S, void *g,
+                                                void *stream) {
+    if (N * S == 0) return OK;
+    MATPL_CHECK_ARG(NL && Ri && g && (gF || gV) && (!gV || atom_frame), "nep_force_virial_ri_grad: null pointer");
+    MATPL_CHECK_ARG(aligned16(Ri) && aligned16(g), "nep_force_virial_ri_grad: Ri and g must be 16-byte aligned");
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned blocks = (unsigned)(ceil_div(N, 4) < 8192 ? ceil_div(N, 4) : 8192);
+    if (f64)
+        hipLaunchKernelGGL(nep_fv_grad_kernel<double>, dim3(blocks), dim3(256), 0, st, NL, (const double *)Ri,
+                           (const double *)gF, (const double *)gV, atom_frame, N, S, (double *)g);
+    else
+        hipLaunchKernelGGL(nep_fv_grad_kernel<float>, dim3(blocks), dim3(256), 0, st, NL, (const float *)Ri,
+                           (const float *)gF, (const float *)gV, atom_frame, N, S, (float *)g);
+    return check_launch("nep_fv_grad_kernel");
 }
 
 MATPL_EXPORT int matpl_nep_virial_grad(int f64, const int64_t *NL, const void *Rij, const void *Ri_d, const void *gV,

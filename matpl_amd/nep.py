@@ -3,11 +3,13 @@
 Mirrors src/model/nep_net.py:293-648 (device != cpu): features from
 CalcOps_cuda.calculateNepFeat / calculateNepMbFeat, per-type fitting nets,
 dE = dEtot/dRi by autograd (create_graph, so force losses train the
-parameters), Ri_d from [r, x, y, z] (nep_net.py:440-468), forces and virial
-from calculateNepForce / calculateNepVirial, returned negated as the
-reference does. Where the reference forms the centre force with a batched
-torch.matmul (nep_net.py:608) this uses a HIP reduction; the virial comes
-from the frame-only variant of the virial op (the per-atom rows are unused).
+parameters), forces and virial returned negated as the reference does. The
+reference's sequence per list (Ri_d from [r, x, y, z], nep_net.py:440-468;
+the centre force by a batched torch.matmul, nep_net.py:608;
+calculateNepForce; the frame virial of calculateNepVirial) runs here as one
+kernel that forms Ri_d in registers (nep_ops.nep_force_virial), with its
+backward in one kernel too; the reference's operators themselves stay
+available with their own signatures (nep_ops / calcops).
 Fitting-layer products run tall-by-small or split over atoms (_TallMM), at
 every derivative order.
 
@@ -318,15 +320,13 @@ class NEPTorch(torch.nn.Module):
             g = torch.autograd.grad([feats], ins, grad_outputs=[gq], create_graph=create_graph, allow_unused=True)
         else:
             g = torch.autograd.grad([Etot.sum()], ins, create_graph=create_graph, allow_unused=True)
-        ops = torch.ops.CalcOps_cuda
         Force = Virial = None
         for Ri, NL, dE in zip(ins, (NL_r, NL_a), g):
             if dE is None:
                 continue
-            N, S = NL.shape
-            d = ri_d(Ri.detach())
-            F = ops.calculateNepForce(NL, dE, d, nep_ops.nep_centre_force(dE, d))[0]
-            V = nep_ops.nep_frame_virial(NL, dE, Ri.detach()[..., 1:].contiguous(), d, num_atom)
+            # ri_d, the centre term, calculateNepForce and the frame sum of calculateNepVirial in one
+            # pass, with the position derivative formed from Ri in the kernel
+            F, V = nep_ops.nep_force_virial(NL, dE, Ri, num_atom)
             Force = F if Force is None else Force + F
             Virial = V if Virial is None else Virial + V
         return Etot, Ei, -Force, -Virial

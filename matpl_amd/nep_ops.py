@@ -64,6 +64,7 @@ class StaticFrames:
     start: torch.Tensor      # [F + 1] int64, first atom of each frame
     af: torch.Tensor         # [N] int32, frame of each atom
     status: torch.Tensor     # [1] int32, neighbor-overflow flag of the ops called inside (sticky)
+    max_frame: int = 0       # atoms in the largest frame
 
 
 _STATIC = None
@@ -78,7 +79,8 @@ def static_frames(num_atom: torch.Tensor):
     through the checked ops first and reads ``status`` after the work."""
     global _STATIC
     start, af = _frame_index(num_atom, num_atom.device)
-    ctx = StaticFrames(num_atom, start, af, torch.zeros(1, dtype=torch.int32, device=num_atom.device))
+    ctx = StaticFrames(num_atom, start, af, torch.zeros(1, dtype=torch.int32, device=num_atom.device),
+                       max(_int_max(num_atom), 1))
     prev = _STATIC
     _STATIC = ctx
     try:
@@ -341,6 +343,60 @@ class NepFrameVirial(torch.autograd.Function):
 
 def nep_frame_virial(NL, dE, Rij, Ri_d, num_atom):
     return NepFrameVirial.apply(NL, dE, Rij, Ri_d, num_atom)
+
+
+class NepForceVirialRi(torch.autograd.Function):
+    """The NEP model's force and frame virial of one list from dE and Ri = [r, x, y, z], with the position
+    derivative formed in the kernel: the sequence ri_d (src/model/nep_net.py:440-468), centre term
+    (nep_net.py:608), calculateNepForce (CalcOps.cpp:75-158) and the frame sum of calculateNepVirial
+    (CalcOps.cpp:239-328, nep_net.py:613-622) in one pass. Returns F [N, 3] (whole force, the
+    reference's sign before the model's negation) and V [frames, 9]; differentiable in dE only (Ri is
+    taken as a constant, as the model detaches it for Ri_d)."""
+
+    @staticmethod
+    def forward(ctx, NL, dE, Ri, num_atom):
+        NL = NL.to(torch.int64).contiguous()
+        dE = dE.contiguous()
+        Ri = Ri.detach().to(dE.dtype).contiguous()
+        N, S = NL.shape
+        _expect(dE, N * S * 4, "dE [N, S, 4]")
+        _expect(Ri, N * S * 4, "Ri [N, S, 4]")
+        _check_list(NL, N)
+        if _STATIC is not None and num_atom is _STATIC.num_atom:
+            maxf = _STATIC.max_frame
+        else:
+            if _int_sum(num_atom) != N:
+                raise ValueError(f"nep force/virial: num_atom sums to {_int_sum(num_atom)}, list has {N} atoms")
+            maxf = max(_int_max(num_atom), 1)
+        start, af = _frame_index(num_atom, dE.device)
+        nf = start.numel() - 1
+        F = torch.empty((N, 3), dtype=dE.dtype, device=dE.device)
+        V = torch.empty((nf, 9), dtype=dE.dtype, device=dE.device)
+        work = torch.empty(max(_lib.query("matpl_nep_force_virial_ri_work", _f64(dE), N, maxf), 1), dtype=dE.dtype,
+                           device=dE.device)
+        _lib.call("matpl_nep_force_virial_ri", _f64(dE), _ptr(NL), _ptr(dE), _ptr(Ri), _ptr(start), _ptr(af), nf, N,
+                  S, maxf, _ptr(F), _ptr(V), _ptr(work), work.numel(), _stream())
+        ctx.save_for_backward(NL, Ri, af)
+        ctx.dE_shape = dE.shape
+        return F, V
+
+    @staticmethod
+    def backward(ctx, gF, gV):
+        NL, Ri, af = ctx.saved_tensors
+        N, S = NL.shape
+        if gF is None and gV is None:
+            return None, None, None, None
+        g = torch.empty(ctx.dE_shape, dtype=Ri.dtype, device=Ri.device)
+        gF = None if gF is None else gF.to(Ri.dtype).contiguous()
+        gV = None if gV is None else gV.to(Ri.dtype).contiguous()
+        _lib.call("matpl_nep_force_virial_ri_grad", _f64(Ri), _ptr(NL), _ptr(Ri), _ptr(gF), _ptr(gV), _ptr(af), N, S,
+                  _ptr(g), _stream())
+        return None, g, None, None
+
+
+def nep_force_virial(NL, dE, Ri, num_atom):
+    """(F, V) of one neighbor list from dE and Ri; see NepForceVirialRi."""
+    return NepForceVirialRi.apply(NL, dE, Ri, num_atom)
 
 
 def nep_centre_force(dE, Ri_d):

@@ -226,6 +226,66 @@ def test_frame_virial_many_frames(dev):
         assert torch.allclose(got.double().cpu(), want, rtol=tol, atol=tol * want.abs().max().item())
 
 
+def _framed_lists(dev, sizes, S, seed):
+    """Random lists whose entries stay in their atom's frame (-1 padded at the tail), with Ri rows
+    [r, x, y, z] and one used entry per frame of 2+ atoms at |r| below 1e-5 (its Ri_d is zero)."""
+    g = torch.Generator().manual_seed(seed)
+    N = sum(sizes)
+    start = np.cumsum([0] + sizes)
+    fr = np.repeat(np.arange(len(sizes)), sizes)
+    lo = torch.as_tensor(start[fr]).unsqueeze(1)
+    width = torch.as_tensor(np.maximum(np.array(sizes)[fr], 1)).unsqueeze(1)
+    NL = lo + torch.randint(0, 1 << 20, (N, S), generator=g) % width
+    cnt = torch.randint(0, S + 1, (N,), generator=g)
+    NL[torch.arange(S)[None, :] >= cnt[:, None]] = -1
+    xyz = torch.randn(N, S, 3, generator=g, dtype=torch.float64) * 2.0
+    Ri = torch.cat([xyz.norm(dim=-1, keepdim=True), xyz], -1)
+    for f in range(len(sizes)):
+        if sizes[f] >= 2 and cnt[start[f]] > 0:
+            Ri[start[f], 0] = torch.tensor([5e-6, 3e-6, 4e-6, 0.0], dtype=torch.float64)
+    Ri[NL < 0] = 0
+    dE = torch.randn(N, S, 4, generator=g, dtype=torch.float64) * (NL >= 0).unsqueeze(-1)
+    return NL.to(dev), Ri.to(dev), dE.to(dev), torch.tensor(sizes, device=dev)
+
+
+@pytest.mark.parametrize("sizes,S", [([3, 0, 1, 70, 200, 9, 64, 65], 24), ([3000, 5, 40], 12)])
+def test_fused_force_virial_matches_op_sequence(dev, sizes, S):
+    """nep_force_virial (Ri_d formed in the kernel, LDS force copies per block; with a 3,000-atom
+    frame the float64 copy exceeds its 64 KB and those blocks add to global memory) against the
+    reference's sequence ri_d -> centre term -> calculateNepForce -> calculateNepVirial frame sums,
+    values and dE gradients. FP64 within 1e-12 relative, FP32 within 1e-5."""
+    NL, Ri, dE, num_atom = _framed_lists(dev, sizes, S, seed=len(sizes))
+    N = NL.shape[0]
+    print(f"case fused force/virial sizes={sizes} S={S}", flush=True)
+    gF = torch.randn(N, 3, dtype=torch.float64, device=dev)
+    gV = torch.randn(len(sizes), 9, dtype=torch.float64, device=dev)
+
+    def seq(d_):
+        # the op's backward carries the centre term, so the centre force enters as a constant
+        Rid = mn.ri_d(Ri)
+        centre = -torch.matmul(d_.detach().reshape(N, 1, -1), Rid.reshape(N, -1, 3)).squeeze(-2)
+        F = nep_ops.NepForce.apply(NL, d_, Rid, centre)
+        V = nep_ops.nep_frame_virial(NL, d_, Ri[..., 1:].contiguous(), Rid, num_atom)
+        return F, V
+
+    d64 = dE.clone().requires_grad_()
+    Fw, Vw = seq(d64)
+    gw = torch.autograd.grad((Fw * gF).sum() + (Vw * gV).sum(), d64)[0]
+    for dt, tol in ((torch.float64, 1e-12), (torch.float32, 1e-5)):
+        d = dE.to(dt).clone().requires_grad_()
+        F, V = nep_ops.nep_force_virial(NL, d, Ri.to(dt), num_atom)
+        g = torch.autograd.grad((F * gF.to(dt)).sum() + (V * gV.to(dt)).sum(), d)[0]
+        for got, want in ((F, Fw), (V, Vw), (g, gw)):
+            err = (got.double() - want).abs().max().item()
+            assert err <= tol * max(want.abs().max().item(), 1.0), (dt, err)
+    # upstream gradient of F only, and of V only
+    d = dE.clone().requires_grad_()
+    F, V = nep_ops.nep_force_virial(NL, d, Ri, num_atom)
+    gf_only = torch.autograd.grad((F * gF).sum(), d, retain_graph=True)[0]
+    gv_only = torch.autograd.grad((V * gV).sum(), d)[0]
+    assert torch.allclose(gf_only + gv_only, gw, rtol=1e-12, atol=1e-12)
+
+
 @pytest.mark.parametrize("lmax", [(4, 2, 0), (4, 2, 1)])
 def test_model_energy_force_virial(dev, lmax):
     frames = _frames()
