@@ -668,40 +668,73 @@ nep_angular_fwd_kernel(FeatArgs a) {
 }
 
 // ------------------------------------------------------------------ force / virial ops
-// calculateNepForce: F[i] += ... in place: F[j] += dE . Ri_d for every slot j >= 0
-// (calculateNepForce.cu:25-70). dE [N][S][4], Ri_d [N][S][4][3], NL int64 [N][S].
-template <typename T>
-__global__ void nep_force_kernel(const int64_t *__restrict__ NL, const T *__restrict__ dE, const T *__restrict__ Ri_d,
-                                 T *__restrict__ F, int64_t nslots) {
-    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nslots;
-         s += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = NL[s];
-        if (j < 0) continue;
-        const T *e = dE + 4 * s, *d = Ri_d + 12 * s;
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-            atomic_add(F + 3 * j + a, e[0] * d[a] + e[1] * d[3 + a] + e[2] * d[6 + a] + e[3] * d[9 + a]);
-    }
-}
+// Neighbor terms of calculateNepForce (NV = 3: out[j] += f, calculateNepForce.cu:25-70) and of
+// calculateNepVirial (NV = 9: out[j][3a+c] += f_a Rij[s][c], calculateNepVirial.cu:19-96), with
+// f = sum_w dE[s][w] Ri_d[s][w][:] for every slot s whose NL[s] = j >= 0. dE [N][S][4],
+// Ri_d [N][S][4][3], Rij [N][S][3], NL int64 [N][S]; out is accumulated into.
+// A block takes SC_CH consecutive atoms: a first pass over their slots finds the range [lo, hi) of
+// the atoms they name, the terms are added into an LDS copy of out over that range, and the copy's
+// nonzero entries are added to out once per block. Per-slot global atomics, which on this GPU are
+// carried out beyond the per-XCD L2 caches, remain only for blocks whose range exceeds the copy.
+constexpr int SC_CH = 64;       // atoms per block
 
-// calculateNepVirial: atom_virial[j][3a+c] += (dE.Ri_d)_a Rij_c; virial per frame
-// (calculateNepVirial.cu:19-96).
-template <typename T>
-__global__ void nep_virial_kernel(const int64_t *__restrict__ NL, const T *__restrict__ dE, const T *__restrict__ Rij,
-                                  const T *__restrict__ Ri_d, T *__restrict__ atom_virial, int64_t nslots) {
-    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nslots;
-         s += (int64_t)gridDim.x * blockDim.x) {
+template <typename T, int NV>
+__global__ void __launch_bounds__(256)
+nep_scatter_kernel(const int64_t *__restrict__ NL, const T *__restrict__ dE, const T *__restrict__ Ri_d,
+                   const T *__restrict__ Rij, T *__restrict__ out, int64_t N, int S, int64_t rcap) {
+    extern __shared__ double sc_smem[];
+    T *acc = reinterpret_cast<T *>(sc_smem);
+    __shared__ int64_t wlo[4], whi[4];
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int64_t i0 = blockIdx.x * (int64_t)SC_CH, i1 = i0 + SC_CH < N ? i0 + SC_CH : N;
+    const int64_t s0 = i0 * S, s1 = i1 * S;
+    int64_t mn = INT64_MAX, mx = -1;
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+        const int64_t j = NL[s];
+        if (j >= 0) { mn = j < mn ? j : mn; mx = j > mx ? j : mx; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int64_t a = __shfl_xor(mn, off, WAVE), b = __shfl_xor(mx, off, WAVE);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    if (lane == 0) { wlo[wid] = mn; whi[wid] = mx; }
+    __syncthreads();
+    mn = wlo[0]; mx = whi[0];
+    for (int w = 1; w < 4; w++) { mn = wlo[w] < mn ? wlo[w] : mn; mx = whi[w] > mx ? whi[w] : mx; }
+    const int64_t lo = mn, hi = mx + 1;
+    const bool lds = mx >= 0 && hi - lo <= rcap;
+    if (lds)
+        for (int64_t r = threadIdx.x; r < NV * (hi - lo); r += blockDim.x) acc[r] = T(0);
+    __syncthreads();
+    for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
         const int64_t j = NL[s];
         if (j < 0) continue;
         const T *e = dE + 4 * s, *d = Ri_d + 12 * s;
-        T f[3];
+        T v[NV];
 #pragma unroll
-        for (int a = 0; a < 3; a++) f[a] = e[0] * d[a] + e[1] * d[3 + a] + e[2] * d[6 + a] + e[3] * d[9 + a];
+        for (int a = 0; a < 3; a++) {
+            const T f = e[0] * d[a] + e[1] * d[3 + a] + e[2] * d[6 + a] + e[3] * d[9 + a];
+            if constexpr (NV == 3) {
+                v[a] = f;
+            } else {
 #pragma unroll
-        for (int a = 0; a < 3; a++)
+                for (int c = 0; c < 3; c++) v[3 * a + c] = f * Rij[3 * s + c];
+            }
+        }
 #pragma unroll
-            for (int cc = 0; cc < 3; cc++) atomic_add(atom_virial + 9 * j + 3 * a + cc, f[a] * Rij[3 * s + cc]);
+        for (int c = 0; c < NV; c++) {
+            if (lds) atomicAdd(acc + NV * (j - lo) + c, v[c]);
+            else atomic_add(out + NV * j + c, v[c]);
+        }
     }
+    __syncthreads();
+    if (lds)
+        for (int64_t r = threadIdx.x; r < NV * (hi - lo); r += blockDim.x) {
+            const T x = acc[r];
+            if (x != T(0)) atomic_add(out + NV * lo + r, x);
+        }
 }
 
 // Centre term of the NEP force, F[i] = -sum_{k, w} dE[i][k][w] Ri_d[i][k][w][:],
@@ -793,7 +826,7 @@ __global__ void __launch_bounds__(256) nep_virial_reduce_kernel(const T *__restr
 // Unlike per-slot global atomics, which on this GPU are carried out beyond the per-XCD L2 caches, the
 // LDS adds stay on the CU.
 constexpr int FV_CH = 64;       // atoms per block
-constexpr int FV_WAVES = 4;
+constexpr int FV_WAVES = 16;    // one LDS copy shared by many wavefronts, each with few atoms
 
 template <typename T>
 struct V4 { T x, y, z, w; };
@@ -836,20 +869,36 @@ nep_fv_kernel(const int64_t *__restrict__ NL, const T *__restrict__ dE, const T 
         T c[3] = {T(0), T(0), T(0)}, v[9];
 #pragma unroll
         for (int q = 0; q < 9; q++) v[q] = T(0);
-        for (int k = lane; k < S; k += WAVE) {
-            const int64_t s = i * S + k;
-            const int64_t j = NL[s];
-            const V4<T> q = ld4(Ri + 4 * s), e = ld4(dE + 4 * s);
-            if (j < 0 || !(fabs(q.x) > T(1e-5))) continue;
-            const T f[3] = {e.x * (q.y / q.x) + e.y, e.x * (q.z / q.x) + e.z, e.x * (q.w / q.x) + e.w};
-            const T x[3] = {q.y, q.z, q.w};
+        for (int k0 = 0; k0 < S; k0 += 2 * WAVE) {
+            // the loads of two passes are issued before either is used
+            int64_t jj[2];
+            V4<T> qq[2], ee[2];
 #pragma unroll
-            for (int a = 0; a < 3; a++) {
-                c[a] += f[a];
+            for (int u = 0; u < 2; u++) {
+                const int k = k0 + u * WAVE + lane;
+                jj[u] = -1;
+                if (k < S) {
+                    const int64_t s = i * S + k;
+                    jj[u] = NL[s];
+                    qq[u] = ld4(Ri + 4 * s);
+                    ee[u] = ld4(dE + 4 * s);
+                }
+            }
 #pragma unroll
-                for (int cc = 0; cc < 3; cc++) v[3 * a + cc] += f[a] * x[cc];
-                if (lds && j >= lo && j < hi) atomicAdd(acc + 3 * (j - lo) + a, f[a]);
-                else atomic_add(F + 3 * j + a, f[a]);
+            for (int u = 0; u < 2; u++) {
+                const int64_t j = jj[u];
+                const V4<T> q = qq[u], e = ee[u];
+                if (j < 0 || !(fabs(q.x) > T(1e-5))) continue;
+                const T f[3] = {e.x * (q.y / q.x) + e.y, e.x * (q.z / q.x) + e.z, e.x * (q.w / q.x) + e.w};
+                const T x[3] = {q.y, q.z, q.w};
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    c[a] += f[a];
+#pragma unroll
+                    for (int cc = 0; cc < 3; cc++) v[3 * a + cc] += f[a] * x[cc];
+                    if (lds && j >= lo && j < hi) atomicAdd(acc + 3 * (j - lo) + a, f[a]);
+                    else atomic_add(F + 3 * j + a, f[a]);
+                }
             }
         }
 #pragma unroll
@@ -1429,11 +1478,17 @@ int feat_launch(FeatArgs &a, int angular, int mode, hipStream_t st) {
     return check_launch("nep_angular_fwd_kernel");
 }
 
-template <typename T>
-int force_launch(const int64_t *NL, const T *dE, const T *Ri_d, T *F, int64_t nslots, hipStream_t st) {
-    if (nslots == 0) return OK;
-    hipLaunchKernelGGL(nep_force_kernel<T>, dim3(grid_cap(nslots)), dim3(256), 0, st, NL, dE, Ri_d, F, nslots);
-    return check_launch("nep_force_kernel");
+template <typename T, int NV>
+int scatter_launch(const int64_t *NL, const T *dE, const T *Ri_d, const T *Rij, T *out, int64_t N, int S,
+                   hipStream_t st) {
+    if (N * S == 0) return OK;
+    // LDS copy of at most 63 KB, so that with the kernel's own 64 bytes the block stays within 64 KB
+    // (and of no more atoms than there are)
+    int64_t rcap = 64512 / (NV * (int64_t)sizeof(T));
+    if (rcap > N) rcap = N;
+    hipLaunchKernelGGL((nep_scatter_kernel<T, NV>), dim3(ceil_div(N, SC_CH)), dim3(256), NV * rcap * sizeof(T), st, NL,
+                       dE, Ri_d, Rij, out, N, S, rcap);
+    return check_launch("nep_scatter_kernel");
 }
 
 template <typename T>
@@ -1441,11 +1496,7 @@ int virial_launch(const int64_t *NL, const T *dE, const T *Rij, const T *Ri_d, c
                   int64_t nframes, int64_t N, int S, T *virial, T *atom_virial, hipStream_t st) {
     int rc = fill_async<T>(atom_virial, T(0), N * 9, st);
     if (rc) return rc;
-    if (N * S > 0) {
-        hipLaunchKernelGGL(nep_virial_kernel<T>, dim3(grid_cap(N * S)), dim3(256), 0, st, NL, dE, Rij, Ri_d,
-                           atom_virial, N * S);
-        if ((rc = check_launch("nep_virial_kernel"))) return rc;
-    }
+    if ((rc = scatter_launch<T, 9>(NL, dE, Ri_d, Rij, atom_virial, N, S, st))) return rc;
     if (nframes == 0) return OK;
     hipLaunchKernelGGL(nep_virial_reduce_kernel<T>, dim3(9, (unsigned)nframes), dim3(256), 0, st, atom_virial,
                        frame_start, virial);
@@ -1553,8 +1604,9 @@ MATPL_EXPORT int matpl_nep_force(int f64, const int64_t *NL, const void *dE, con
                                  void *stream) {
     MATPL_CHECK_ARG(N == 0 || (NL && dE && Ri_d && F), "nep_force: null pointer");
     hipStream_t st = (hipStream_t)stream;
-    if (f64) return force_launch<double>(NL, (const double *)dE, (const double *)Ri_d, (double *)F, N * S, st);
-    return force_launch<float>(NL, (const float *)dE, (const float *)Ri_d, (float *)F, N * S, st);
+    if (f64)
+        return scatter_launch<double, 3>(NL, (const double *)dE, (const double *)Ri_d, nullptr, (double *)F, N, S, st);
+    return scatter_launch<float, 3>(NL, (const float *)dE, (const float *)Ri_d, nullptr, (float *)F, N, S, st);
 }
 
 MATPL_EXPORT int matpl_nep_virial(int f64, const int64_t *NL, const void *dE, const void *Rij, const void *Ri_d,

@@ -204,6 +204,39 @@ def test_force_virial_ops_and_grads(dev):
     assert torch.allclose(gotf.cpu(), wantv, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("N", [600, 2000])
+def test_force_virial_ops_wide_lists(dev, N):
+    """calculateNepForce / calculateNepVirial on lists naming atoms anywhere in the batch: per block of
+    atoms the named range is up to N wide, so the LDS copies hold it (600 atoms) or, for the virial at
+    2,000 atoms, exceed their 64 KB and the block adds to global memory. Against index_add on the CPU;
+    FP64 within 1e-12, FP32 within 1e-5 relative."""
+    S = 16
+    g = torch.Generator().manual_seed(N)
+    NL = torch.randint(0, N, (N, S), generator=g)
+    NL[torch.rand(N, S, generator=g) < 0.2] = -1
+    Ri = torch.cat([torch.rand(N, S, 1, generator=g, dtype=torch.float64) + 1.0,
+                    torch.randn(N, S, 3, generator=g, dtype=torch.float64)], -1)
+    Rid = mn.ri_d(Ri)
+    dE = torch.randn(N, S, 4, generator=g, dtype=torch.float64)
+    F0 = torch.randn(N, 3, generator=g, dtype=torch.float64)
+    m = NL >= 0
+    pair = torch.einsum("nsk,nska->nsa", dE, Rid)[m]
+    j = NL[m]
+    Fw = F0.clone().index_add_(0, j, pair)
+    AVw = torch.zeros(N, 9, dtype=torch.float64).index_add_(
+        0, j, (pair.unsqueeze(-1) * Ri[..., 1:][m].unsqueeze(-2)).reshape(-1, 9))
+    print(f"case wide lists N={N}", flush=True)
+    for dt, tol in ((torch.float64, 1e-12), (torch.float32, 1e-5)):
+        T = lambda x: x.to(dt).to(dev)
+        NLd = NL.to(dev)
+        F = nep_ops.calculateNepForce(NLd, T(dE), T(Rid), T(F0))[0]
+        V, AV = nep_ops.calculateNepVirial(NLd, T(dE), T(Ri[..., 1:].contiguous()), T(Rid),
+                                           torch.tensor([N], device=dev))
+        for got, want in ((F, Fw), (AV, AVw), (V, AVw.sum(0, keepdim=True))):
+            err = (got.double().cpu() - want).abs().max().item()
+            assert err <= tol * want.abs().max().item(), (dt, err)
+
+
 def test_frame_virial_many_frames(dev):
     """Frames shorter and longer than one wavefront's run of atoms, and an empty frame."""
     _, Ri, NL, _ = _rand_inputs(dev, N=40, S=70, seed=7)
