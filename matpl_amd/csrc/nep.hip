@@ -818,15 +818,17 @@ __global__ void __launch_bounds__(256) nep_virial_reduce_kernel(const T *__restr
 // is formed in registers, so f(s) = dE[s][0] (x, y, z)/r + dE[s][1:4] per used slot s of atom i.
 // calculateNepForce's terms (CalcOps.cpp:75-158): F[i] -= f(s), F[j] += f(s); calculateNepVirial's
 // frame sum (calculateNepVirial.cu:19-96): V[frame] += f(s) (x) Rij[s].
-// A block takes FV_CH consecutive atoms i and adds the force terms of their slots into an LDS copy of
-// F over the frames those atoms belong to (every j of a slot lies in its atom's frame), then writes
-// that copy to its own row of `part`; fv_reduce_kernel adds the rows of the blocks covering each
-// atom. A block whose frames exceed the LDS copy adds its terms to F with global atomics instead, as
-// does any entry naming an atom outside its block's frames.
+// A block takes `ch` consecutive atoms i (one wavefront per atom at a time). With LDS copies
+// (rcap > 0) it adds the neighbor terms of their slots into an LDS copy of F over the frames those
+// atoms belong to (every j of a slot lies in its atom's frame), then writes that copy to its own row
+// of `part`; fv_reduce_kernel adds the rows of the blocks covering each atom. A block whose frames
+// exceed the LDS copy adds its terms to F with global atomics instead, as does any entry naming an
+// atom outside its block's frames, and so does every block without LDS copies (rcap = 0).
 // Unlike per-slot global atomics, which on this GPU are carried out beyond the per-XCD L2 caches, the
-// LDS adds stay on the CU.
-constexpr int FV_CH = 64;       // atoms per block
-constexpr int FV_WAVES = 16;    // one LDS copy shared by many wavefronts, each with few atoms
+// LDS adds stay on the CU; the copies pay when many small frames fill the GPU with blocks (fv_plan).
+constexpr int FV_CH = 64;        // atoms per block with LDS copies
+constexpr int FV_CH_DIRECT = 16; // atoms per block without
+constexpr int FV_WAVES = 16;     // one LDS copy shared by many wavefronts, each with few atoms
 
 template <typename T>
 struct V4 { T x, y, z, w; };
@@ -844,8 +846,8 @@ __device__ __forceinline__ V4<T> ld4(const T *p) {
 
 // the frames of atoms [i0, i1): [lo, hi)
 __device__ __forceinline__ void fv_range(const int64_t *frame_start, const int32_t *atom_frame, int64_t N, int64_t b,
-                                         int64_t &lo, int64_t &hi) {
-    const int64_t i0 = b * FV_CH, i1 = i0 + FV_CH < N ? i0 + FV_CH : N;
+                                         int ch, int64_t &lo, int64_t &hi) {
+    const int64_t i0 = b * ch, i1 = i0 + ch < N ? i0 + ch : N;
     lo = frame_start[atom_frame[i0]];
     hi = frame_start[atom_frame[i1 - 1] + 1];
 }
@@ -854,13 +856,13 @@ template <typename T>
 __global__ void __launch_bounds__(FV_WAVES * WAVE)
 nep_fv_kernel(const int64_t *__restrict__ NL, const T *__restrict__ dE, const T *__restrict__ Ri,
               const int64_t *__restrict__ frame_start, const int32_t *__restrict__ atom_frame, int64_t N, int S,
-              int64_t rcap, T *__restrict__ F, T *__restrict__ part, T *__restrict__ rows) {
+              int ch, int64_t rcap, T *__restrict__ F, T *__restrict__ part, T *__restrict__ rows) {
     extern __shared__ double fv_smem[];
     T *acc = reinterpret_cast<T *>(fv_smem);
     const int lane = threadIdx.x & (WAVE - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-    const int64_t b = blockIdx.x, i0 = b * FV_CH, i1 = i0 + FV_CH < N ? i0 + FV_CH : N;
+    const int64_t b = blockIdx.x, i0 = b * ch, i1 = i0 + ch < N ? i0 + ch : N;
     int64_t lo, hi;
-    fv_range(frame_start, atom_frame, N, b, lo, hi);
+    fv_range(frame_start, atom_frame, N, b, ch, lo, hi);
     const bool lds = hi - lo <= rcap;
     if (lds)
         for (int64_t r = threadIdx.x; r < 3 * (hi - lo); r += blockDim.x) acc[r] = T(0);
@@ -924,15 +926,15 @@ nep_fv_kernel(const int64_t *__restrict__ NL, const T *__restrict__ dE, const T 
 template <typename T>
 __global__ void __launch_bounds__(256)
 nep_fv_reduce_kernel(const int64_t *__restrict__ frame_start, const int32_t *__restrict__ atom_frame, int64_t N,
-                     int64_t rcap, const T *__restrict__ part, T *__restrict__ F) {
+                     int ch, int64_t rcap, const T *__restrict__ part, T *__restrict__ F) {
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < N; j += (int64_t)gridDim.x * blockDim.x) {
         const int f = atom_frame[j];
         const int64_t fs = frame_start[f], fe = frame_start[f + 1];
         if (fe - fs > rcap) continue;             // no block covering this frame used LDS
         T s[3] = {T(0), T(0), T(0)};
-        for (int64_t b = fs / FV_CH; b <= (fe - 1) / FV_CH; b++) {
+        for (int64_t b = fs / ch; b <= (fe - 1) / ch; b++) {
             int64_t lo, hi;
-            fv_range(frame_start, atom_frame, N, b, lo, hi);
+            fv_range(frame_start, atom_frame, N, b, ch, lo, hi);
             if (hi - lo > rcap) continue;
             const T *p = part + b * 3 * rcap + 3 * (j - lo);
             s[0] += p[0]; s[1] += p[1]; s[2] += p[2];
@@ -1636,18 +1638,25 @@ MATPL_EXPORT int matpl_nep_force_grad(int f64, const int64_t *NL, const void *Ri
     return check_launch("nep_force_grad_kernel");
 }
 
-// LDS copy of F per block: at most 64 KB, and no more than the largest span of frames a block of
-// FV_CH atoms can touch (two partial frames and whole ones in between)
-inline int64_t fv_rcap(int64_t max_frame, size_t tsize) {
+// LDS copies of F (blocks of FV_CH atoms) when the blocks fill the GPU and every copy spans at most
+// the largest span of frames a block can touch (two partial frames and whole ones in between) within
+// 64 KB; otherwise blocks of FV_CH_DIRECT atoms and global atomics (a few large frames: a block's
+// slots name few of a frame's atoms, and few blocks would share many copies per atom).
+struct FvPlan {
+    int ch;
+    int64_t rcap;
+};
+inline FvPlan fv_plan(int64_t N, int64_t max_frame, size_t tsize) {
     const int64_t cap = (int64_t)(65536 / (3 * tsize));
     const int64_t want = 2 * max_frame + FV_CH;
-    return want < cap ? want : cap;
+    if (N >= 256 * (int64_t)FV_CH && max_frame <= 2048 && want <= cap) return {FV_CH, want};
+    return {FV_CH_DIRECT, 0};
 }
 
 MATPL_EXPORT int64_t matpl_nep_force_virial_ri_work(int f64, int64_t N, int64_t max_frame) {
     if (N <= 0) return 0;
-    const int64_t rcap = fv_rcap(max_frame, f64 ? 8 : 4);
-    return 9 * N + (int64_t)ceil_div(N, FV_CH) * 3 * rcap;
+    const FvPlan pl = fv_plan(N, max_frame, f64 ? 8 : 4);
+    return 9 * N + (int64_t)ceil_div(N, pl.ch) * 3 * pl.rcap;
 }
 
 template <typename T>
@@ -1656,15 +1665,17 @@ int fv_launch(const int64_t *NL, const T *dE, const T *Ri, const int64_t *frame_
     int rc = fill_async<T>(F, T(0), 3 * N, st);
     if (rc) return rc;
     if (N == 0 || S == 0) return fill_async<T>(virial, T(0), 9 * nframes, st);
-    const int64_t rcap = fv_rcap(max_frame, sizeof(T));
+    const FvPlan pl = fv_plan(N, max_frame, sizeof(T));
     T *rows = work, *part = work + 9 * N;
-    const int64_t blocks = ceil_div(N, FV_CH);
-    hipLaunchKernelGGL(nep_fv_kernel<T>, dim3((unsigned)blocks), dim3(FV_WAVES * WAVE), 3 * rcap * sizeof(T), st, NL,
-                       dE, Ri, frame_start, atom_frame, N, S, rcap, F, part, rows);
+    const int64_t blocks = ceil_div(N, pl.ch);
+    hipLaunchKernelGGL(nep_fv_kernel<T>, dim3((unsigned)blocks), dim3(FV_WAVES * WAVE), 3 * pl.rcap * sizeof(T), st,
+                       NL, dE, Ri, frame_start, atom_frame, N, S, pl.ch, pl.rcap, F, part, rows);
     if ((rc = check_launch("nep_fv_kernel"))) return rc;
-    hipLaunchKernelGGL(nep_fv_reduce_kernel<T>, dim3(grid_cap(N)), dim3(256), 0, st, frame_start, atom_frame, N, rcap,
-                       part, F);
-    if ((rc = check_launch("nep_fv_reduce_kernel"))) return rc;
+    if (pl.rcap > 0) {
+        hipLaunchKernelGGL(nep_fv_reduce_kernel<T>, dim3(grid_cap(N)), dim3(256), 0, st, frame_start, atom_frame, N,
+                           pl.ch, pl.rcap, part, F);
+        if ((rc = check_launch("nep_fv_reduce_kernel"))) return rc;
+    }
     hipLaunchKernelGGL(nep_virial_reduce_kernel<T>, dim3(9, (unsigned)nframes), dim3(256), 0, st, rows, frame_start,
                        virial);
     return check_launch("nep_virial_reduce_kernel");

@@ -281,12 +281,13 @@ def _framed_lists(dev, sizes, S, seed):
     return NL.to(dev), Ri.to(dev), dE.to(dev), torch.tensor(sizes, device=dev)
 
 
-@pytest.mark.parametrize("sizes,S", [([3, 0, 1, 70, 200, 9, 64, 65], 24), ([3000, 5, 40], 12)])
+@pytest.mark.parametrize("sizes,S", [([3, 0, 1, 70, 200, 9, 64, 65], 24), ([3000, 5, 40], 12),
+                                     ([420, 3, 0, 77] * 10 + [420] * 30, 8)])
 def test_fused_force_virial_matches_op_sequence(dev, sizes, S):
-    """nep_force_virial (Ri_d formed in the kernel, LDS force copies per block; with a 3,000-atom
-    frame the float64 copy exceeds its 64 KB and those blocks add to global memory) against the
-    reference's sequence ri_d -> centre term -> calculateNepForce -> calculateNepVirial frame sums,
-    values and dE gradients. FP64 within 1e-12 relative, FP32 within 1e-5."""
+    """nep_force_virial (Ri_d formed in the kernel) against the reference's sequence ri_d -> centre
+    term -> calculateNepForce -> calculateNepVirial frame sums, values and dE gradients: few atoms or
+    a large frame (global atomics per entry), and 17,600 atoms in frames of up to 420 (per-block LDS
+    copies of F and their reduction). FP64 within 1e-12 relative, FP32 within 1e-5."""
     NL, Ri, dE, num_atom = _framed_lists(dev, sizes, S, seed=len(sizes))
     N = NL.shape[0]
     print(f"case fused force/virial sizes={sizes} S={S}", flush=True)
