@@ -309,7 +309,11 @@ class NEPTorch(torch.nn.Module):
             Ei = self.energies(feats, atom_type_map, type_index)
         nf = num_atom.numel()
         frame = nep_ops._frame_index(num_atom, Ei.device)[1]
-        Etot = torch.zeros(nf, dtype=Ei.dtype, device=Ei.device).index_add(0, frame, Ei).unsqueeze(-1)
+        if nf == 1:
+            # one frame: a plain reduction (index_add would send every atom's atomic add to one address)
+            Etot = Ei.sum().reshape(1, 1)
+        else:
+            Etot = torch.zeros(nf, dtype=Ei.dtype, device=Ei.device).index_add(0, frame, Ei).unsqueeze(-1)
         if not is_calc_f:
             return Etot, Ei, None, None
         if create_graph is None:
