@@ -50,6 +50,20 @@ def test_neighbor_bit_exact(dev):
         assert np.array_equal(g[0].cpu().numpy(), r) and np.array_equal(g[1].cpu().numpy(), a)
 
 
+def test_neighbor_bit_exact_larger_frames(dev):
+    """Frames of 512 and 640 atoms in boxes thicker than 2 rc (one replica: one pass counts and writes,
+    8 and 10 chunks of 64 candidates per atom); lists and Ri bit-identical to the oracle."""
+    frames = [st.zincblende_sic((4, 4, 4), seed=6), st.zincblende_sic((5, 4, 4), seed=7)]
+    pk = on.pack_frames(frames, 6.0)
+    want = on.neighbor(pk, 6.0, 4.0, 200, 60)
+    gp = mn.pack_frames(frames, 6.0, dev)
+    assert int(gp["num_cell"].max()) == 1
+    got = nep_ops.calculate_neighbor(gp["num_atoms"], gp["atom_type_map"], gp["atom_type_map"], gp["box"],
+                                     gp["box_orig"], gp["num_cell"], gp["position"], 6.0, 4.0, 200, 60, True)
+    for name, t in zip(("NN_r", "NN_a", "NL_r", "NL_a", "Ri_r", "Ri_a"), got):
+        assert np.array_equal(t.cpu().numpy(), want[name]), name
+
+
 def test_neighbor_overflow_raises(dev):
     gp = mn.pack_frames([st.zincblende_sic((2, 2, 2), seed=4)], 6.0, dev)
     with pytest.raises(OverflowError):
