@@ -102,31 +102,66 @@ def pack_fused_batch(frames, cfg, device="cuda", seed=0, type_present=None) -> F
 
 class _EmbPack:
     """Gathers the embedding parameters (torch, on the device) into the fused kernels' per-pair
-    operand layouts (matpl_amd.dp_fused.pack_fused's layout, built on the device every step)."""
+    operand layouts (matpl_amd.dp_fused.pack_fused's layout, built on the device every step).
+    The parameters of all pairs are concatenated kind by kind (W1, b1, W2, b2, W3, b3, then G at
+    the empty-slot input, a 0 and a 1), so the stacked per-kind arrays are views of one vector and
+    each packed array is one gather from it through an index map fixed at construction, times a
+    fixed factor (TANH_K on the forward weights and biases in BF16 mode)."""
 
-    def __init__(self, E: int, device):
+    def __init__(self, E: int, P: int, K: float, device):
+        self.E, self.P = E, P
         h = np.repeat([0, 1], 16)
         r = np.tile(np.arange(16), 2)
         f = np.array([feat(rr, hh) for hh, rr in zip(h, r)]).reshape(2, 16)
-        self.fvec = torch.as_tensor(np.where(f < E, f, 0), device=device)
-        self.mvec = torch.as_tensor(f < E, device=device)
+        mv = f < E                                                                  # [2, 16]
+        fv = np.where(mv, f, 0)
         lanes = np.arange(64)
         hl, il = lanes // 32, lanes % 32
         fm = np.array([[feat(rr, hh) for rr in range(16)] for hh in hl])          # [64, 16]
         im = np.repeat(il[:, None], 16, 1)
         ok = (fm < E) & (im < E)
-        self.fm = torch.as_tensor(np.where(ok, fm, 0), device=device)
-        self.im = torch.as_tensor(np.where(ok, im, 0), device=device)
-        self.ok = torch.as_tensor(ok, device=device)
+        fm, im = np.where(ok, fm, 0), np.where(ok, im, 0)
+        PE, PEE = P * E, P * E * E
+        o = {"W1": 0, "b1": PE, "W2": 2 * PE, "b2": 2 * PE + PEE, "W3": 3 * PE + PEE, "b3": 3 * PE + 2 * PEE,
+             "g": 4 * PE + 2 * PEE}
+        self.off = o
+        ZERO, ONE = 5 * PE + 2 * PEE, 5 * PE + 2 * PEE + 1
+        self.nflat = ONE + 1
+        pp = np.arange(P)[:, None, None]
+        iv = np.full((P, NVEC, 2, 16), ZERO, np.int64)
+        kv = np.ones((P, NVEC, 2, 16), np.float32)
+        for slot, key, scaled in ((V_W1, "W1", True), (V_B1, "b1", True), (V_B2, "b2", True), (V_B3, "b3", True),
+                                  (V_GPAD, "g", False)):
+            iv[:, slot] = np.where(mv[None], o[key] + pp * E + fv[None], ZERO)
+            if scaled:
+                kv[:, slot] = K
+        for slot in (V_R2, V_R3):
+            iv[:, slot] = np.where(mv[None], ONE, ZERO)
+        im_ = np.full((P, NMAT, 64, 16), ZERO, np.int64)
+        km = np.ones((P, NMAT, 64, 16), np.float32)
+        for slot, key, fwd, scaled in ((M_W2F, "W2", True, True), (M_W3F, "W3", True, True),
+                                       (M_W2B, "W2", False, False), (M_W3B, "W3", False, False)):
+            row, col = (fm, im) if fwd else (im, fm)                   # fwd: W[f(r,h)][lane % 32]
+            im_[:, slot] = np.where(ok[None], o[key] + pp * E * E + row[None] * E + col[None], ZERO)
+            if scaled:
+                km[:, slot] = K
+        self.i_vec = torch.as_tensor(iv, device=device)
+        self.k_vec = torch.as_tensor(kv, device=device)
+        self.i_mat = torch.as_tensor(im_, device=device)
+        self.k_mat = torch.as_tensor(km, device=device)
+        self.const = torch.tensor([0.0, 1.0], dtype=torch.float32, device=device)
 
-    def vec(self, v: torch.Tensor) -> torch.Tensor:      # [P, E] -> [P, 2, 16]
-        return torch.where(self.mvec, v[:, self.fvec], torch.zeros((), dtype=v.dtype, device=v.device))
+    def stacked(self, flat):
+        """W1, b1, W2, b2, W3, b3 of all pairs ([P, E] / [P, E, E]) as views of the concatenation."""
+        P, E, o = self.P, self.E, self.off
+        v = lambda k, n: flat[o[k]:o[k] + n]
+        return (v("W1", P * E).view(P, E), v("b1", P * E).view(P, E), v("W2", P * E * E).view(P, E, E),
+                v("b2", P * E).view(P, E), v("W3", P * E * E).view(P, E, E), v("b3", P * E).view(P, E))
 
-    def fwd(self, W):   # [P, E, E] -> [P, 64, 16]: [lane][r] = W[f(r,h)][lane % 32]
-        return torch.where(self.ok, W[:, self.fm, self.im], torch.zeros((), dtype=W.dtype, device=W.device))
-
-    def bwd(self, W):   # [lane][r] = W[lane % 32][f(r,h)]
-        return torch.where(self.ok, W[:, self.im, self.fm], torch.zeros((), dtype=W.dtype, device=W.device))
+    def pack(self, flat, gpad):
+        """(vec [P, NVEC, 2, 16], mat [P, NMAT, 64, 16]) in float32."""
+        full = torch.cat([flat, gpad.reshape(-1), self.const])
+        return full[self.i_vec] * self.k_vec, full[self.i_mat] * self.k_mat
 
 
 class _FitPack:
@@ -254,7 +289,7 @@ class DPTrainFused(torch.nn.Module):
             tinfo[ti, 12:16] = ri_pad[ti]
         self.register_buffer("tinfo", torch.as_tensor(tinfo, device=dev))
         self.register_buffer("ri_pad", torch.as_tensor(ri_pad, dtype=f32, device=dev))
-        self._pk = _EmbPack(self.E, dev)
+        self._pk = _EmbPack(self.E, T * T, TANH_K if self.bf16 else 1.0, dev)
         self.gsize = _lib.query("matpl_dp_train_grad_size", self.E)
         self.chunk = 0          # atoms per wavefront of the gradient kernel (0: chosen from the batch)
         fit0 = w.fit[0]
@@ -269,16 +304,17 @@ class DPTrainFused(torch.nn.Module):
         self._tiles = {}
 
     # ---------------------------------------------------------------- weights
+    def _flat(self):
+        """The embedding parameters of all type pairs in one vector, kind by kind (_EmbPack)."""
+        ns = self.emb
+        return torch.cat([n.W[0].reshape(-1) for n in ns] + [n.b[0] for n in ns] +
+                         [n.W[1].reshape(-1) for n in ns] + [n.b[1] for n in ns] +
+                         [n.W[2].reshape(-1) for n in ns] + [n.b[2] for n in ns])
+
     def _stacked(self):
         """The embedding parameters of all type pairs stacked: W1, b1, b2, b3 [P, E], W2, W3 [P, E, E]
-        (returned as W1, b1, W2, b2, W3, b3)."""
-        W1 = torch.stack([n.W[0].reshape(-1) for n in self.emb])
-        b1 = torch.stack([n.b[0] for n in self.emb])
-        W2 = torch.stack([n.W[1] for n in self.emb])
-        b2 = torch.stack([n.b[1] for n in self.emb])
-        W3 = torch.stack([n.W[2] for n in self.emb])
-        b3 = torch.stack([n.b[2] for n in self.emb])
-        return W1, b1, W2, b2, W3, b3
+        (returned as W1, b1, W2, b2, W3, b3; views of one concatenation)."""
+        return self._pk.stacked(self._flat())
 
     def _gpad(self, W1, b1, W2, b2, W3, b3):
         """G(s_pad) of every pair [P, E] (the embedding layer rules: plain first layer, residual
@@ -291,25 +327,11 @@ class DPTrainFused(torch.nn.Module):
 
     def _pack(self):
         """(vec, mat) from the current parameters, and G at each pair's empty-slot input."""
-        cfg, T, pk = self.cfg, self.cfg.ntypes, self._pk
-        K = TANH_K if self.bf16 else 1.0
-        P = T * T
-        vec = torch.zeros((P, NVEC, 2, 16), dtype=torch.float32, device=self.tinfo.device)
-        mat = torch.zeros((P, NMAT, 64, 16), dtype=torch.float32, device=self.tinfo.device)
+        pk = self._pk
         with torch.no_grad():
-            W1, b1, W2, b2, W3, b3 = self._stacked()
-            vec[:, V_W1] = pk.vec(W1) * K
-            vec[:, V_B1] = pk.vec(b1) * K
-            vec[:, V_B2] = pk.vec(b2) * K
-            vec[:, V_B3] = pk.vec(b3) * K
-            vec[:, V_R2] = pk.vec(torch.ones_like(b2))
-            vec[:, V_R3] = pk.vec(torch.ones_like(b3))
-            vec[:, V_GPAD] = pk.vec(self._gpad(W1, b1, W2, b2, W3, b3))
-            mat[:, M_W2F] = pk.fwd(W2) * K
-            mat[:, M_W3F] = pk.fwd(W3) * K
-            mat[:, M_W2B] = pk.bwd(W2)
-            mat[:, M_W3B] = pk.bwd(W3)
-        return vec.contiguous(), (mat.to(torch.bfloat16) if self.bf16 else mat).contiguous()
+            flat = self._flat()
+            vec, mat = pk.pack(flat, self._gpad(*pk.stacked(flat)))
+        return vec, (mat.to(torch.bfloat16) if self.bf16 else mat)
 
     # ---------------------------------------------------------------- fitting
     def _energies(self, A: torch.Tensor, b: FusedBatch):
