@@ -542,7 +542,8 @@ class DPTrainFused(torch.nn.Module):
         _lib.call("matpl_dp_fitting_perm", _ptr(A), _ptr(tiles), _ptr(perm), tiles.shape[0], _ptr(w0f), _ptr(w0b),
                   _ptr(whf), _ptr(whb), _ptr(fvec), *strides, E, self.L, int(self.bf16), _ptr(Ei), _ptr(abar),
                   _stream())
-        af = b.atom_frame.to(torch.int64)
+        k = self._batch_const(b)
+        af = k["af"]
         Ef = torch.zeros(b.nframes, dtype=torch.float64, device=dev).index_add(0, af, Ei.double())
         # 3. forces
         F = torch.empty((N, 3), dtype=torch.float32, device=dev)
@@ -581,20 +582,35 @@ class DPTrainFused(torch.nn.Module):
             b._lists_checked = True
         return loss, gemb + gfit
 
+    def _batch_const(self, b: FusedBatch):
+        """Per-batch arrays the step reuses (a training batch's atoms and lists do not change): the
+        frame index as int64, each atom's listed types and 1 / (m * their number), Ri_pad of each
+        atom's type and the type-grouped atom order; kept on the batch for this model."""
+        got = getattr(b, "_step_const", None)
+        if got is not None and got[0] is self:
+            return got[1]
+        m = self.cfg.m
+        af = b.atom_frame.to(torch.int64)
+        listed = b.type_present[af].to(torch.float32)
+        k = {"af": af, "listed": listed, "c": 1.0 / (m * listed.sum(1)),
+             "rp": self.ri_pad[b.type_map.to(torch.int64)][:, :, None],
+             "perm": self._fit_tiles(b)[0].to(torch.int64)}
+        b._step_const = (self, k)
+        return k
+
     def _emb_grads(self, grad, b: FusedBatch, ahat, nin):
         """Per-parameter embedding gradients: the kernel's sums plus the empty-slot term."""
         cfg, T, E, m = self.cfg, self.cfg.ntypes, self.E, self.cfg.m
         # empty slots: u[p] = sum_i c_i listed(i, tj) (m - n_ij) ahat_i^T Ri_pad(ti)
-        listed = b.type_present[b.atom_frame.to(torch.int64)].to(torch.float32)         # [N, T]
-        c = 1.0 / (m * listed.sum(1))                                                   # [N]
+        k = self._batch_const(b)
+        listed, c = k["listed"], k["c"]                                                # [N, T], [N]
         npad = (m - nin.to(torch.float32)) * listed                                     # [N, T]
-        ti = b.type_map.to(torch.int64)
-        proj = (ahat * self.ri_pad[ti][:, :, None]).sum(1)                                 # [N, E]
+        proj = (ahat * k["rp"]).sum(1)                                                  # [N, E]
         wsum = (c[:, None] * npad)[:, :, None] * proj[:, None, :]                        # [N, T, E]
         # per centre type sums over its atoms (one gather into type order, then contiguous sums; an
         # index_add onto T rows would serialize on atomics)
-        perm, _, ranges = self._fit_tiles(b)
-        wp = wsum[perm.to(torch.int64)]
+        _, _, ranges = self._fit_tiles(b)
+        wp = wsum[k["perm"]]
         u = torch.stack([wp[a:e].sum(0) if e > a else torch.zeros((T, E), dtype=wsum.dtype, device=wsum.device)
                          for a, e in ranges])
         has_atoms = [bool(idx.numel()) for idx in b.by_type]
