@@ -407,7 +407,7 @@ class DPTrainFused(torch.nn.Module):
         _lib.call("matpl_dp_fit_train", _ptr(A), _ptr(dA), _ptr(gE), _ptr(perm), _ptr(tiles), tiles.shape[0],
                   _ptr(w0f), _ptr(w0b), _ptr(whf), _ptr(whb), _ptr(vec), *strides, E, L, int(self.bf16), _ptr(ahat),
                   _ptr(xw0), _ptr(zw0), _ptr(xh), _ptr(zh), _ptr(rb), _ptr(xo), _ptr(tz), _stream())
-        gE_pos = gE[perm.to(torch.int64)]
+        gE_pos = gE[self._batch_const(b)["perm"]]
         # weight gradients: per type the products over its atoms' (value, tangent) rows, all types
         # in one launch per layer (matpl_tall_xtz)
         T = len(self.fit)
