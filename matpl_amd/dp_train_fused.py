@@ -587,7 +587,8 @@ class DPTrainFused(torch.nn.Module):
         frame index as int64, each atom's listed types and 1 / (m * their number), Ri_pad of each
         atom's type and the type-grouped atom order; kept on the batch for this model."""
         got = getattr(b, "_step_const", None)
-        if got is not None and got[0] is self:
+        key = (self, self.ri_pad._version)         # an in-place change of Ri_pad (load_state_dict) renews it
+        if got is not None and got[0][0] is self and got[0][1] == key[1]:
             return got[1]
         m = self.cfg.m
         af = b.atom_frame.to(torch.int64)
@@ -595,7 +596,7 @@ class DPTrainFused(torch.nn.Module):
         k = {"af": af, "listed": listed, "c": 1.0 / (m * listed.sum(1)),
              "rp": self.ri_pad[b.type_map.to(torch.int64)][:, :, None],
              "perm": self._fit_tiles(b)[0].to(torch.int64)}
-        b._step_const = (self, k)
+        b._step_const = (key, k)
         return k
 
     def _emb_grads(self, grad, b: FusedBatch, ahat, nin):
