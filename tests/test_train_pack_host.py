@@ -43,3 +43,27 @@ def test_fitting_weight_packing_equals_host_packing():
                 assert a.shape == b.shape and torch.equal(a.float(), b.float())
             assert torch.allclose(vec, ref.vec, rtol=0, atol=1e-7)
             assert tuple(strides) == tuple(ref.strides)
+
+
+def test_batch_constants_follow_ri_pad():
+    """The step's per-batch constants (kept on the batch) are reused while the model's Ri_pad is
+    unchanged and renewed after load_state_dict changes it in place."""
+    from types import SimpleNamespace
+    import numpy as np
+    import torch
+    from matpl_amd.dp_train_fused import DPTrainFused
+    from matpl_amd.dpnet import DPConfig, DPWeights
+    cfg = DPConfig(ntypes=2, m=60)
+    w = DPWeights.random(cfg, seed=3, davg=np.tile([0.1, 0, 0, 0], (2, 1)), dstd=np.tile([0.2, 0.1, 0.1, 0.1], (2, 1)))
+    model = DPTrainFused(w, "fp32", "cpu")
+    model._fit_tiles = lambda b: (torch.arange(3, dtype=torch.int32), None, None)
+    b = SimpleNamespace(atom_frame=torch.zeros(3, dtype=torch.int32), type_present=torch.ones(1, 2, dtype=torch.bool),
+                        type_map=torch.tensor([0, 1, 1], dtype=torch.int32))
+    k1 = model._batch_const(b)
+    assert model._batch_const(b) is k1
+    assert torch.equal(k1["rp"][:, :, 0], model.ri_pad[b.type_map.long()])
+    sd = model.state_dict()
+    sd["ri_pad"] = sd["ri_pad"] * 2
+    model.load_state_dict(sd)
+    k2 = model._batch_const(b)
+    assert k2 is not k1 and torch.equal(k2["rp"], 2 * k1["rp"])
