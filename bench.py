@@ -170,7 +170,7 @@ def nep_leg(steps=20):
         model = mn.NEPTorch(w, dt, dev)
 
         def step():
-            nb = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, 2, maxr, maxa)
+            nb = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, 2, maxr, maxa, ri_dtype=dt)
             return model(nb["NN_r"], nb["NL_r"], nb["Ri_r"], nb["NN_a"], nb["NL_a"], nb["Ri_a"], gp["num_atoms"],
                          gp["atom_type_map"], create_graph=False)
         for _ in range(5):
@@ -232,7 +232,7 @@ def nep_train_leg(steps=3):
         params = [q for q in model.parameters() if q.requires_grad]
 
         def step():
-            nb = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes)
+            nb = mn.neighbors(gp, cfg.rcut_radial, cfg.rcut_angular, cfg.ntypes, ri_dtype=dt)
             E, Ei, F, V = model(nb["NN_r"], nb["NL_r"], nb["Ri_r"], nb["NN_a"], nb["NL_a"], nb["Ri_a"],
                                 gp["num_atoms"], gp["atom_type_map"], create_graph=True)
             loss = (E ** 2).sum() + (F ** 2).sum() + (V ** 2).sum()

@@ -478,6 +478,19 @@ int matpl_nep_neighbor(const double *box, const double *box_orig, const int64_t 
                        double *Ri_a, int32_t *status, void *stream);
 
 /*
+ * matpl_nep_neighbor's lists (mode 0) with float Ri rows: each value is the
+ * FP64 one rounded once, so the rows equal a float cast of matpl_nep_neighbor's
+ * (the reference's NEP model computes in Ri's dtype, src/model/nep_net.py:
+ * 331-340, so a float model gets float rows; here the cast happens in the
+ * search kernel). Same arguments as mode 0 otherwise.
+ */
+int matpl_nep_neighbor_ri32(const double *box, const double *box_orig, const int64_t *num_cell,
+                            const int64_t *frame_start, const int32_t *atom_frame, const double *pos, double rc_r,
+                            double rc_a, int64_t maxr, int64_t maxa, int64_t n, int rij4, int64_t *NN_r,
+                            int64_t *NL_r, int64_t *NN_a, int64_t *NL_a, float *Ri_r, float *Ri_a, int32_t *status,
+                            void *stream);
+
+/*
  * NEP features of one family and their derivatives; f64 selects double,
  * else float. angular = 0: radial features q[n] = sum_j sum_k c[ti][tj][n][k]
  * f_k(r_ij); angular = 1: angular features from s_nlm sums (lmax l3 of the

@@ -74,6 +74,7 @@ SIGNATURES = {
     "matpl_dp_compress_grad_f64": (I32, [P, P, P, I64, I32, I32, P, P]),
     # NEP operators
     "matpl_nep_neighbor": (I32, [P] * 7 + [F64, F64, I64, I64, I64, I32, I32, I32] + [P] * 8),
+    "matpl_nep_neighbor_ri32": (I32, [P] * 6 + [F64, F64, I64, I64, I64, I32] + [P] * 8),
     "matpl_nep_feat": (I32, [I32, I32, I32, P, P, P, P, I64] + [I32] * 9 + [F64] + [P] * 8),
     "matpl_nep_force": (I32, [I32, P, P, P, P, I64, I32, P]),
     "matpl_nep_virial": (I32, [I32, P, P, P, P, P, I64, I64, I32, P, P, P]),

@@ -234,12 +234,14 @@ __device__ __forceinline__ double replica_disp(const double *h, const double *bo
     return x12 * x12 + y12 * y12 + z12 * z12;
 }
 
-__device__ __forceinline__ void put_entry(int64_t *NL, double *Ri, int64_t idx, int w, int rij4, int64_t n2, double d2,
+// (R = float: the FP64 values rounded once, as a cast of the double rows would)
+template <typename R>
+__device__ __forceinline__ void put_entry(int64_t *NL, R *Ri, int64_t idx, int w, int rij4, int64_t n2, double d2,
                                           double x, double y, double z) {
     NL[idx] = n2;
-    double *o = Ri + idx * w;
-    if (rij4) { o[0] = sqrt(d2); o[1] = x; o[2] = y; o[3] = z; }
-    else { o[0] = x; o[1] = y; o[2] = z; }
+    R *o = Ri + idx * w;
+    if (rij4) { o[0] = (R)sqrt(d2); o[1] = (R)x; o[2] = (R)y; o[3] = (R)z; }
+    else { o[0] = (R)x; o[1] = (R)y; o[2] = (R)z; }
 }
 
 // MODE 0: lists; 1: counts; 2: per-type counts. One wavefront per atom n1, lanes over the frame's
@@ -249,13 +251,13 @@ __device__ __forceinline__ void put_entry(int64_t *NL, double *Ri, int64_t idx, 
 // writing pass. The next chunk's positions are loaded before the current chunk is tested, and the
 // per-type counts of MODE 2 are held by lane t for type t, so that the lists and counts modes keep
 // their register count low (several wavefronts per SIMD hide the load latency).
-template <int MODE>
+template <int MODE, typename R = double>
 __global__ void __launch_bounds__(256)
 nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int32_t *__restrict__ atom_frame,
                     const int64_t *__restrict__ type_map, const double *__restrict__ pos, double rc_r, double rc_a,
                     int64_t maxr, int64_t maxa, int64_t n, int ntypes, int rij4, int64_t *__restrict__ NN_r,
                     int64_t *__restrict__ NL_r, int64_t *__restrict__ NN_a, int64_t *__restrict__ NL_a,
-                    double *__restrict__ Ri_r, double *__restrict__ Ri_a, int32_t *__restrict__ status) {
+                    R *__restrict__ Ri_r, R *__restrict__ Ri_a, int32_t *__restrict__ status) {
     const int lane = threadIdx.x & (WAVE - 1);
     // the wave index is uniform: the atom, its frame and the frame's boxes stay in scalar registers
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
@@ -375,11 +377,11 @@ nep_neighbor_kernel(NepGeo g, const int64_t *__restrict__ frame_start, const int
     // pad the tails (-1 / 0)
     for (int64_t k = (baser < maxr ? baser : maxr) + lane; k < maxr; k += WAVE) {
         NL_r[n1 * maxr + k] = -1;
-        for (int q = 0; q < w; q++) Ri_r[(n1 * maxr + k) * w + q] = 0.0;
+        for (int q = 0; q < w; q++) Ri_r[(n1 * maxr + k) * w + q] = R(0);
     }
     for (int64_t k = (basea < maxa ? basea : maxa) + lane; k < maxa; k += WAVE) {
         NL_a[n1 * maxa + k] = -1;
-        for (int q = 0; q < w; q++) Ri_a[(n1 * maxa + k) * w + q] = 0.0;
+        for (int q = 0; q < w; q++) Ri_a[(n1 * maxa + k) * w + q] = R(0);
     }
 }
 
@@ -1580,6 +1582,25 @@ MATPL_EXPORT int matpl_nep_neighbor(const double *box, const double *box_orig, c
     else
         hipLaunchKernelGGL(nep_neighbor_kernel<2>, grid, block, 0, st, g, frame_start, atom_frame, type_map, pos, rc_r,
                            rc_a, maxr, maxa, n, ntypes, rij4, NN_r, NL_r, NN_a, NL_a, Ri_r, Ri_a, status);
+    return check_launch("nep_neighbor_kernel");
+}
+
+// matpl_nep_neighbor's lists (mode 0) with the [r, x, y, z] rows in float: each value the FP64 one
+// rounded once, as casting matpl_nep_neighbor's double rows would give (for float models).
+MATPL_EXPORT int matpl_nep_neighbor_ri32(const double *box, const double *box_orig, const int64_t *num_cell,
+                                         const int64_t *frame_start, const int32_t *atom_frame, const double *pos,
+                                         double rc_r, double rc_a, int64_t maxr, int64_t maxa, int64_t n, int rij4,
+                                         int64_t *NN_r, int64_t *NL_r, int64_t *NN_a, int64_t *NL_a, float *Ri_r,
+                                         float *Ri_a, int32_t *status, void *stream) {
+    MATPL_CHECK_ARG(n >= 0, "nep_neighbor_ri32: bad size");
+    if (n == 0) return OK;
+    MATPL_CHECK_ARG(box && box_orig && num_cell && frame_start && atom_frame && pos && NN_r && NN_a && NL_r && NL_a &&
+                        Ri_r && Ri_a && status,
+                    "nep_neighbor_ri32: null pointer");
+    NepGeo g{box, box_orig, num_cell};
+    hipLaunchKernelGGL((nep_neighbor_kernel<0, float>), dim3(ceil_div(n, 4)), dim3(256), 0, (hipStream_t)stream, g,
+                       frame_start, atom_frame, nullptr, pos, rc_r, rc_a, maxr, maxa, n, 0, rij4, NN_r, NL_r, NN_a,
+                       NL_a, Ri_r, Ri_a, status);
     return check_launch("nep_neighbor_kernel");
 }
 

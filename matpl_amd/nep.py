@@ -84,9 +84,10 @@ def pack_frames(frames, cutoff_radial, device="cuda"):
                 atom_type_map=t(np.concatenate([f.types for f in frames]), torch.int64))
 
 
-def neighbors(packed, rc_r, rc_a, ntypes, max_r=None, max_a=None):
+def neighbors(packed, rc_r, rc_a, ntypes, max_r=None, max_a=None, ri_dtype=None):
     """calculate_maxneigh then calculate_neighbor with Rij (as the reference's NEP data path,
-    nep_data_loader.py / nep_net.py), lists sized by the largest count unless given.
+    nep_data_loader.py / nep_net.py), lists sized by the largest count unless given. ri_dtype:
+    torch.float32 gives the Ri rows in float (a float model's input) straight from the search.
 
     The list widths found by calculate_maxneigh are kept in ``packed`` (per cutoff pair): a later
     call on the same batch (a training loop revisiting it, or MD between rebuilds of the batch)
@@ -100,7 +101,7 @@ def neighbors(packed, rc_r, rc_a, ntypes, max_r=None, max_a=None):
         # (inside nep_ops.static_frames overflow is recorded, not raised: no reuse there)
         if widths is not None and nep_ops._STATIC is None:
             try:
-                return _build_lists(p, rc_r, rc_a, max_r or widths[0], max_a or widths[1])
+                return _build_lists(p, rc_r, rc_a, max_r or widths[0], max_a or widths[1], ri_dtype)
             except OverflowError:
                 pass
         NNr, NNa = nep_ops.calculate_maxneigh(p["num_atoms"], p["box"], p["box_orig"], p["num_cell"], p["position"],
@@ -110,13 +111,13 @@ def neighbors(packed, rc_r, rc_a, ntypes, max_r=None, max_a=None):
         if max_r is None and max_a is None:
             p[key] = (wr, wa)
         max_r, max_a = max_r or wr, max_a or wa
-    return _build_lists(p, rc_r, rc_a, max_r, max_a)
+    return _build_lists(p, rc_r, rc_a, max_r, max_a, ri_dtype)
 
 
-def _build_lists(p, rc_r, rc_a, max_r, max_a):
+def _build_lists(p, rc_r, rc_a, max_r, max_a, ri_dtype=None):
     NN_r, NN_a, NL_r, NL_a, Ri_r, Ri_a = nep_ops.calculate_neighbor(
         p["num_atoms"], p["atom_type_map"], p["atom_type_map"], p["box"], p["box_orig"], p["num_cell"], p["position"],
-        rc_r, rc_a, max_r, max_a, True)
+        rc_r, rc_a, max_r, max_a, True, ri_dtype=ri_dtype)
     return dict(NN_r=NN_r, NN_a=NN_a, NL_r=NL_r, NL_a=NL_a, Ri_r=Ri_r, Ri_a=Ri_a)
 
 
@@ -369,7 +370,8 @@ class NEPGraph:
 
     def _run(self):
         p = self.p
-        nb = neighbors(p, *self.args)
+        # a float model takes float rows straight from the search
+        nb = neighbors(p, *self.args, ri_dtype=torch.float32 if self.model.dtype == torch.float32 else None)
         return self.model(nb["NN_r"], nb["NL_r"], nb["Ri_r"], nb["NN_a"], nb["NL_a"], nb["Ri_a"], p["num_atoms"],
                           p["atom_type_map"], create_graph=False, type_index=self.type_index)
 

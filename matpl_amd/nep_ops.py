@@ -445,26 +445,39 @@ def calculate_maxneigh(num_atoms, box, box_orig, num_cell, position, cutoff_2b, 
 
 
 def calculate_neighbor(num_atoms, atom_type_map, atom_types, box, box_orig, num_cell, position, cutoff_2b, cutoff_3b,
-                       max_NN_radial, max_NN_angular, with_rij=False):
-    """[NN_radial, NN_angular, NL_radial, NL_angular, Ri_radial, Ri_angular] (CalcOps.cpp:1089-1172)."""
+                       max_NN_radial, max_NN_angular, with_rij=False, ri_dtype=None):
+    """[NN_radial, NN_angular, NL_radial, NL_angular, Ri_radial, Ri_angular] (CalcOps.cpp:1089-1172).
+    ri_dtype=torch.float32 (not in the reference's signature): the Ri rows in float, rounded once from
+    the FP64 values in the search kernel (equal to casting the default rows)."""
     start, af, box, bo, nc, pos = _nep_geo(num_atoms, box, box_orig, num_cell, position)
     N = pos.shape[0]
     dev = pos.device
     w = 4 if with_rij else 3
+    ri32 = ri_dtype == torch.float32
+    if ri_dtype is not None and ri_dtype not in (torch.float32, torch.float64):
+        raise ValueError("calculate_neighbor: ri_dtype must be float32 or float64")
     NN_r = torch.empty(N, dtype=torch.int64, device=dev)
     NN_a = torch.empty(N, dtype=torch.int64, device=dev)
     NL_r = torch.empty((N, max_NN_radial), dtype=torch.int64, device=dev)
     NL_a = torch.empty((N, max_NN_angular), dtype=torch.int64, device=dev)
-    Ri_r = torch.empty((N, max_NN_radial, w), dtype=torch.float64, device=dev)
-    Ri_a = torch.empty((N, max_NN_angular, w), dtype=torch.float64, device=dev)
+    rdt = torch.float32 if ri32 else torch.float64
+    Ri_r = torch.empty((N, max_NN_radial, w), dtype=rdt, device=dev)
+    Ri_a = torch.empty((N, max_NN_angular, w), dtype=rdt, device=dev)
     status = _STATIC.status if _STATIC is not None else torch.zeros(1, dtype=torch.int32, device=dev)
-    _lib.call("matpl_nep_neighbor", _ptr(box), _ptr(bo), _ptr(nc), _ptr(start), _ptr(af), None, _ptr(pos),
-              float(cutoff_2b), float(cutoff_3b), int(max_NN_radial), int(max_NN_angular), N, 0, 0, int(with_rij),
-              _ptr(NN_r), _ptr(NL_r), _ptr(NN_a), _ptr(NL_a), _ptr(Ri_r), _ptr(Ri_a), _ptr(status), _stream())
+    if ri32:
+        _lib.call("matpl_nep_neighbor_ri32", _ptr(box), _ptr(bo), _ptr(nc), _ptr(start), _ptr(af), _ptr(pos),
+                  float(cutoff_2b), float(cutoff_3b), int(max_NN_radial), int(max_NN_angular), N, int(with_rij),
+                  _ptr(NN_r), _ptr(NL_r), _ptr(NN_a), _ptr(NL_a), _ptr(Ri_r), _ptr(Ri_a), _ptr(status), _stream())
+    else:
+        _lib.call("matpl_nep_neighbor", _ptr(box), _ptr(bo), _ptr(nc), _ptr(start), _ptr(af), None, _ptr(pos),
+                  float(cutoff_2b), float(cutoff_3b), int(max_NN_radial), int(max_NN_angular), N, 0, 0,
+                  int(with_rij), _ptr(NN_r), _ptr(NL_r), _ptr(NN_a), _ptr(NL_a), _ptr(Ri_r), _ptr(Ri_a), _ptr(status),
+                  _stream())
     s = int(status.item()) if _STATIC is None else 0
     if s:
         raise OverflowError(f"calculate_neighbor: more than max_NN neighbors (atom {s - 1})")
-    it, dt = num_atoms.dtype, position.dtype
+    it = num_atoms.dtype
+    dt = ri_dtype if ri_dtype is not None else position.dtype
     out = [NN_r.to(it), NN_a.to(it), NL_r.to(it), NL_a.to(it), Ri_r.to(dt), Ri_a.to(dt)]
     # the lists address atoms of this batch only (-1 marks an empty slot): record it for the ops' checks
     for NL in out[2:4]:

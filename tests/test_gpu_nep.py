@@ -64,6 +64,21 @@ def test_neighbor_bit_exact_larger_frames(dev):
         assert np.array_equal(t.cpu().numpy(), want[name]), name
 
 
+def test_neighbor_float_rows_equal_cast(dev):
+    """calculate_neighbor(ri_dtype=float32): the same lists and counts, and Ri rows equal to the
+    float64 rows cast to float (each value rounded once), for one-replica and replicated frames."""
+    frames = _frames() + [st.zincblende_sic((4, 4, 4), seed=6)]
+    gp = mn.pack_frames(frames, 6.0, dev)
+    args = (gp["num_atoms"], gp["atom_type_map"], gp["atom_type_map"], gp["box"], gp["box_orig"], gp["num_cell"],
+            gp["position"], 6.0, 4.0, 200, 60, True)
+    d = nep_ops.calculate_neighbor(*args)
+    f = nep_ops.calculate_neighbor(*args, ri_dtype=torch.float32)
+    for k in range(4):
+        assert torch.equal(d[k], f[k])
+    for k in (4, 5):
+        assert f[k].dtype == torch.float32 and torch.equal(f[k], d[k].float())
+
+
 def test_neighbor_overflow_raises(dev):
     gp = mn.pack_frames([st.zincblende_sic((2, 2, 2), seed=4)], 6.0, dev)
     with pytest.raises(OverflowError):
