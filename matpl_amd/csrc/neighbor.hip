@@ -391,14 +391,23 @@ cell_kernel(Lattice lat, Grid g, const double *__restrict__ fw, const int32_t *_
 // exact negation of dR_ij in the cutoff test; a missing partner (per-type
 // cutoffs that differ, or a tie in the nearest-image rule) sets status.
 //
-// One wavefront per atom i. Its padding entries get -1 with plain stores; its
-// used entries (all types, in order) go to the lanes 64 at a time, each
-// binary-searching j's bucket (sorted by (j, image)) for i. Every load of the
-// search touches a different cache line per lane, so the kernel is bound by
-// the number of such loads: a two-pass scan (every 16th entry, then 15) took
-// twice as long as the seven or eight probes of the binary search (807 against
-// 386 us on a 99,636-atom list of 160 slots per type).
+// Each pair of entries is matched once, by its owner: the entry of the atom with the smaller
+// index (for an atom's own images, the entry whose packed shift is the larger of the two). A
+// first pass sets every slot to -1; the owner then binary-searches j's bucket (sorted by
+// (j, image)) for its partner and writes both directions. Every load of the search touches a
+// different cache line per lane, so the kernel is bound by the number of such loads, which the
+// owner rule halves (a two-pass scan, every 16th entry then 15, took twice as long as the seven
+// or eight probes of a binary search: 807 against 386 us on a 99,636-atom list of 160 slots per
+// type, when every entry searched). An entry whose owner-side partner is missing sets status
+// directly; one missing on the other side stays -1, which reverse_check_kernel (a pass over the
+// used entries) turns into status.
 constexpr int REV_WAVES = 4;
+
+__global__ void __launch_bounds__(256)
+reverse_init_kernel(int32_t *__restrict__ rev, int64_t nslots) {
+    for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * blockDim.x)
+        rev[s] = -1;
+}
 
 __global__ void __launch_bounds__(REV_WAVES * WAVE)
 reverse_kernel(const int32_t *__restrict__ list, const int32_t *__restrict__ shift,
@@ -412,38 +421,54 @@ reverse_kernel(const int32_t *__restrict__ list, const int32_t *__restrict__ shi
         const int ti = type_map[i];
         const int target = (int)(i - fb) + 1;
         int total = 0;
-        for (int t = 0; t < T; t++) {
-            const int c = counts[i * T + t];
-            const int64_t row = (i * T + t) * (int64_t)m;
-            for (int k = c + lane; k < m; k += WAVE) rev[row + k] = -1;
-            total += c;
-        }
+        for (int t = 0; t < T; t++) total += counts[i * T + t];
         for (int e0 = 0; e0 < total; e0 += WAVE) {
             const int e = e0 + lane;
-            if (e >= total) break;
-            int t = 0, k = e;
-            while (k >= counts[i * T + t]) k -= counts[i * T + t++];
-            const int64_t s = (i * T + t) * (int64_t)m + k;
-            const int64_t j = fb + list[s] - 1;
-            const int sh = shift[s];
-            const int want = pack_shift(-((sh & 255) - 128), -(((sh >> 8) & 255) - 128), -(((sh >> 16) & 255) - 128));
-            const int64_t base = (j * T + ti) * (int64_t)m;
-            const int cnt = counts[j * T + ti];
-            int lo = 0, hi = cnt;          // first entry naming an atom >= i (buckets are sorted)
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (list[base + mid] < target) lo = mid + 1; else hi = mid;
-            }
-            int found = -1;
-            for (int q = lo; q < cnt && list[base + q] == target; q++)
-                if (shift[base + q] == want) { found = q; break; }
-            if (found < 0) {
-                rev[s] = -1;
-                atomicCAS(status, 0, (int)(i + 1));
-            } else {
-                rev[s] = (int32_t)(base + found);
+            if (e < total) {
+                int t = 0, k = e;
+                while (k >= counts[i * T + t]) k -= counts[i * T + t++];
+                const int64_t s = (i * T + t) * (int64_t)m + k;
+                const int64_t j = fb + list[s] - 1;
+                const int sh = shift[s];
+                const int want = pack_shift(-((sh & 255) - 128), -(((sh >> 8) & 255) - 128),
+                                            -(((sh >> 16) & 255) - 128));
+                if (i < j || (i == j && sh > want)) {
+                    const int64_t base = (j * T + ti) * (int64_t)m;
+                    const int cnt = counts[j * T + ti];
+                    int lo = 0, hi = cnt;          // first entry naming an atom >= i (buckets are sorted)
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (list[base + mid] < target) lo = mid + 1; else hi = mid;
+                    }
+                    int found = -1;
+                    for (int q = lo; q < cnt && list[base + q] == target; q++)
+                        if (shift[base + q] == want) { found = q; break; }
+                    if (found < 0) {
+                        atomicCAS(status, 0, (int)(i + 1));
+                    } else {
+                        rev[s] = (int32_t)(base + found);
+                        rev[base + found] = (int32_t)s;
+                    }
+                }
             }
         }
+    }
+}
+
+// status = 1 + i for the first atom i with a used entry left without its partner (the owner's list
+// lacks it); one wavefront per atom over its used entries
+__global__ void __launch_bounds__(REV_WAVES * WAVE)
+reverse_check_kernel(const int32_t *__restrict__ counts, const int32_t *__restrict__ rev, int T, int m, int64_t n,
+                     int32_t *__restrict__ status) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int64_t i = (int64_t)blockIdx.x * REV_WAVES + threadIdx.x / WAVE; i < n;
+         i += (int64_t)gridDim.x * REV_WAVES) {
+        bool bad = false;
+        for (int t = 0; t < T; t++) {
+            const int c = counts[i * T + t];
+            for (int k = lane; k < c; k += WAVE) bad |= rev[(i * T + t) * (int64_t)m + k] < 0;
+        }
+        if (__ballot(bad) != 0ull && lane == 0) atomicCAS(status, 0, (int)(i + 1));
     }
 }
 
@@ -691,9 +716,17 @@ MATPL_EXPORT int matpl_dp_neighbor_reverse(const int32_t *list, const int32_t *s
     MATPL_CHECK_ARG(list && shift && counts && type_map && rev && status, "dp_neighbor_reverse: null pointer");
     MATPL_CHECK_ARG((frame_start == nullptr) == (atom_frame == nullptr),
                     "dp_neighbor_reverse: frame_start and atom_frame go together");
-    hipLaunchKernelGGL(reverse_kernel, dim3(grid_cap(n, REV_WAVES)), dim3(REV_WAVES * WAVE), 0, (hipStream_t)stream,
-                       list, shift, counts, type_map, frame_start, atom_frame, ntypes, m, n, rev, status);
-    return check_launch("reverse_kernel");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t nslots = n * ntypes * (int64_t)m;
+    hipLaunchKernelGGL(reverse_init_kernel, dim3(grid_cap(nslots)), dim3(256), 0, st, rev, nslots);
+    int rc = check_launch("reverse_init_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(reverse_kernel, dim3(grid_cap(n, REV_WAVES)), dim3(REV_WAVES * WAVE), 0, st, list, shift, counts,
+                       type_map, frame_start, atom_frame, ntypes, m, n, rev, status);
+    if ((rc = check_launch("reverse_kernel"))) return rc;
+    hipLaunchKernelGGL(reverse_check_kernel, dim3(grid_cap(n, REV_WAVES)), dim3(REV_WAVES * WAVE), 0, st, counts, rev,
+                       ntypes, m, n, status);
+    return check_launch("reverse_check_kernel");
 }
 
 MATPL_EXPORT int matpl_dp_lmp_neighdata(const double *x, const int32_t *type, const int32_t *type_to_model,
